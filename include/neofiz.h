@@ -1,0 +1,191 @@
+/*
+ * neofiz.h -- C ABI of the MI355X explicit-dynamics step loop.
+ *
+ * The reference has no FFI: its boundary is the public Java API of
+ * org.neofiz.solver.ExplicitSolver and its configuration objects. Each entry point below
+ * replaces one member of that API; the citation names the Java member it stands for
+ * (paths relative to src/main/java/org/neofiz/). A Java Panama FFM shim binds these symbols
+ * one-for-one and presents the unchanged ExplicitSolver surface.
+ *
+ * Conventions
+ *  - Every call returns an int status: NF_OK, or NF_E_ARGUMENT where the Java method throws
+ *    IllegalArgumentException, NF_E_STATE where it throws IllegalStateException, NF_E_DEVICE
+ *    for a HIP failure, NF_E_MEMORY when device memory runs out. nf_last_error() returns the
+ *    message of the calling thread's last failure. No call aborts the process.
+ *  - The library owns device memory. Arrays passed in are copied at the call; getters copy
+ *    out into caller buffers (the Java accessors return live arrays; the shim re-reads them
+ *    after each run call).
+ *  - One handle is one HIP stream. Calls on one handle must be serialised by the caller;
+ *    different handles may be driven from different threads.
+ *  - Element connectivity is counter-clockwise in (r, z), 4 int32 node indices per element.
+ */
+#ifndef NEOFIZ_H
+#define NEOFIZ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NF_ABI_VERSION 1
+
+enum nf_status {
+    NF_OK = 0,
+    NF_E_ARGUMENT = -1, /* IllegalArgumentException */
+    NF_E_STATE = -2,    /* IllegalStateException */
+    NF_E_DEVICE = -3,   /* HIP runtime failure */
+    NF_E_MEMORY = -4    /* device or host allocation failed */
+};
+
+/* core/Formulation.java:17-21 */
+enum nf_formulation { NF_AXISYMMETRIC = 0, NF_PLANE_STRAIN = 1, NF_PLANE_STRESS = 2 };
+/* solver/Integration.java:30-32 */
+enum nf_integration { NF_FULL = 0, NF_REDUCED = 1 };
+/* solver/Kinematics.java:20, :35 */
+enum nf_kinematics { NF_SMALL_STRAIN = 0, NF_FINITE_STRAIN = 1 };
+
+/* core/JohnsonCook.java:68-71 -- the record's components, in order. */
+typedef struct nf_johnson_cook {
+    double a, b, n, c, m;
+    double reference_strain_rate, room_temperature, melt_temperature;
+    double specific_heat, taylor_quinney;
+} nf_johnson_cook;
+
+/* core/Material.java:17-19 -- the record's components. yield_stress may be +infinity
+ * (the elastic material, core/Material.java:54-56). */
+typedef struct nf_material {
+    double youngs_modulus, poisson_ratio, density;
+    double yield_stress, isotropic_hardening, kinematic_hardening;
+    int32_t has_johnson_cook;
+    int32_t reserved;
+    nf_johnson_cook johnson_cook;
+} nf_material;
+
+typedef struct nf_solver nf_solver;
+
+/* Message of the calling thread's last failure, or "" after a success. */
+const char* nf_last_error(void);
+
+/* Validates a material as the Material and JohnsonCook compact constructors do
+ * (core/Material.java:21-37, core/JohnsonCook.java:73-89). */
+int nf_material_check(const nf_material* material);
+
+/*
+ * new ExplicitSolver(mesh, materials, formulation, integration, kinematics, thickness, cfl)
+ * solver/ExplicitSolver.java:264-339, with new QuadMesh(r, z, conn, pressureEdges)
+ * (mesh/QuadMesh.java:37-54) and Materials.uniform/of (core/Materials.java:36-53).
+ * material_of is NULL for a uniform mesh (material_count must then be 1), else one index
+ * per element into materials.
+ */
+int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const double* z,
+                     int32_t element_count, const int32_t* conn,
+                     int32_t pressure_edge_count, const int32_t* pressure_edges,
+                     int32_t material_count, const nf_material* materials,
+                     const int32_t* material_of, int32_t formulation, int32_t integration,
+                     int32_t kinematics, double thickness, double cfl_safety);
+
+/* ExplicitSolver.shutdown() (solver/ExplicitSolver.java:395-400) plus release of the handle.
+ * NULL is accepted; calling it once per handle is the caller's duty. */
+int nf_solver_destroy(nf_solver* s);
+
+/* setThreads(n) solver/ExplicitSolver.java:386-392. n >= 1; the answer never depends on it. */
+int nf_set_threads(nf_solver* s, int32_t n);
+/* fixRadial / fixAxial / fixAllAxial :408-425 */
+int nf_fix_radial(nf_solver* s, int32_t node);
+int nf_fix_axial(nf_solver* s, int32_t node);
+int nf_fix_all_axial(nf_solver* s);
+/* setPressureRamp :432-435 */
+int nf_set_pressure_ramp(nf_solver* s, double pascals, double ramp_seconds);
+/* setRelaxationDamping :450-455 */
+int nf_set_relaxation_damping(nf_solver* s, double fraction_of_critical,
+                              double angular_frequency);
+/* setHourglassCoefficient :466-469 */
+int nf_set_hourglass_coefficient(nf_solver* s, double q);
+/* setRigidWall(RigidWall.atZ(z, mu)) :475-477, solver/RigidWall.java:109-136;
+ * enabled = 0 is setRigidWall(null). friction may be +infinity (welded). */
+int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double friction);
+/* setContact(new Contact(mesh).withFriction(mu).withDamping(zeta).withStiffnessScale(k))
+ * :487-489, solver/Contact.java:273-364; enabled = 0 is setContact(null). */
+int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double damping,
+                   double stiffness_scale);
+/* setGravity :509-512 */
+int nf_set_gravity(nf_solver* s, double acceleration_r, double acceleration_z);
+/* setDamage(double[] fractureEnergy, double[] failureStrain) :729-791, one per element. */
+int nf_set_damage(nf_solver* s, const double* fracture_energy, const double* failure_strain);
+/* setErosion :549-561 */
+int nf_set_erosion(nf_solver* s, double damage_threshold);
+/* setMassScaling :601-619 */
+int nf_set_mass_scaling(nf_solver* s, double target_step);
+/* setVelocity / setUniformVelocity :799-808 */
+int nf_set_velocity(nf_solver* s, int32_t node, double velocity_r, double velocity_z);
+int nf_set_uniform_velocity(nf_solver* s, double velocity_r, double velocity_z);
+
+/* run(steps) :936-938 */
+int nf_run(nf_solver* s, int64_t steps);
+/* while (time() < end_time) step(), validate/TaylorImpactCase.java:256-259, stopping early
+ * after max_steps (<= 0 means no limit). steps_taken may be NULL. */
+int nf_run_until(nf_solver* s, double end_time, int64_t max_steps, int64_t* steps_taken);
+/* advance() :1069-1075 */
+int nf_advance(nf_solver* s);
+
+/* Node fields, node_count doubles each. radialDisplacement :842, axialDisplacement :846,
+ * velocityR/Z :905-912, forceR/Z :915-922; mass and the reference coordinates are the
+ * solver's private arrays (:107, mesh/QuadMesh.java:21-23); ON_WALL, FIXED_R/Z are 0 or 1. */
+enum nf_node_field {
+    NF_NODE_UR = 0, NF_NODE_UZ, NF_NODE_VR, NF_NODE_VZ, NF_NODE_FR, NF_NODE_FZ,
+    NF_NODE_MASS, NF_NODE_R, NF_NODE_Z, NF_NODE_ON_WALL, NF_NODE_FIXED_R, NF_NODE_FIXED_Z,
+    NF_NODE_FIELD_COUNT
+};
+int nf_get_node_field(nf_solver* s, int32_t field, double* out);
+/* Writes UR, UZ, VR or VZ (the mutation a Java caller makes through a live array). */
+int nf_put_node_field(nf_solver* s, int32_t field, const double* in);
+
+/* Gauss-point fields: stress() :930 (4 per point {rr, zz, tt, rz}), back stress (4 per
+ * point), plasticStrain() :925, plasticWorkDensity() :1712, and the damage state
+ * (solver/Damage.java:188-192: strength, softening, band width). */
+enum nf_point_field {
+    NF_POINT_STRESS = 0, NF_POINT_BACK, NF_POINT_PLASTIC_STRAIN, NF_POINT_PLASTIC_WORK,
+    NF_POINT_DAMAGE_STRENGTH, NF_POINT_DAMAGE_SOFTENING, NF_POINT_DAMAGE_WIDTH,
+    NF_POINT_FIELD_COUNT
+};
+int nf_get_point_field(nf_solver* s, int32_t field, double* out);
+
+/* Element fields: hourglass force hgR/hgZ :134-135, stiffness :137, eroded flag isEroded :627
+ * (0 or 1). */
+enum nf_element_field {
+    NF_ELEMENT_HG_R = 0, NF_ELEMENT_HG_Z, NF_ELEMENT_HG_STIFFNESS, NF_ELEMENT_ERODED,
+    NF_ELEMENT_FIELD_COUNT
+};
+int nf_get_element_field(nf_solver* s, int32_t field, double* out);
+
+/* Scalar state: time() :834, timestep() :830, steps() :838, massScale() :622,
+ * erodedElements/erodedEnergy/invertedElements :632-652, RigidWall audits
+ * (solver/RigidWall.java:285-315) and Contact audits (solver/Contact.java:371-404). */
+typedef struct nf_state {
+    double time, dt, dt_previous, dt0, mass_scale;
+    int64_t steps;
+    int32_t node_count, element_count, points_per_element, eroded_elements;
+    int32_t inverted_elements, wall_contact_count, wall_stuck_count, contact_pairs;
+    double eroded_energy;
+    double wall_impulse, wall_energy_loss, wall_friction_dissipation;
+    double contact_energy, contact_dissipation, contact_max_penetration;
+    int64_t contact_escaped_nodes;
+} nf_state;
+int nf_get_state(nf_solver* s, nf_state* out);
+
+/* Audits, solver/ExplicitSolver.java:875-886, :1610-1860. max_temperature is NaN when no
+ * material has a Johnson-Cook law (temperature() throws there, :1834-1837). */
+typedef struct nf_audits {
+    double axial_momentum, radial_momentum, kinetic_energy, centred_kinetic_energy;
+    double strain_energy, plastic_dissipation, hourglass_energy, fracture_dissipation;
+    double max_plastic_strain, yielded_fraction, max_temperature, max_damage;
+    double damaged_fraction, failed_fraction;
+} nf_audits;
+int nf_get_audits(nf_solver* s, nf_audits* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEOFIZ_H */

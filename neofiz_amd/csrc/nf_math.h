@@ -1,0 +1,350 @@
+// Deterministic double-precision transcendentals, identical bit-for-bit on the host (g++,
+// x86-64 SSE2) and on gfx950.
+//
+// Why these exist: the Johnson-Cook return map calls pow and log inside its Newton loop
+// (core/JohnsonCook.java:139, :182, :196) and damage onset calls atan2, sin, cos and hypot
+// (solver/Damage.java:315, :321-322). Library implementations differ between glibc and the
+// AMD device library in the last bit, and a plastic run amplifies a last-bit difference over
+// tens of thousands of steps. Both the CPU restatement and the HIP kernels therefore call the
+// functions below, which use only IEEE-754 add, subtract, multiply, divide and square root
+// (each correctly rounded on both targets), integer bit manipulation and comparisons. The
+// file must be compiled with floating-point contraction off (-ffp-contract=off) on every
+// target, so that no multiply-add is fused on one side and not the other.
+//
+// Methods (all from the mathematics, no library constants):
+//  - log: x = 2^k m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1),
+//    with s carried as a double-double (Dekker's exact product by Veltkamp splitting,
+//    Knuth's two-sum) and the odd series 2 s^3 (1/3 + s^2/5 + ...) to 1/23 in double.
+//  - exp: Cody-Waite reduction by ln 2 (split constant), Taylor series of expm1 on
+//    |r| <= ln2/2 to degree 15, recombined in double-double.
+//  - pow(x, y) for x > 0: exp(y log x) with log x carried as a double-double.
+//  - atan: reduction atan t = pi/4 + atan((t-1)/(t+1)) and one half-angle step, then the
+//    Gregory series; atan2 by quadrant.
+//  - sin, cos: reduction by pi/2 (two-part constant) to |r| <= pi/4, Taylor series.
+//  - hypot: scaled sqrt(a^2 + b^2).
+// Accuracy against correctly rounded values (mpmath) is checked in the test suite.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define NF_HD __host__ __device__ __forceinline__
+#else
+#define NF_HD static inline
+#endif
+
+namespace nfm {
+
+NF_HD uint64_t bits_of(double x) {
+    uint64_t u;
+    __builtin_memcpy(&u, &x, sizeof u);
+    return u;
+}
+
+NF_HD double from_bits(uint64_t u) {
+    double x;
+    __builtin_memcpy(&x, &u, sizeof x);
+    return x;
+}
+
+NF_HD double inf() { return from_bits(0x7FF0000000000000ULL); }
+NF_HD double qnan() { return from_bits(0x7FF8000000000000ULL); }
+NF_HD bool is_nan(double x) { return x != x; }
+
+// Error-free transforms. Contraction must be off for these to be exact.
+NF_HD void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+NF_HD void fast_two_sum(double a, double b, double& s, double& e) {  // requires |a| >= |b|
+    s = a + b;
+    e = b - (s - a);
+}
+
+NF_HD void split(double a, double& hi, double& lo) {
+    const double t = 134217729.0 * a;  // 2^27 + 1
+    hi = t - (t - a);
+    lo = a - hi;
+}
+
+NF_HD void two_prod(double a, double b, double& p, double& e) {
+    p = a * b;
+    double ah, al, bh, bl;
+    split(a, ah, al);
+    split(b, bh, bl);
+    e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
+}
+
+// ln 2 split so that k * LN2_HI is exact for |k| < 2^11 (LN2_HI has 32 significant bits).
+#define NFM_LN2_HI 0x1.62e42fefa0000p-1
+#define NFM_LN2_LO 0x1.cf79abc9e3b3ap-40  // ln2 - LN2_HI, rounded
+
+// log(x) as an unevaluated sum hi + lo (|lo| <= ulp(hi)). x must be positive and finite.
+NF_HD void log_dd(double x, double& hi, double& lo) {
+    uint64_t b = bits_of(x);
+    int k = 0;
+    if (b < 0x0010000000000000ULL) {  // subnormal
+        x = x * 0x1p54;
+        b = bits_of(x);
+        k = -54;
+    }
+    k += (int)(b >> 52) - 1023;
+    double m = from_bits((b & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        k += 1;
+    }
+    const double f = m - 1.0;  // exact (Sterbenz)
+
+    // s = f / (2 + f) as a double-double.
+    double d_hi, d_lo;
+    fast_two_sum(2.0, f, d_hi, d_lo);
+    const double s_hi = f / d_hi;
+    double p_hi, p_lo;
+    two_prod(s_hi, d_hi, p_hi, p_lo);
+    const double s_lo = (((f - p_hi) - p_lo) - s_hi * d_lo) / d_hi;
+
+    // Tail 2 s^3 (1/3 + z/5 + ... + z^10/23), z = s^2.
+    const double z = s_hi * s_hi;
+    double t = 1.0 / 23.0;
+    t = 1.0 / 21.0 + z * t;
+    t = 1.0 / 19.0 + z * t;
+    t = 1.0 / 17.0 + z * t;
+    t = 1.0 / 15.0 + z * t;
+    t = 1.0 / 13.0 + z * t;
+    t = 1.0 / 11.0 + z * t;
+    t = 1.0 / 9.0 + z * t;
+    t = 1.0 / 7.0 + z * t;
+    t = 1.0 / 5.0 + z * t;
+    t = 1.0 / 3.0 + z * t;
+    const double tail = 2.0 * (s_hi * z) * t;
+
+    // log m = 2 s_hi + (2 s_lo + tail), then add k ln2.
+    const double lm_hi = 2.0 * s_hi;
+    const double lm_lo = 2.0 * s_lo + tail;
+    const double kd = (double)k;
+    double h, e;
+    two_sum(kd * NFM_LN2_HI, lm_hi, h, e);
+    e = e + (lm_lo + kd * NFM_LN2_LO);
+    fast_two_sum(h, e, hi, lo);
+}
+
+NF_HD double log(double x) {
+    if (!(x > 0.0)) return x == 0.0 ? -inf() : qnan();
+    if (x == inf()) return x;
+    double hi, lo;
+    log_dd(x, hi, lo);
+    return hi + lo;
+}
+
+// exp(a + b) with |b| <= ulp(a).
+NF_HD double exp_dd(double a, double b) {
+    if (is_nan(a)) return a;
+    if (a > 709.782712893384) return inf();
+    if (a < -745.2) return 0.0;
+    // k = round(a / ln2), by the 1.5 * 2^52 shift (round to nearest even, exact here).
+    const double shifter = 0x1.8p52;
+    const double kd = (a * 1.4426950408889634 + shifter) - shifter;
+    const int k = (int)kd;
+    // r = a - k ln2 + b, as a double-double.
+    double r_hi, r_lo;
+    two_sum(a - kd * NFM_LN2_HI, b - kd * NFM_LN2_LO, r_hi, r_lo);
+
+    // expm1(r_hi) = r + r^2 (1/2! + r/3! + ... + r^13/15!).
+    const double r = r_hi;
+    double q = 1.0 / 1307674368000.0;   // 1/15!
+    q = 1.0 / 87178291200.0 + r * q;    // 1/14!
+    q = 1.0 / 6227020800.0 + r * q;     // 1/13!
+    q = 1.0 / 479001600.0 + r * q;      // 1/12!
+    q = 1.0 / 39916800.0 + r * q;       // 1/11!
+    q = 1.0 / 3628800.0 + r * q;        // 1/10!
+    q = 1.0 / 362880.0 + r * q;         // 1/9!
+    q = 1.0 / 40320.0 + r * q;          // 1/8!
+    q = 1.0 / 5040.0 + r * q;           // 1/7!
+    q = 1.0 / 720.0 + r * q;            // 1/6!
+    q = 1.0 / 120.0 + r * q;            // 1/5!
+    q = 1.0 / 24.0 + r * q;             // 1/4!
+    q = 1.0 / 6.0 + r * q;              // 1/3!
+    q = 0.5 + r * q;                    // 1/2!
+    double em_hi, em_lo;
+    two_sum(r, (r * r) * q, em_hi, em_lo);
+    // exp(r_hi + r_lo) = 1 + em + r_lo (1 + em)
+    double s, e;
+    fast_two_sum(1.0, em_hi, s, e);
+    const double y = s + (e + (em_lo + r_lo * (1.0 + em_hi)));
+
+    // Scale by 2^k exactly where the result is normal.
+    if (k > 1023) {
+        return (y * from_bits((uint64_t)(1023 + 1023) << 52)) *
+               from_bits((uint64_t)(k - 1023 + 1023) << 52);
+    }
+    if (k < -1021) {
+        return (y * from_bits((uint64_t)(k + 54 + 1023) << 52)) * 0x1p-54;
+    }
+    return y * from_bits((uint64_t)(k + 1023) << 52);
+}
+
+NF_HD double exp(double x) { return exp_dd(x, 0.0); }
+
+// x^y for the cases the solver needs: x > 0 finite, y finite. Special values follow IEEE pow
+// for x = 0, x = 1, y = 0, NaN; a negative base yields NaN.
+NF_HD double pow(double x, double y) {
+    if (y == 0.0) return 1.0;
+    if (is_nan(x) || is_nan(y)) return qnan();
+    if (x == 1.0) return 1.0;
+    if (x == 0.0) return y > 0.0 ? 0.0 : inf();
+    if (x < 0.0) return qnan();
+    if (x == inf()) return y > 0.0 ? inf() : 0.0;
+    double l_hi, l_lo;
+    log_dd(x, l_hi, l_lo);
+    double p_hi, p_lo;
+    two_prod(y, l_hi, p_hi, p_lo);
+    p_lo = p_lo + y * l_lo;
+    double t_hi, t_lo;
+    fast_two_sum(p_hi, p_lo, t_hi, t_lo);
+    return exp_dd(t_hi, t_lo);
+}
+
+#define NFM_PI 3.141592653589793
+#define NFM_PI_2_HI 0x1.921fb54442d18p0    // pi/2 rounded
+#define NFM_PI_2_LO 0x1.1a62633145c07p-54  // pi/2 - NFM_PI_2_HI
+#define NFM_PI_4_HI 0x1.921fb54442d18p-1
+#define NFM_PI_4_LO 0x1.1a62633145c07p-55
+
+// atan(t) for 0 <= t <= 1.
+NF_HD double atan_unit(double t) {
+    double base_hi = 0.0, base_lo = 0.0;
+    double u = t;
+    if (t > 0.41421356237309503) {  // tan(pi/8)
+        u = (t - 1.0) / (t + 1.0);
+        base_hi = NFM_PI_4_HI;
+        base_lo = NFM_PI_4_LO;
+    }
+    // Half-angle step: atan u = 2 atan(u / (1 + sqrt(1 + u^2))), |u'| <= tan(pi/16).
+    const double w = u / (1.0 + __builtin_sqrt(1.0 + u * u));
+    const double z = w * w;
+    double p = -1.0 / 27.0;
+    p = 1.0 / 25.0 + z * p;
+    p = -1.0 / 23.0 + z * p;
+    p = 1.0 / 21.0 + z * p;
+    p = -1.0 / 19.0 + z * p;
+    p = 1.0 / 17.0 + z * p;
+    p = -1.0 / 15.0 + z * p;
+    p = 1.0 / 13.0 + z * p;
+    p = -1.0 / 11.0 + z * p;
+    p = 1.0 / 9.0 + z * p;
+    p = -1.0 / 7.0 + z * p;
+    p = 1.0 / 5.0 + z * p;
+    p = -1.0 / 3.0 + z * p;
+    const double a = 2.0 * (w + (w * z) * p);
+    return base_hi + (a + base_lo);
+}
+
+NF_HD double atan2(double y, double x) {
+    if (is_nan(x) || is_nan(y)) return qnan();
+    const double ay = y < 0.0 ? -y : y;
+    const double ax = x < 0.0 ? -x : x;
+    const bool neg_y = __builtin_signbit(y) != 0;
+    if (ay == 0.0) {
+        // atan2(+-0, x): +-0 for x > 0 or x = +0, +-pi for x < 0 or x = -0.
+        const bool neg_x = __builtin_signbit(x) != 0;
+        const double r = neg_x ? NFM_PI : 0.0;
+        return neg_y ? -r : r;
+    }
+    if (ax == inf() || ay == inf()) {
+        double r;
+        if (ax == inf() && ay == inf()) r = x > 0.0 ? 0.25 * NFM_PI : 0.75 * NFM_PI;
+        else if (ax == inf()) r = x > 0.0 ? 0.0 : NFM_PI;
+        else r = 0.5 * NFM_PI;
+        return neg_y ? -r : r;
+    }
+    double a;
+    if (ay <= ax) {
+        a = atan_unit(ay / ax);
+    } else {
+        a = NFM_PI_2_HI + (NFM_PI_2_LO - atan_unit(ax / ay));
+    }
+    if (x < 0.0) a = (NFM_PI_2_HI - a) + NFM_PI_2_HI + 2.0 * NFM_PI_2_LO;
+    return neg_y ? -a : a;
+}
+
+// sin and cos of r with |r| <= pi/4 + a little.
+NF_HD double sin_kernel(double r) {
+    const double z = r * r;
+    double p = -1.0 / 121645100408832000.0;  // -1/19!
+    p = 1.0 / 355687428096000.0 + z * p;   // 1/17!
+    p = -1.0 / 1307674368000.0 + z * p;    // -1/15!
+    p = 1.0 / 6227020800.0 + z * p;        // 1/13!
+    p = -1.0 / 39916800.0 + z * p;         // -1/11!
+    p = 1.0 / 362880.0 + z * p;            // 1/9!
+    p = -1.0 / 5040.0 + z * p;             // -1/7!
+    p = 1.0 / 120.0 + z * p;               // 1/5!
+    p = -1.0 / 6.0 + z * p;                // -1/3!
+    return r + (r * z) * p;
+}
+
+NF_HD double cos_kernel(double r) {
+    const double z = r * r;
+    double p = -1.0 / 6402373705728000.0;  // -1/18!
+    p = 1.0 / 20922789888000.0 + z * p;    // 1/16!
+    p = -1.0 / 87178291200.0 + z * p;      // -1/14!
+    p = 1.0 / 479001600.0 + z * p;         // 1/12!
+    p = -1.0 / 3628800.0 + z * p;          // -1/10!
+    p = 1.0 / 40320.0 + z * p;             // 1/8!
+    p = -1.0 / 720.0 + z * p;              // -1/6!
+    p = 1.0 / 24.0 + z * p;                // 1/4!
+    // cos r = 1 - z/2 + z^2 p, summed so the leading 1 - z/2 keeps its low bits.
+    const double hz = 0.5 * z;
+    const double w = 1.0 - hz;
+    return w + (((1.0 - w) - hz) + (z * z) * p);
+}
+
+// Quadrant reduction for |x| up to a few thousand (the solver only needs |x| <= pi/2).
+NF_HD void reduce_pi_2(double x, double& r, int& q) {
+    const double shifter = 0x1.8p52;
+    const double kd = (x * 0.6366197723675814 + shifter) - shifter;  // 2/pi
+    q = (int)kd;
+    r = (x - kd * NFM_PI_2_HI) - kd * NFM_PI_2_LO;
+}
+
+NF_HD double sin(double x) {
+    if (is_nan(x) || x == inf() || x == -inf()) return qnan();
+    double r;
+    int q;
+    reduce_pi_2(x, r, q);
+    switch (q & 3) {
+        case 0: return sin_kernel(r);
+        case 1: return cos_kernel(r);
+        case 2: return -sin_kernel(r);
+        default: return -cos_kernel(r);
+    }
+}
+
+NF_HD double cos(double x) {
+    if (is_nan(x) || x == inf() || x == -inf()) return qnan();
+    double r;
+    int q;
+    reduce_pi_2(x, r, q);
+    switch (q & 3) {
+        case 0: return cos_kernel(r);
+        case 1: return -sin_kernel(r);
+        case 2: return -cos_kernel(r);
+        default: return sin_kernel(r);
+    }
+}
+
+NF_HD double hypot(double a, double b) {
+    if (a < 0.0) a = -a;
+    if (b < 0.0) b = -b;
+    if (a == inf() || b == inf()) return inf();
+    if (is_nan(a) || is_nan(b)) return qnan();
+    double big = a > b ? a : b;
+    double small = a > b ? b : a;
+    if (big == 0.0) return 0.0;
+    const double ratio = small / big;
+    return big * __builtin_sqrt(1.0 + ratio * ratio);
+}
+
+}  // namespace nfm

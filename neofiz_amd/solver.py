@@ -1,0 +1,569 @@
+"""Host-side mirror of Neofiz's solver API over the C ABI.
+
+Class and method names follow the Java reference (src/main/java/org/neofiz/...), so a caller
+of `org.neofiz.solver.ExplicitSolver` finds the same surface here:
+
+  QuadMesh            mesh/QuadMesh.java:15-403 (solidCylinder, cylinderWall, axisNodes, ...)
+  JohnsonCook         core/JohnsonCook.java:68-227 (STEEL_4340, COPPER_OFHC)
+  Material            core/Material.java:17-136 (withJohnsonCook, yielding)
+  Materials           core/Materials.java:23-129 (uniform, of, byElement)
+  RigidWall           solver/RigidWall.java:95-316 (atZ)
+  Contact             solver/Contact.java:132-364 (withFriction, withDamping, withStiffnessScale)
+  ExplicitSolver      solver/ExplicitSolver.java:38-1966
+
+Every compute call goes through the HIP library (`neofiz_amd._native`); there is no CPU path.
+Errors map as the Java ones do: IllegalArgumentException and IllegalStateException below are
+raised where the reference throws them.
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass, replace
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+
+AXISYMMETRIC, PLANE_STRAIN, PLANE_STRESS = 0, 1, 2   # core/Formulation.java:17-21
+FULL, REDUCED = 0, 1                                 # solver/Integration.java:30-32
+SMALL_STRAIN, FINITE_STRAIN = 0, 1                   # solver/Kinematics.java:20, :35
+
+
+class IllegalArgumentException(ValueError):
+    pass
+
+
+class IllegalStateException(RuntimeError):
+    pass
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+def check(binding, rc):
+    if rc == 0:
+        return
+    msg = binding.last_error().decode(errors="replace")
+    if rc == -1:
+        raise IllegalArgumentException(msg)
+    if rc == -2:
+        raise IllegalStateException(msg)
+    raise DeviceError(f"status {rc}: {msg}")
+
+
+# ---------------------------------------------------------------- materials
+
+@dataclass(frozen=True)
+class JohnsonCook:
+    """core/JohnsonCook.java:68-71. Validation happens in the C ABI (nf_material_check)."""
+    a: float
+    b: float
+    n: float
+    c: float
+    m: float
+    referenceStrainRate: float
+    roomTemperature: float
+    meltTemperature: float
+    specificHeat: float
+    taylorQuinney: float
+
+    def quasiStatic(self):  # :125-128
+        return replace(self, c=0.0, taylorQuinney=0.0)
+
+
+JohnsonCook.STEEL_4340 = JohnsonCook(792.0e6, 510.0e6, 0.26, 0.014, 1.03,
+                                     1.0, 293.0, 1793.0, 477.0, 0.9)   # :95-97
+JohnsonCook.COPPER_OFHC = JohnsonCook(90.0e6, 292.0e6, 0.31, 0.025, 1.09,
+                                      1.0, 293.0, 1356.0, 383.0, 0.9)  # :100-102
+
+
+@dataclass(frozen=True)
+class Material:
+    """core/Material.java:17-19."""
+    name: str
+    youngsModulus: float
+    poissonRatio: float
+    density: float
+    yieldStress: float = math.inf
+    isotropicHardening: float = 0.0
+    kinematicHardening: float = 0.0
+    johnsonCook: Optional[JohnsonCook] = None
+
+    def isElastic(self):
+        return math.isinf(self.yieldStress)
+
+    def yielding(self, yieldStress, isotropic, kinematic=0.0):  # :71-80
+        return replace(self, yieldStress=yieldStress, isotropicHardening=isotropic,
+                       kinematicHardening=kinematic, johnsonCook=None)
+
+    def withJohnsonCook(self, law):  # :87-90
+        return replace(self, yieldStress=law.a, isotropicHardening=0.0, johnsonCook=law)
+
+    def lambda_(self):
+        e, nu = self.youngsModulus, self.poissonRatio
+        return e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+
+    def mu(self):
+        return self.youngsModulus / (2.0 * (1.0 + self.poissonRatio))
+
+    def dilatationalWaveSpeed(self):
+        return math.sqrt((self.lambda_() + 2.0 * self.mu()) / self.density)
+
+    def to_c(self):
+        out = _abi.MaterialC()
+        out.youngs_modulus = self.youngsModulus
+        out.poisson_ratio = self.poissonRatio
+        out.density = self.density
+        out.yield_stress = self.yieldStress
+        out.isotropic_hardening = self.isotropicHardening
+        out.kinematic_hardening = self.kinematicHardening
+        if self.johnsonCook is not None:
+            j = self.johnsonCook
+            out.has_johnson_cook = 1
+            out.johnson_cook = _abi.JohnsonCookC(j.a, j.b, j.n, j.c, j.m, j.referenceStrainRate,
+                                                 j.roomTemperature, j.meltTemperature,
+                                                 j.specificHeat, j.taylorQuinney)
+        return out
+
+
+Material.STEEL_4340 = Material("AISI 4340", 205.0e9, 0.29, 7850.0)      # :132
+Material.COPPER_OFHC = Material("OFHC Cu", 117.0e9, 0.34, 8960.0)      # :135
+
+
+class Materials:
+    """core/Materials.java:23-129: one material, or one per element."""
+
+    def __init__(self, distinct, index):
+        self.distinct = list(distinct)
+        self.index = index  # None when uniform
+
+    @staticmethod
+    def uniform(material):
+        return Materials([material], None)
+
+    @staticmethod
+    def of(by_element: Sequence[Material]):
+        distinct, index, seen = [], np.empty(len(by_element), np.int32), {}
+        for e, m in enumerate(by_element):
+            k = id(m)
+            if k not in seen:
+                seen[k] = len(distinct)
+                distinct.append(m)
+            index[e] = seen[k]
+        return Materials(distinct, index)
+
+    @staticmethod
+    def byElement(count, pick):
+        return Materials.of([pick(e) for e in range(count)])
+
+
+# ---------------------------------------------------------------- mesh
+
+class QuadMesh:
+    """mesh/QuadMesh.java:15-403: SoA node coordinates, CCW connectivity, pressure edges."""
+
+    def __init__(self, r, z, conn, pressureEdges=()):
+        self.r = np.ascontiguousarray(r, dtype=np.float64)
+        self.z = np.ascontiguousarray(z, dtype=np.float64)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1)
+        self.pressureEdges = np.ascontiguousarray(pressureEdges, dtype=np.int32).reshape(-1)
+        if self.r.shape != self.z.shape:
+            raise IllegalArgumentException("r and z must have the same length")
+        if self.conn.size % 4:
+            raise IllegalArgumentException("connectivity must be a multiple of 4")
+        if self.pressureEdges.size % 2:
+            raise IllegalArgumentException("pressure edges must be a multiple of 2")
+        self.nodeCount = self.r.size
+        self.elementCount = self.conn.size // 4
+
+    @staticmethod
+    def _lattice(r_of, z_of, nr, nz):
+        nodes_r = nr + 1
+        j, i = np.meshgrid(np.arange(nz + 1), np.arange(nodes_r), indexing="ij")
+        r = r_of(i.reshape(-1))
+        z = z_of(j.reshape(-1))
+        jj, ii = np.meshgrid(np.arange(nz), np.arange(nr), indexing="ij")
+        base = (jj * nodes_r + ii).reshape(-1)
+        conn = np.stack([base, base + 1, base + nodes_r + 1, base + nodes_r], axis=1)
+        return r, z, conn.astype(np.int32)
+
+    @staticmethod
+    def solidCylinder(radius, height, nr, nz):  # :324-356
+        if radius <= 0.0:
+            raise IllegalArgumentException("radius must be positive")
+        if height <= 0.0:
+            raise IllegalArgumentException("height must be positive")
+        if nr < 1 or nz < 1:
+            raise IllegalArgumentException("need at least one element in each direction")
+        # radius * i / nr and height * j / nz, as the reference computes them (multiply, then
+        # divide), element by element in double precision.
+        r, z, conn = QuadMesh._lattice(lambda i: radius * i.astype(np.float64) / nr,
+                                       lambda j: height * j.astype(np.float64) / nz, nr, nz)
+        return QuadMesh(r, z, conn)
+
+    @staticmethod
+    def cylinderWall(inner, outer, height, nr, nz):  # :254-296
+        if outer <= inner:
+            raise IllegalArgumentException("outer radius must exceed inner radius")
+        if nr < 1 or nz < 1:
+            raise IllegalArgumentException("need at least one element in each direction")
+        r, z, conn = QuadMesh._lattice(lambda i: inner + (outer - inner) * i.astype(np.float64) / nr,
+                                       lambda j: height * j.astype(np.float64) / nz, nr, nz)
+        nodes_r = nr + 1
+        j = np.arange(nz)
+        edges = np.stack([j * nodes_r, (j + 1) * nodes_r], axis=1)
+        return QuadMesh(r, z, conn, edges)
+
+    @staticmethod
+    def axisNodes(nr, nz):  # :362-364
+        return [j * (nr + 1) for j in range(nz + 1)]
+
+    innerSurfaceNodes = axisNodes
+
+    @staticmethod
+    def outerSurfaceNodes(nr, nz):
+        return [j * (nr + 1) + nr for j in range(nz + 1)]
+
+    @staticmethod
+    def nearFaceNodes(nr, nz):
+        return list(range(nr + 1))
+
+    @staticmethod
+    def farFaceNodes(nr, nz):
+        return [nz * (nr + 1) + i for i in range(nr + 1)]
+
+    def signedArea(self, e):
+        c = self.conn[4 * e:4 * e + 4]
+        a = 0.0
+        for k in range(4):
+            i, j = c[k], c[(k + 1) % 4]
+            a += self.r[i] * self.z[j] - self.r[j] * self.z[i]
+        return 0.5 * a
+
+
+# ---------------------------------------------------------------- boundary objects
+
+@dataclass(frozen=True)
+class RigidWall:
+    """solver/RigidWall.java:95-136: the plane z = plane, friction coefficient (inf = welded)."""
+    plane: float
+    friction: float = 0.0
+
+    @staticmethod
+    def atZ(z, friction=0.0):
+        if friction < 0.0 or math.isnan(friction):
+            raise IllegalArgumentException("friction coefficient must be >= 0")
+        return RigidWall(z, friction)
+
+
+class Contact:
+    """solver/Contact.java:273-364: penalty contact settings (built into the solver's state)."""
+
+    def __init__(self, mesh=None):
+        self.friction = 0.0
+        self.damping = 0.0
+        self.scale = 0.1
+
+    def withFriction(self, mu):
+        if mu < 0.0:
+            raise IllegalArgumentException("friction must not be negative")
+        self.friction = mu
+        return self
+
+    def withDamping(self, zeta):
+        if zeta < 0.0 or zeta > 1.0:
+            raise IllegalArgumentException("contact damping is a fraction of critical, so it lies in [0, 1]")
+        self.damping = zeta
+        return self
+
+    def withStiffnessScale(self, s):
+        if not (s > 0.0) or s >= 2.0:
+            raise IllegalArgumentException("the stiffness scale must lie in (0, 2)")
+        self.scale = s
+        return self
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_abi.c_double_p)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_abi.c_int32_p)
+
+
+class ExplicitSolver:
+    """solver/ExplicitSolver.java:38-1966 over a native handle.
+
+    `binding` selects the library; the default is the HIP library, loaded through
+    `neofiz_amd._native.binding()`, which raises if it is not built.
+    """
+
+    def __init__(self, mesh: QuadMesh, materials, formulation=AXISYMMETRIC, integration=FULL,
+                 kinematics=SMALL_STRAIN, thickness=1.0, cflSafety=0.5, binding=None):
+        if binding is None:
+            from . import _native
+            binding = _native.binding()
+        self._b = binding
+        if isinstance(materials, Material):
+            materials = Materials.uniform(materials)
+        self.mesh = mesh
+        self.materials = materials
+        mats = (_abi.MaterialC * len(materials.distinct))(*[m.to_c() for m in materials.distinct])
+        index = None
+        if materials.index is not None:
+            self._index = np.ascontiguousarray(materials.index, dtype=np.int32)
+            index = _iptr(self._index)
+        handle = C.c_void_p()
+        edges = mesh.pressureEdges if mesh.pressureEdges.size else np.zeros(2, np.int32)
+        check(self._b, self._b.solver_create(
+            C.byref(handle), mesh.nodeCount, _dptr(mesh.r), _dptr(mesh.z), mesh.elementCount,
+            _iptr(mesh.conn), mesh.pressureEdges.size // 2, _iptr(edges),
+            len(materials.distinct), mats, index, formulation, integration, kinematics,
+            float(thickness), float(cflSafety)))
+        self._h = handle
+        self.integration = integration
+        self.formulation = formulation
+        self.kinematics = kinematics
+        self.wall = None
+        self._contact = None
+
+    # lifecycle
+    def shutdown(self):
+        if getattr(self, "_h", None):
+            self._b.solver_destroy(self._h)
+            self._h = None
+
+    close = shutdown
+
+    def __del__(self):
+        try:
+            self.shutdown()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        check(self._b, getattr(self._b, name)(self._h, *args))
+
+    # setters, solver/ExplicitSolver.java:386-808
+    def setThreads(self, n):
+        self._call("set_threads", int(n))
+
+    def fixRadial(self, node):
+        self._call("fix_radial", int(node))
+
+    def fixAxial(self, node):
+        self._call("fix_axial", int(node))
+
+    def fixAllAxial(self):
+        self._call("fix_all_axial")
+
+    def setPressureRamp(self, pascals, rampSeconds):
+        self._call("set_pressure_ramp", float(pascals), float(rampSeconds))
+
+    def setRelaxationDamping(self, fraction, omega):
+        self._call("set_relaxation_damping", float(fraction), float(omega))
+
+    def setHourglassCoefficient(self, q):
+        self._call("set_hourglass_coefficient", float(q))
+
+    def setRigidWall(self, wall: Optional[RigidWall]):
+        if wall is None:
+            self._call("set_rigid_wall", 0, 0.0, 0.0)
+        else:
+            self._call("set_rigid_wall", 1, float(wall.plane), float(wall.friction))
+        self.wall = wall
+
+    def setContact(self, contact: Optional[Contact]):
+        if contact is None:
+            self._call("set_contact", 0, 0.0, 0.0, 0.1)
+        else:
+            self._call("set_contact", 1, float(contact.friction), float(contact.damping),
+                       float(contact.scale))
+        self._contact = contact
+
+    def setGravity(self, ar, az):
+        self._call("set_gravity", float(ar), float(az))
+
+    def setDamage(self, fractureEnergy, failureStrain):
+        E = self.mesh.elementCount
+        gf = np.ascontiguousarray(np.broadcast_to(np.asarray(fractureEnergy, np.float64), (E,)))
+        eps = np.ascontiguousarray(failureStrain, dtype=np.float64)
+        if eps.size != E:
+            raise IllegalArgumentException(
+                f"failure strain has {eps.size} entries for a mesh of {E} elements")
+        self._call("set_damage", _dptr(gf), _dptr(eps))
+
+    def setErosion(self, threshold):
+        self._call("set_erosion", float(threshold))
+
+    def setMassScaling(self, targetStep):
+        self._call("set_mass_scaling", float(targetStep))
+
+    def setVelocity(self, node, vr, vz):
+        self._call("set_velocity", int(node), float(vr), float(vz))
+
+    def setUniformVelocity(self, vr, vz):
+        self._call("set_uniform_velocity", float(vr), float(vz))
+
+    # run
+    def step(self):
+        self._call("run", 1)
+
+    def run(self, steps):
+        self._call("run", int(steps))
+
+    def runUntil(self, endTime, maxSteps=0):
+        taken = C.c_int64()
+        self._call("run_until", float(endTime), int(maxSteps), C.byref(taken))
+        return taken.value
+
+    def advance(self):
+        self._call("advance")
+
+    # state
+    def nodeField(self, name):
+        out = np.empty(self.mesh.nodeCount, np.float64)
+        self._call("get_node_field", _abi.NODE_FIELDS[name], _dptr(out))
+        return out
+
+    def putNodeField(self, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._call("put_node_field", _abi.NODE_FIELDS[name], _dptr(v))
+
+    def pointField(self, name):
+        st = self.state()
+        P = st.element_count * st.points_per_element
+        width = 4 if name in ("stress", "back") else 1
+        out = np.empty(P * width, np.float64)
+        self._call("get_point_field", _abi.POINT_FIELDS[name], _dptr(out))
+        return out
+
+    def elementField(self, name):
+        out = np.empty(self.mesh.elementCount, np.float64)
+        self._call("get_element_field", _abi.ELEMENT_FIELDS[name], _dptr(out))
+        return out
+
+    def state(self):
+        st = _abi.StateC()
+        self._call("get_state", C.byref(st))
+        return st
+
+    def audits(self):
+        a = _abi.AuditsC()
+        self._call("get_audits", C.byref(a))
+        return a
+
+    # Java accessor names, solver/ExplicitSolver.java:830-932, :1610-1860
+    def radialDisplacement(self):
+        return self.nodeField("ur")
+
+    def axialDisplacement(self):
+        return self.nodeField("uz")
+
+    def velocityR(self):
+        return self.nodeField("vr")
+
+    def velocityZ(self):
+        return self.nodeField("vz")
+
+    def forceR(self):
+        return self.nodeField("fr")
+
+    def forceZ(self):
+        return self.nodeField("fz")
+
+    def stress(self):
+        return self.pointField("stress")
+
+    def plasticStrain(self):
+        return self.pointField("plastic_strain")
+
+    def plasticWorkDensity(self):
+        return self.pointField("plastic_work")
+
+    def time(self):
+        return self.state().time
+
+    def timestep(self):
+        return self.state().dt
+
+    def steps(self):
+        return self.state().steps
+
+    def massScale(self):
+        return self.state().mass_scale
+
+    def erodedElements(self):
+        return self.state().eroded_elements
+
+    def invertedElements(self):
+        return self.state().inverted_elements
+
+    def erodedEnergy(self):
+        return self.state().eroded_energy
+
+    def isEroded(self, e):
+        return bool(self.elementField("eroded")[e])
+
+    def axialMomentum(self):
+        return self.audits().axial_momentum
+
+    def radialMomentum(self):
+        return self.audits().radial_momentum
+
+    def kineticEnergy(self):
+        return self.audits().kinetic_energy
+
+    def centredKineticEnergy(self):
+        return self.audits().centred_kinetic_energy
+
+    def strainEnergy(self):
+        return self.audits().strain_energy
+
+    def plasticDissipation(self):
+        return self.audits().plastic_dissipation
+
+    def hourglassEnergy(self):
+        return self.audits().hourglass_energy
+
+    def fractureDissipation(self):
+        return self.audits().fracture_dissipation
+
+    def maxPlasticStrain(self):
+        return self.audits().max_plastic_strain
+
+    def yieldedFraction(self):
+        return self.audits().yielded_fraction
+
+    def maxTemperature(self):
+        t = self.audits().max_temperature
+        if math.isnan(t):
+            raise IllegalStateException("this material has no thermal model; temperature is not defined")
+        return t
+
+    def maxDamage(self):
+        return self.audits().max_damage
+
+    # RigidWall audits live in the solver's state on this side of the boundary.
+    def wallImpulse(self):
+        return self.state().wall_impulse
+
+    def wallEnergyLoss(self):
+        return self.state().wall_energy_loss
+
+    def wallFrictionDissipation(self):
+        return self.state().wall_friction_dissipation
+
+    def wallContactCount(self):
+        return self.state().wall_contact_count
+
+    def contactPairs(self):
+        """(node, segment end a, segment end b) for every pair of the last step, node order."""
+        if not hasattr(self._b, "contact_pairs"):
+            raise IllegalStateException("this library does not report contact pairs")
+        n = C.c_int32()
+        self._call("contact_pairs", 0, None, C.byref(n))
+        out = np.empty(max(1, 3 * n.value), np.int32)
+        self._call("contact_pairs", n.value, _iptr(out), C.byref(n))
+        return out[:3 * n.value].reshape(-1, 3)

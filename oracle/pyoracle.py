@@ -1,0 +1,34 @@
+"""TEST INFRASTRUCTURE -- Python binding of the oracle (libneofiz_oracle.so, prefix nfo_).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this, and only as
+the checker. It exposes the same ExplicitSolver surface as the product over the oracle's
+C ABI.
+"""
+import ctypes
+import os
+import subprocess
+
+from neofiz_amd._abi import Binding
+from neofiz_amd.solver import ExplicitSolver
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "_build", "libneofiz_oracle.so")
+_binding = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def binding():
+    global _binding
+    if _binding is None:
+        if not os.path.exists(LIBRARY):
+            build()
+        _binding = Binding(ctypes.CDLL(LIBRARY), "nfo_")
+    return _binding
+
+
+def OracleSolver(*args, **kwargs):
+    kwargs["binding"] = binding()
+    return ExplicitSolver(*args, **kwargs)
