@@ -1,0 +1,252 @@
+// Device-side constitutive and element arithmetic for the gfx950 kernels.
+//
+// Each function restates one method of the Java reference operation for operation, in the
+// same evaluation order, so that the kernels reproduce the reference's arithmetic exactly
+// (compiled with -ffp-contract=off; IEEE division and square root). State is passed in
+// registers rather than through the reference's strided arrays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nf_math.h"
+
+namespace nfd {
+
+#define NFD_MAX_VALUE 1.7976931348623157e308
+#define NFD_PI 3.141592653589793
+
+// sqrt(2/3) and 2/3 as Java computes them (solver/J2.java:82-83).
+#define NFD_SQRT_2_3 0x1.a20bd700c2c3ep-1
+#define NFD_TWO_THIRDS (2.0 / 3.0)
+
+// Math.min / Math.max with Java's NaN and signed-zero rules.
+__device__ __forceinline__ double jmin(double a, double b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    if (a == 0.0 && b == 0.0) return __builtin_signbit(a) ? a : b;
+    return a <= b ? a : b;
+}
+__device__ __forceinline__ double jmax(double a, double b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    if (a == 0.0 && b == 0.0) return __builtin_signbit(a) ? b : a;
+    return a >= b ? a : b;
+}
+__device__ __forceinline__ double jabs(double a) { return __builtin_fabs(a); }
+
+// Per-material constants, unrolled once on the host (solver/ExplicitSolver.java:289-299,
+// solver/J2.java:105-113, core/JohnsonCook.java:68-71).
+struct Mat {
+    double lambda, mu, bulk, density;
+    double sy0, hIso, hKin, thermalCoefficient;
+    double a, b, n, c, m, eps0, troom, tmelt, cp, beta;
+    double youngs;
+    int32_t hasJC;
+    int32_t pad;
+};
+
+// core/JohnsonCook.java:138-197
+__device__ __forceinline__ double jc_strainTerm(const Mat& j, double e) {
+    return e <= 0.0 ? j.a : j.a + j.b * nfm::pow(e, j.n);
+}
+__device__ __forceinline__ double jc_rateTerm(const Mat& j, double rate) {
+    if (rate <= j.eps0) return 1.0;
+    return 1.0 + j.c * nfm::log(rate / j.eps0);
+}
+__device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t) {
+    const double homologous = (t - j.troom) / (j.tmelt - j.troom);
+    if (homologous <= 0.0) return 1.0;
+    if (homologous >= 1.0) return 0.0;
+    return 1.0 - nfm::pow(homologous, j.m);
+}
+__device__ __forceinline__ double jc_quasiStatic(const Mat& j, double e, double t) {
+    return jc_strainTerm(j, e) * jc_thermalTerm(j, t);
+}
+
+// solver/J2.java:332-375, the bracketed Newton solve for the Johnson-Cook multiplier.
+__device__ double j2_solve(double norm, double stiffness, double plasticStrain,
+                           double temperature, double dt, const Mat& j) {
+    const double thermal = jc_thermalTerm(j, temperature);
+    const double tolerance = 1.0e-10 * j.a;
+    double lo = 0.0;
+    double hi = norm / stiffness;
+    double dGamma = hi;
+    for (int iteration = 0; iteration < 25; iteration++) {
+        const double dEps = NFD_SQRT_2_3 * dGamma;
+        const double eps = plasticStrain + dEps;
+        const double rate = dt > 0.0 ? dEps / dt : 0.0;
+        const double strain = jc_strainTerm(j, eps);
+        const double rateFactor = jc_rateTerm(j, rate);
+        const double g = norm - stiffness * dGamma - NFD_SQRT_2_3 * strain * rateFactor * thermal;
+        if (g > 0.0) lo = dGamma;
+        else hi = dGamma;
+        if (jabs(g) <= tolerance) return dGamma;
+        double slope = -stiffness - NFD_TWO_THIRDS * (j.n * (strain - j.a) / eps) * rateFactor * thermal;
+        if (!(rate <= j.eps0)) {
+            slope -= NFD_SQRT_2_3 * strain * j.c * thermal / dGamma;
+        }
+        double next = dGamma - g / slope;
+        if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+        dGamma = next;
+    }
+    return dGamma;
+}
+
+// Damage state of one Gauss point (solver/Damage.java:156-192).
+struct DamagePoint {
+    double strength, softening, width;  // mutable: written at onset
+    double failureStrain, fractureEnergy, youngs;
+    double area, uR, uZ, vR, vZ;
+};
+
+// solver/Damage.java:306-336
+__device__ void damage_begin(DamagePoint& d, double flowStress, double nRR, double nZZ,
+                             double nRZ) {
+    const double diff = nRR - nZZ;
+    const double cross = 2.0 * nRZ;
+    const double anisotropy = nfm::hypot(diff, cross);
+    double h;
+    if (anisotropy < 1.0e-9) {
+        h = __builtin_sqrt(d.area);
+    } else {
+        const double phi = 0.5 * nfm::atan2(cross, diff);
+        const double tR = -nfm::sin(phi), tZ = nfm::cos(phi);
+        const double line = jabs(d.uR * tR + d.uZ * tZ) + jabs(d.vR * tR + d.vZ * tZ);
+        h = d.area / line;
+    }
+    d.width = h;
+    d.strength = flowStress;
+    const double slope = -flowStress * flowStress * h / (2.0 * d.fractureEnergy);
+    const double floor = -d.youngs;
+    d.softening = slope < floor ? floor : slope;
+}
+
+// solver/Damage.java:342-347
+__device__ __forceinline__ double damage_at(double strength, double softening,
+                                            double failureStrain, double plasticStrain) {
+    const double sf = strength;
+    if (sf <= 0.0) return 0.0;
+    const double d = -softening * (plasticStrain - failureStrain) / sf;
+    return d <= 0.0 ? 0.0 : (d >= 1.0 ? 1.0 : d);
+}
+
+// Point state carried through the update.
+struct PointState {
+    double s0, s1, s2, s3;  // stress rr, zz, tt, rz
+    double b0, b1, b2, b3;  // back stress
+    double epsP, work;
+};
+
+// solver/J2.java:163-296. `dmg` is null when the solver has no damage model.
+template <bool DAMAGE>
+__device__ void j2_update(PointState& q, double dRR, double dZZ, double dTT, double dGamRZ,
+                          const Mat& fl, double strainDt, DamagePoint* dmg) {
+    const double lambda = fl.lambda, mu = fl.mu;
+    const double lt = lambda * (dRR + dZZ + dTT);
+    const double tRR = q.s0 + lt + 2.0 * mu * dRR;
+    const double tZZ = q.s1 + lt + 2.0 * mu * dZZ;
+    const double tTT = q.s2 + lt + 2.0 * mu * dTT;
+    const double tRZ = q.s3 + mu * dGamRZ;
+
+    const double p = (tRR + tZZ + tTT) / 3.0;
+    const double xRR = tRR - p - q.b0;
+    const double xZZ = tZZ - p - q.b1;
+    const double xTT = tTT - p - q.b2;
+    const double xRZ = tRZ - q.b3;
+    const double norm = __builtin_sqrt(xRR * xRR + xZZ * xZZ + xTT * xTT + 2.0 * xRZ * xRZ);
+
+    const bool jc = fl.hasJC != 0;
+    const double temperature = jc ? fl.troom + fl.thermalCoefficient * q.work : 0.0;
+
+    const bool softening = DAMAGE && dmg->strength > 0.0;
+    double hardening = 0.0;
+    double flowStress;
+    if (softening) {
+        flowStress = dmg->strength + dmg->softening * (q.epsP - dmg->failureStrain);
+        if (flowStress > 0.0) {
+            hardening = dmg->softening;
+        } else {
+            flowStress = 0.0;
+        }
+    } else if (!jc) {
+        flowStress = fl.sy0 + fl.hIso * q.epsP;
+        hardening = fl.hIso;
+    } else {
+        flowStress = jc_quasiStatic(fl, q.epsP, temperature);
+    }
+    const double radius = NFD_SQRT_2_3 * flowStress;
+
+    if (!(norm > radius)) {
+        q.s0 = tRR;
+        q.s1 = tZZ;
+        q.s2 = tTT;
+        q.s3 = tRZ;
+        return;
+    }
+
+    const double stiffness = 2.0 * mu + NFD_TWO_THIRDS * fl.hKin;
+    double dGamma;
+    if (!jc || softening) {
+        dGamma = (norm - radius) / (stiffness + NFD_TWO_THIRDS * hardening);
+        if (softening && flowStress + hardening * NFD_SQRT_2_3 * dGamma < 0.0) {
+            dGamma = norm / stiffness;
+        }
+    } else {
+        dGamma = j2_solve(norm, stiffness, q.epsP, temperature, strainDt, fl);
+    }
+    const double inv = 1.0 / norm;
+    const double nRR = xRR * inv, nZZ = xZZ * inv, nTT = xTT * inv, nRZ = xRZ * inv;
+
+    const double pull = 2.0 * mu * dGamma;
+    q.s0 = tRR - pull * nRR;
+    q.s1 = tZZ - pull * nZZ;
+    q.s2 = tTT - pull * nTT;
+    q.s3 = tRZ - pull * nRZ;
+
+    const double alphaDotN = q.b0 * nRR + q.b1 * nZZ + q.b2 * nTT + 2.0 * q.b3 * nRZ;
+    q.work += dGamma * (norm - pull + alphaDotN);
+
+    const double shift = NFD_TWO_THIRDS * fl.hKin * dGamma;
+    q.b0 += shift * nRR;
+    q.b1 += shift * nZZ;
+    q.b2 += shift * nTT;
+    q.b3 += shift * nRZ;
+
+    q.epsP += NFD_SQRT_2_3 * dGamma;
+
+    if (DAMAGE && !softening && q.epsP >= dmg->failureStrain) {
+        damage_begin(*dmg, !jc ? fl.sy0 + fl.hIso * q.epsP : jc_quasiStatic(fl, q.epsP, temperature),
+                     nRR, nZZ, nRZ);
+    }
+}
+
+// solver/Corotational.java:59-97, applied to one tensor held in registers.
+__device__ __forceinline__ void corot_rotate(double& rr, double& zz, double& rz, double cos,
+                                             double sin) {
+    const double cc = cos * cos, ss = sin * sin, cs = cos * sin;
+    const double r0 = rr, z0 = zz, x0 = rz;
+    rr = cc * r0 - 2.0 * cs * x0 + ss * z0;
+    zz = ss * r0 + 2.0 * cs * x0 + cc * z0;
+    rz = cs * (r0 - z0) + (cc - ss) * x0;
+}
+__device__ __forceinline__ double corot_cos(double spinDt) {
+    const double h = 0.5 * spinDt;
+    return (1.0 - h * h) / (1.0 + h * h);
+}
+__device__ __forceinline__ double corot_sin(double spinDt) {
+    const double h = 0.5 * spinDt;
+    return 2.0 * h / (1.0 + h * h);
+}
+
+__device__ __forceinline__ double sq(double a, double b) { return a * a + b * b; }
+
+// solver/ExplicitSolver.java:1191-1197
+__device__ __forceinline__ double shortestEdge(double running, double r0, double r1, double r2,
+                                               double r3, double z0, double z1, double z2,
+                                               double z3) {
+    return jmin(running, jmin(jmin(sq(r1 - r0, z1 - z0), sq(r2 - r1, z2 - z1)),
+                              jmin(sq(r3 - r2, z3 - z2), sq(r0 - r3, z0 - z3))));
+}
+
+}  // namespace nfd

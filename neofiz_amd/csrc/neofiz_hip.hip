@@ -1,0 +1,1799 @@
+// libneofiz_hip -- Neofiz's explicit step loop (solver/ExplicitSolver.java:940-1115) as
+// hand-written gfx950 FP64 kernels behind the C ABI of include/neofiz.h.
+//
+// One step is three launches on the handle's stream:
+//   k_nodal    central-difference update of every node, rigid anvil with Coulomb cone
+//              (solver/ExplicitSolver.java:998-1062, solver/RigidWall.java:163-279);
+//   k_element  erosion test, then the element pass: mid-step kinematics, Cayley co-rotation,
+//              J2 / Johnson-Cook return map, damage onset, internal and hourglass forces,
+//              written to per-element force slots (:1219-1504); shortest edge by an exact
+//              atomic minimum (min is associative, so the result is order-free);
+//   k_gather   fixed-order CSR sum of each node's slots plus gravity (:1147-1165), and the
+//              adaptive CFL step (:985-990).
+// Every control scalar (time, dt, step count, the while-time-below-end guard) lives on the
+// device, so steps chain without a host round trip and a block of steps can be captured in a
+// hipGraph. No atomic touches a force or a state value: forces are summed in the reference's
+// order, so the GPU reproduces the reference arithmetic bit for bit. Audit accumulators that
+// the reference sums serially (wall impulse, eroded energy) are reduced per step in a fixed
+// order instead; they agree to rounding, not bitwise.
+//
+// Layout (structure of arrays, 16-byte pairs where a thread reads both components):
+//   nodes     x0[N] (r, z), u[N] (ur, uz), v[N] (vr, vz), f[N] (fr, fz), mass[N], flags[N]
+//   elements  conn[E] int4, matIndex[E], gone[E]
+//   points    sig0..3[P], back0..3[P], epsP[P], work[P]; hgR, hgZ, hgK [E]
+//   slots     slot[4 E + 2 edges] (gr, gz); CSR slotStart[N + 1], slotOf[4 E + 2 edges]
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/neofiz.h"
+#include "device_physics.h"
+
+using nfd::Mat;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t err_ = (expr);                                                      \
+        if (err_ != hipSuccess) {                                                      \
+            return fail(err_ == hipErrorOutOfMemory ? NF_E_MEMORY : NF_E_DEVICE,       \
+                        std::string(#expr) + ": " + hipGetErrorString(err_));          \
+        }                                                                              \
+    } while (0)
+
+constexpr int BLOCK = 256;
+constexpr uint8_t FIXED_R = 1, FIXED_Z = 2, ON_WALL = 4;
+constexpr int MODE_STEP = 0, MODE_INITIAL = 1, MODE_ADVANCE = 2;
+
+// Device-resident control block. Each field has one writer per launch (block 0, thread 0)
+// and is read only by later launches, or is written only through atomics.
+struct Ctl {
+    double time, dt, dtPrev;
+    double endTime;
+    long long steps, stepLimit;
+    double timeNext, dtStep;
+    int go;
+    int nanEdge;
+    unsigned long long minEdgeBits;
+    unsigned int wallTicket, erodeTicket;
+    int wallContact, wallStuck;
+    double wallImpulse, wallEnergyLoss, wallFriction;
+    int erodedCount, invertedCount;
+    double erodedEnergy;
+    int surfaceStale;
+    int pad;
+};
+
+// Everything a kernel needs, passed by value.
+struct Dev {
+    Ctl* ctl;
+    int N, E, P, edges, ppe;
+    const double2* x0;
+    double2* u;
+    double2* v;
+    double2* f;
+    const double* mass;
+    uint8_t* flags;
+    const int4* conn;
+    const int* matIndex;  // null when uniform
+    const Mat* mats;
+    uint8_t* gone;        // null when erosion is off
+    double* sig0; double* sig1; double* sig2; double* sig3;
+    double* back0; double* back1; double* back2; double* back3;
+    double* epsP; double* work;
+    double* hgR; double* hgZ; double* hgK;
+    double2* slot;
+    const int* slotStart;
+    const int* slotOf;
+    const int2* pedges;
+    // damage (null when off)
+    double* dStrength; double* dSoftening; double* dWidth;
+    const double* dFailure; const double* dGf; const double* dYoungs; const double* dGeom;
+    double erosionDamage;
+    // per-block partial sums for the ordered reductions
+    double* wallPartial;
+    int* wallCountPartial;
+    double* erodePartial;
+    // uniforms
+    int axi, finite, wall, wallFrictional;
+    double thickness, qhg, damping, cfl, fastestWave, massScale;
+    double gravityR, gravityZ;
+    double plane, friction;  // friction may be +inf
+    double targetPressure, rampDuration;
+};
+
+__device__ __forceinline__ double pressureAt(const Dev& d, double t) {  // :1203-1207
+    if (d.rampDuration <= 0.0) return d.targetPressure;
+    double s = nfd::jmin(1.0, t / d.rampDuration);
+    return d.targetPressure * s * s * (3.0 - 2.0 * s);
+}
+
+// Fixed-order block sum of up to 8 doubles (wave shuffle tree, then LDS in wave order).
+template <int K>
+__device__ void block_sum(double (&vals)[K], double* lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double x = vals[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        vals[k] = x;
+    }
+    if (lane == 0)
+        for (int k = 0; k < K; k++) lds[wave * K + k] = vals[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; k++) {
+            double s = 0.0;
+            for (int w = 0; w < BLOCK / 64; w++) s += lds[w * K + k];
+            vals[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ nodal update
+// solver/ExplicitSolver.java:947-974, :998-1062 with solver/RigidWall.java:153-279.
+template <bool WALL>
+__global__ void __launch_bounds__(BLOCK) k_nodal(Dev d) {
+    Ctl* c = d.ctl;
+    const bool go = c->time < c->endTime && c->steps < c->stepLimit;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->go = go ? 1 : 0;
+        if (go) {
+            c->timeNext = c->time + c->dt;
+            c->dtStep = c->dt;
+        }
+    }
+    if (!go) return;
+    const double dt = c->dt;
+    const double dtAvg = 0.5 * (c->dtPrev + dt);
+    const double half = 0.5 * d.damping * dtAvg;
+    const double decay = (1.0 - half) / (1.0 + half);
+    const double gain = dtAvg / (1.0 + half);
+
+    double acc[3] = {0.0, 0.0, 0.0};  // impulse, energy loss, friction dissipation
+    int contacts = 0, stuck = 0;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < d.N) {
+        const double m = d.mass[i];
+        const double invM = 1.0 / m;
+        uint8_t fl = d.flags[i];
+        double2 vi = d.v[i];
+        double2 ui = d.u[i];
+        const double2 fi = d.f[i];
+        const bool resting = WALL && (fl & ON_WALL) && fi.y <= 0.0;
+        if (fl & FIXED_R) {
+            vi.x = 0.0;
+        } else if (WALL && resting && d.wallFrictional) {
+            // RigidWall.slide :252-279
+            const double normal = -fi.y;
+            const double trial = -m * decay * vi.x / gain - fi.x;
+            const double limit = __builtin_isinf(d.friction) ? HUGE_VAL : d.friction * normal;
+            if (nfd::jabs(trial) <= limit) {
+                acc[1] += 0.5 * m * vi.x * vi.x;
+                stuck++;
+                vi.x = 0.0;
+            } else {
+                const double tangential = __builtin_copysign(limit, trial);
+                const double slid = decay * vi.x + gain * (fi.x + tangential) / m;
+                acc[2] += -tangential * slid * dt;
+                vi.x = slid;
+            }
+            ui.x += vi.x * dt;
+        } else {
+            vi.x = decay * vi.x + gain * fi.x * invM;
+            ui.x += vi.x * dt;
+        }
+        if (fl & FIXED_Z) {
+            vi.y = 0.0;
+        } else if (WALL && resting) {
+            acc[0] += -fi.y * dtAvg;  // RigidWall.hold :175-178
+            contacts++;
+        } else {
+            const double trial = decay * vi.y + gain * fi.y * invM;
+            const double zc = d.x0[i].y + ui.y;
+            if (WALL && zc + trial * dt < d.plane) {
+                acc[0] += -m * trial;  // RigidWall.arrive :201-205
+                acc[1] += 0.5 * m * trial * trial;
+                contacts++;
+                ui.y += d.plane - zc;
+                vi.y = 0.0;
+                fl |= ON_WALL;
+            } else {
+                vi.y = trial;
+                ui.y += vi.y * dt;
+                if (WALL) fl &= (uint8_t)~ON_WALL;
+            }
+        }
+        d.v[i] = vi;
+        d.u[i] = ui;
+        if (WALL) d.flags[i] = fl;
+    }
+    if (!WALL) return;
+
+    __shared__ double lds[4 * 5];
+    __shared__ int ldsi[2 * 4];
+    __shared__ bool last;
+    block_sum<3>(acc, lds);
+    // integer counts: plain wave sums
+    {
+        int a = contacts, b = stuck;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            b += __shfl_xor(b, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            ldsi[(threadIdx.x >> 6) * 2] = a;
+            ldsi[(threadIdx.x >> 6) * 2 + 1] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            contacts = ldsi[0] + ldsi[2] + ldsi[4] + ldsi[6];
+            stuck = ldsi[1] + ldsi[3] + ldsi[5] + ldsi[7];
+        }
+    }
+    if (threadIdx.x == 0) {
+        double* p = d.wallPartial + 3 * blockIdx.x;
+        p[0] = acc[0];
+        p[1] = acc[1];
+        p[2] = acc[2];
+        d.wallCountPartial[2 * blockIdx.x] = contacts;
+        d.wallCountPartial[2 * blockIdx.x + 1] = stuck;
+        __threadfence();
+        const unsigned t = atomicAdd(&c->wallTicket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // Last block: every thread sums a fixed strided subset of the block partials, then the
+    // block tree adds those in a fixed order, so the per-step totals are reproducible.
+    double tot[3] = {0.0, 0.0, 0.0};
+    int cc = 0, cs = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK) {
+        tot[0] += d.wallPartial[3 * b];
+        tot[1] += d.wallPartial[3 * b + 1];
+        tot[2] += d.wallPartial[3 * b + 2];
+        cc += d.wallCountPartial[2 * b];
+        cs += d.wallCountPartial[2 * b + 1];
+    }
+    block_sum<3>(tot, lds);
+    for (int off = 32; off > 0; off >>= 1) {
+        cc += __shfl_xor(cc, off, 64);
+        cs += __shfl_xor(cs, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        ldsi[(threadIdx.x >> 6) * 2] = cc;
+        ldsi[(threadIdx.x >> 6) * 2 + 1] = cs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->wallImpulse += tot[0];
+        c->wallEnergyLoss += tot[1];
+        c->wallFriction += tot[2];
+        c->wallContact = ldsi[0] + ldsi[2] + ldsi[4] + ldsi[6];
+        c->wallStuck = ldsi[1] + ldsi[3] + ldsi[5] + ldsi[7];
+        c->wallTicket = 0u;
+    }
+}
+
+// ------------------------------------------------------------------ element pass
+
+__device__ __forceinline__ Mat load_mat(const Dev& d, int e) {
+    return d.matIndex ? d.mats[d.matIndex[e]] : d.mats[0];
+}
+
+// Signed current area, solver/ExplicitSolver.java:681-691.
+__device__ __forceinline__ double currentArea(const Dev& d, int4 n) {
+    const int idx[4] = {n.x, n.y, n.z, n.w};
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = idx[k], j = idx[(k + 1) & 3];
+        const double2 xi = d.x0[i], ui = d.u[i], xj = d.x0[j], uj = d.u[j];
+        a += (xi.x + ui.x) * (xj.y + uj.y) - (xj.x + uj.x) * (xi.y + ui.y);
+    }
+    return 0.5 * a;
+}
+
+__device__ __forceinline__ nfd::PointState load_point(const Dev& d, int p) {
+    nfd::PointState q;
+    q.s0 = d.sig0[p]; q.s1 = d.sig1[p]; q.s2 = d.sig2[p]; q.s3 = d.sig3[p];
+    q.b0 = d.back0[p]; q.b1 = d.back1[p]; q.b2 = d.back2[p]; q.b3 = d.back3[p];
+    q.epsP = d.epsP[p];
+    q.work = d.work[p];
+    return q;
+}
+
+__device__ __forceinline__ void store_point(const Dev& d, int p, const nfd::PointState& q) {
+    d.sig0[p] = q.s0; d.sig1[p] = q.s1; d.sig2[p] = q.s2; d.sig3[p] = q.s3;
+    d.back0[p] = q.b0; d.back1[p] = q.b1; d.back2[p] = q.b2; d.back3[p] = q.b3;
+    d.epsP[p] = q.epsP;
+    d.work[p] = q.work;
+}
+
+__device__ __forceinline__ void load_damage(const Dev& d, int p, nfd::DamagePoint& dp) {
+    dp.strength = d.dStrength[p];
+    dp.softening = d.dSoftening[p];
+    dp.width = d.dWidth[p];
+    dp.failureStrain = d.dFailure[p];
+    dp.fractureEnergy = d.dGf[p];
+    dp.youngs = d.dYoungs[p];
+    const double* g = d.dGeom + 5 * (size_t)p;
+    dp.area = g[0]; dp.uR = g[1]; dp.uZ = g[2]; dp.vR = g[3]; dp.vZ = g[4];
+}
+
+// Strain energy of one point, solver/ExplicitSolver.java:1666-1675 (without the volume).
+__device__ __forceinline__ double point_energy_density(const Dev& d, int p, const Mat& m) {
+    const double s0 = d.sig0[p], s1 = d.sig1[p], s2 = d.sig2[p], s3 = d.sig3[p];
+    const double press = (s0 + s1 + s2) / 3.0;
+    const double a = s0 - press, bb = s1 - press, cc = s2 - press;
+    return (a * a + bb * bb + cc * cc + 2.0 * s3 * s3) / (4.0 * m.mu) + press * press / (2.0 * m.bulk);
+}
+
+// Gauss-point volume on the reference mesh, solver/ExplicitSolver.java:1939-1965.
+__device__ __forceinline__ double gaussVolume(const Dev& d, int4 n, int g) {
+    const double2 p0 = d.x0[n.x], p1 = d.x0[n.y], p2 = d.x0[n.z], p3 = d.x0[n.w];
+    const double G = 0.5773502691896258;
+    const double xi = d.ppe == 1 ? 0.0 : ((g == 0 || g == 3) ? -G : G);
+    const double eta = d.ppe == 1 ? 0.0 : (g < 2 ? -G : G);
+    const double weight = d.ppe == 1 ? 4.0 : 1.0;
+    const double xm = 1.0 - xi, xp = 1.0 + xi, em = 1.0 - eta, ep = 1.0 + eta;
+    const double N0 = 0.25 * xm * em, N1 = 0.25 * xp * em, N2 = 0.25 * xp * ep, N3 = 0.25 * xm * ep;
+    const double dN0x = -0.25 * em, dN1x = 0.25 * em, dN2x = 0.25 * ep, dN3x = -0.25 * ep;
+    const double dN0e = -0.25 * xm, dN1e = -0.25 * xp, dN2e = 0.25 * xp, dN3e = 0.25 * xm;
+    const double J00 = dN0x * p0.x + dN1x * p1.x + dN2x * p2.x + dN3x * p3.x;
+    const double J01 = dN0x * p0.y + dN1x * p1.y + dN2x * p2.y + dN3x * p3.y;
+    const double J10 = dN0e * p0.x + dN1e * p1.x + dN2e * p2.x + dN3e * p3.x;
+    const double J11 = dN0e * p0.y + dN1e * p1.y + dN2e * p2.y + dN3e * p3.y;
+    const double det = J00 * J11 - J01 * J10;
+    const double rg = N0 * p0.x + N1 * p1.x + N2 * p2.x + N3 * p3.x;
+    return weight * (d.axi ? 2.0 * NFD_PI * rg * det : d.thickness * det);
+}
+
+// Erosion test of one element (solver/ExplicitSolver.java:663-678), fused ahead of its own
+// assembly: the test reads only the element's own points and the step's new displacements,
+// exactly what the reference's separate pass reads. Returns true if the element is gone.
+__device__ bool erode_element(const Dev& d, int e, int4 n, double& energy, int& eroded,
+                              int& inverted, bool& stale) {
+    if (d.gone[e]) return true;
+    bool failed = false;
+    if (d.dStrength) {
+        double sum = 0.0;
+        for (int g = 0; g < d.ppe; g++) {
+            const int p = e * d.ppe + g;
+            sum += nfd::damage_at(d.dStrength[p], d.dSoftening[p], d.dFailure[p], d.epsP[p]);
+        }
+        failed = sum / d.ppe >= d.erosionDamage;
+    }
+    const bool inv = currentArea(d, n) <= 0.0;
+    if (!failed && !inv) return false;
+    d.gone[e] = 1;
+    eroded++;
+    if (inv && !failed) inverted++;
+    const Mat m = load_mat(d, e);
+    for (int g = 0; g < d.ppe; g++) energy += point_energy_density(d, e * d.ppe + g, m) * gaussVolume(d, n, g);
+    stale = true;
+    return true;
+}
+
+__device__ __forceinline__ void write_zero_slots(const Dev& d, int e) {
+    const double2 z = make_double2(0.0, 0.0);
+    d.slot[4 * e] = z;
+    d.slot[4 * e + 1] = z;
+    d.slot[4 * e + 2] = z;
+    d.slot[4 * e + 3] = z;
+}
+
+// One element under reduced integration, solver/ExplicitSolver.java:1373-1504.
+template <bool DAMAGE>
+__device__ double reduced_element(const Dev& d, int e, int4 n, double strainDt) {
+    const Mat fl = load_mat(d, e);
+    const double mu = fl.mu;
+    const double mid = -0.5 * strainDt;
+    const double2 p0 = d.x0[n.x], p1 = d.x0[n.y], p2 = d.x0[n.z], p3 = d.x0[n.w];
+    const double2 u0 = d.u[n.x], u1 = d.u[n.y], u2 = d.u[n.z], u3 = d.u[n.w];
+    const double2 v0 = d.v[n.x], v1 = d.v[n.y], v2 = d.v[n.z], v3 = d.v[n.w];
+    double r0, r1, r2, r3, z0, z1, z2, z3;
+    if (d.finite) {  // cr / cz :1176-1182
+        r0 = p0.x + u0.x + mid * v0.x; r1 = p1.x + u1.x + mid * v1.x;
+        r2 = p2.x + u2.x + mid * v2.x; r3 = p3.x + u3.x + mid * v3.x;
+        z0 = p0.y + u0.y + mid * v0.y; z1 = p1.y + u1.y + mid * v1.y;
+        z2 = p2.y + u2.y + mid * v2.y; z3 = p3.y + u3.y + mid * v3.y;
+    } else {
+        r0 = p0.x; r1 = p1.x; r2 = p2.x; r3 = p3.x;
+        z0 = p0.y; z1 = p1.y; z2 = p2.y; z3 = p3.y;
+    }
+    const double a0 = v0.x * strainDt, a1 = v1.x * strainDt, a2 = v2.x * strainDt, a3 = v3.x * strainDt;
+    const double c0 = v0.y * strainDt, c1 = v1.y * strainDt, c2 = v2.y * strainDt, c3 = v3.y * strainDt;
+    double minEdge = NFD_MAX_VALUE;
+    if (d.finite) minEdge = nfd::shortestEdge(minEdge, r0, r1, r2, r3, z0, z1, z2, z3);
+
+    const double J00 = 0.25 * (-r0 + r1 + r2 - r3);
+    const double J01 = 0.25 * (-z0 + z1 + z2 - z3);
+    const double J10 = 0.25 * (-r0 - r1 + r2 + r3);
+    const double J11 = 0.25 * (-z0 - z1 + z2 + z3);
+    const double det = J00 * J11 - J01 * J10;
+    const double inv = 1.0 / det;
+    const double d0r = 0.25 * (J01 - J11) * inv;
+    const double d1r = 0.25 * (J01 + J11) * inv;
+    const double d2r = -d0r;
+    const double d3r = -d1r;
+    const double d0z = 0.25 * (J10 - J00) * inv;
+    const double d1z = -0.25 * (J10 + J00) * inv;
+    const double d2z = -d0z;
+    const double d3z = -d1z;
+    const double dEpsR = d0r * a0 + d1r * a1 + d2r * a2 + d3r * a3;
+    const double dEpsZ = d0z * c0 + d1z * c1 + d2z * c2 + d3z * c3;
+    const double dRdZ = d0z * a0 + d1z * a1 + d2z * a2 + d3z * a3;
+    const double dZdR = d0r * c0 + d1r * c1 + d2r * c2 + d3r * c3;
+    const double dGamRZ = dRdZ + dZdR;
+    double rg = 0.0, dEpsT = 0.0;
+    if (d.axi) {
+        rg = 0.25 * (r0 + r1 + r2 + r3);
+        dEpsT = 0.25 * (a0 + a1 + a2 + a3) / rg;
+    }
+
+    nfd::PointState q = load_point(d, e);
+    if (d.finite) {
+        const double spinDt = 0.5 * (dZdR - dRdZ);
+        if (spinDt != 0.0) {
+            const double cs = nfd::corot_cos(spinDt), sn = nfd::corot_sin(spinDt);
+            nfd::corot_rotate(q.s0, q.s1, q.s3, cs, sn);
+            nfd::corot_rotate(q.b0, q.b1, q.b3, cs, sn);
+        }
+    }
+    nfd::DamagePoint dp;
+    if (DAMAGE) load_damage(d, e, dp);
+    nfd::j2_update<DAMAGE>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp);
+    store_point(d, e, q);
+    if (DAMAGE) {
+        d.dStrength[e] = dp.strength;
+        d.dSoftening[e] = dp.softening;
+        d.dWidth[e] = dp.width;
+    }
+
+    const double sR = q.s0, sZ = q.s1, sT = q.s2, sRZ = q.s3;
+    const double dV = 4.0 * (d.axi ? 2.0 * NFD_PI * rg * det : d.thickness * det);
+    double g0r, g1r, g2r, g3r;
+    if (d.axi) {
+        const double h = sT / rg * dV * 0.25;
+        g0r = -((d0r * sR + d0z * sRZ) * dV + h);
+        g1r = -((d1r * sR + d1z * sRZ) * dV + h);
+        g2r = -((d2r * sR + d2z * sRZ) * dV + h);
+        g3r = -((d3r * sR + d3z * sRZ) * dV + h);
+    } else {
+        g0r = -(d0r * sR + d0z * sRZ) * dV;
+        g1r = -(d1r * sR + d1z * sRZ) * dV;
+        g2r = -(d2r * sR + d2z * sRZ) * dV;
+        g3r = -(d3r * sR + d3z * sRZ) * dV;
+    }
+    double g0z = -(d0z * sZ + d0r * sRZ) * dV;
+    double g1z = -(d1z * sZ + d1r * sRZ) * dV;
+    double g2z = -(d2z * sZ + d2r * sRZ) * dV;
+    double g3z = -(d3z * sZ + d3r * sRZ) * dV;
+    if (d.qhg > 0.0) {
+        const double hr = r0 - r1 + r2 - r3;
+        const double hz = z0 - z1 + z2 - z3;
+        const double y0 = 1.0 - hr * d0r - hz * d0z;
+        const double y1 = -1.0 - hr * d1r - hz * d1z;
+        const double y2 = 1.0 - hr * d2r - hz * d2z;
+        const double y3 = -1.0 - hr * d3r - hz * d3z;
+        const double k = d.qhg * mu * dV *
+                         (d0r * d0r + d1r * d1r + d2r * d2r + d3r * d3r + d0z * d0z + d1z * d1z +
+                          d2z * d2z + d3z * d3z);
+        const double qr = d.hgR[e] + k * (y0 * a0 + y1 * a1 + y2 * a2 + y3 * a3);
+        const double qz = d.hgZ[e] + k * (y0 * c0 + y1 * c1 + y2 * c2 + y3 * c3);
+        d.hgR[e] = qr;
+        d.hgZ[e] = qz;
+        g0r -= y0 * qr; g1r -= y1 * qr; g2r -= y2 * qr; g3r -= y3 * qr;
+        g0z -= y0 * qz; g1z -= y1 * qz; g2z -= y2 * qz; g3z -= y3 * qz;
+        d.hgK[e] = k;
+    }
+    d.slot[4 * e] = make_double2(g0r, g0z);
+    d.slot[4 * e + 1] = make_double2(g1r, g1z);
+    d.slot[4 * e + 2] = make_double2(g2r, g2z);
+    d.slot[4 * e + 3] = make_double2(g3r, g3z);
+    return minEdge;
+}
+
+// One element under full 2x2 integration, solver/ExplicitSolver.java:1219-1339.
+template <bool DAMAGE>
+__device__ double full_element(const Dev& d, int e, int4 n, double strainDt) {
+    const Mat fl = load_mat(d, e);
+    const double mid = -0.5 * strainDt;
+    const double2 p0 = d.x0[n.x], p1 = d.x0[n.y], p2 = d.x0[n.z], p3 = d.x0[n.w];
+    const double2 u0 = d.u[n.x], u1 = d.u[n.y], u2 = d.u[n.z], u3 = d.u[n.w];
+    const double2 v0 = d.v[n.x], v1 = d.v[n.y], v2 = d.v[n.z], v3 = d.v[n.w];
+    double r0, r1, r2, r3, z0, z1, z2, z3;
+    if (d.finite) {
+        r0 = p0.x + u0.x + mid * v0.x; r1 = p1.x + u1.x + mid * v1.x;
+        r2 = p2.x + u2.x + mid * v2.x; r3 = p3.x + u3.x + mid * v3.x;
+        z0 = p0.y + u0.y + mid * v0.y; z1 = p1.y + u1.y + mid * v1.y;
+        z2 = p2.y + u2.y + mid * v2.y; z3 = p3.y + u3.y + mid * v3.y;
+    } else {
+        r0 = p0.x; r1 = p1.x; r2 = p2.x; r3 = p3.x;
+        z0 = p0.y; z1 = p1.y; z2 = p2.y; z3 = p3.y;
+    }
+    const double a0 = v0.x * strainDt, a1 = v1.x * strainDt, a2 = v2.x * strainDt, a3 = v3.x * strainDt;
+    const double c0 = v0.y * strainDt, c1 = v1.y * strainDt, c2 = v2.y * strainDt, c3 = v3.y * strainDt;
+    double minEdge = NFD_MAX_VALUE;
+    if (d.finite) minEdge = nfd::shortestEdge(minEdge, r0, r1, r2, r3, z0, z1, z2, z3);
+
+    const double G = 0.5773502691896258;
+    double g0r = 0, g1r = 0, g2r = 0, g3r = 0;
+    double g0z = 0, g1z = 0, g2z = 0, g3z = 0;
+    for (int g = 0; g < 4; g++) {
+        const double xi = (g == 0 || g == 3) ? -G : G;
+        const double eta = g < 2 ? -G : G;
+        const double xm = 1.0 - xi, xp = 1.0 + xi;
+        const double em = 1.0 - eta, ep = 1.0 + eta;
+        const double N0 = 0.25 * xm * em, N1 = 0.25 * xp * em;
+        const double N2 = 0.25 * xp * ep, N3 = 0.25 * xm * ep;
+        const double dN0x = -0.25 * em, dN1x = 0.25 * em, dN2x = 0.25 * ep, dN3x = -0.25 * ep;
+        const double dN0e = -0.25 * xm, dN1e = -0.25 * xp, dN2e = 0.25 * xp, dN3e = 0.25 * xm;
+        const double J00 = dN0x * r0 + dN1x * r1 + dN2x * r2 + dN3x * r3;
+        const double J01 = dN0x * z0 + dN1x * z1 + dN2x * z2 + dN3x * z3;
+        const double J10 = dN0e * r0 + dN1e * r1 + dN2e * r2 + dN3e * r3;
+        const double J11 = dN0e * z0 + dN1e * z1 + dN2e * z2 + dN3e * z3;
+        const double det = J00 * J11 - J01 * J10;
+        const double inv = 1.0 / det;
+        const double d0r = (J11 * dN0x - J01 * dN0e) * inv;
+        const double d1r = (J11 * dN1x - J01 * dN1e) * inv;
+        const double d2r = (J11 * dN2x - J01 * dN2e) * inv;
+        const double d3r = (J11 * dN3x - J01 * dN3e) * inv;
+        const double d0z = (-J10 * dN0x + J00 * dN0e) * inv;
+        const double d1z = (-J10 * dN1x + J00 * dN1e) * inv;
+        const double d2z = (-J10 * dN2x + J00 * dN2e) * inv;
+        const double d3z = (-J10 * dN3x + J00 * dN3e) * inv;
+        const double dEpsR = d0r * a0 + d1r * a1 + d2r * a2 + d3r * a3;
+        const double dEpsZ = d0z * c0 + d1z * c1 + d2z * c2 + d3z * c3;
+        const double dRdZ = d0z * a0 + d1z * a1 + d2z * a2 + d3z * a3;
+        const double dZdR = d0r * c0 + d1r * c1 + d2r * c2 + d3r * c3;
+        const double dGamRZ = dRdZ + dZdR;
+        double rg = 0.0, dEpsT = 0.0;
+        if (d.axi) {
+            rg = N0 * r0 + N1 * r1 + N2 * r2 + N3 * r3;
+            dEpsT = (N0 * a0 + N1 * a1 + N2 * a2 + N3 * a3) / rg;
+        }
+        const int p = e * 4 + g;
+        nfd::PointState q = load_point(d, p);
+        if (d.finite) {
+            const double spinDt = 0.5 * (dZdR - dRdZ);
+            if (spinDt != 0.0) {
+                const double cs = nfd::corot_cos(spinDt), sn = nfd::corot_sin(spinDt);
+                nfd::corot_rotate(q.s0, q.s1, q.s3, cs, sn);
+                nfd::corot_rotate(q.b0, q.b1, q.b3, cs, sn);
+            }
+        }
+        nfd::DamagePoint dp;
+        if (DAMAGE) load_damage(d, p, dp);
+        nfd::j2_update<DAMAGE>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp);
+        store_point(d, p, q);
+        if (DAMAGE) {
+            d.dStrength[p] = dp.strength;
+            d.dSoftening[p] = dp.softening;
+            d.dWidth[p] = dp.width;
+        }
+        const double sR = q.s0, sZ = q.s1, sT = q.s2, sRZ = q.s3;
+        const double dV = d.axi ? 2.0 * NFD_PI * rg * det : d.thickness * det;
+        if (d.axi) {
+            const double h = sT / rg * dV;
+            g0r -= (d0r * sR + d0z * sRZ) * dV + N0 * h;
+            g1r -= (d1r * sR + d1z * sRZ) * dV + N1 * h;
+            g2r -= (d2r * sR + d2z * sRZ) * dV + N2 * h;
+            g3r -= (d3r * sR + d3z * sRZ) * dV + N3 * h;
+        } else {
+            g0r -= (d0r * sR + d0z * sRZ) * dV;
+            g1r -= (d1r * sR + d1z * sRZ) * dV;
+            g2r -= (d2r * sR + d2z * sRZ) * dV;
+            g3r -= (d3r * sR + d3z * sRZ) * dV;
+        }
+        g0z -= (d0z * sZ + d0r * sRZ) * dV;
+        g1z -= (d1z * sZ + d1r * sRZ) * dV;
+        g2z -= (d2z * sZ + d2r * sRZ) * dV;
+        g3z -= (d3z * sZ + d3r * sRZ) * dV;
+    }
+    d.slot[4 * e] = make_double2(g0r, g0z);
+    d.slot[4 * e + 1] = make_double2(g1r, g1z);
+    d.slot[4 * e + 2] = make_double2(g2r, g2z);
+    d.slot[4 * e + 3] = make_double2(g3r, g3z);
+    return minEdge;
+}
+
+// Pressure edge k, solver/ExplicitSolver.java:1511-1556 (coordinates at the current, not the
+// mid-step, configuration: cr(node, 0.0)).
+__device__ void pressure_edge(const Dev& d, int k, double p) {
+    const int2 ed = d.pedges[k];
+    const size_t s = 4 * ((size_t)d.E + k);
+    const double2 xa = d.x0[ed.x], xb = d.x0[ed.y];
+    double ra, rb, za, zb;
+    if (d.finite) {
+        const double2 ua = d.u[ed.x], ub = d.u[ed.y], va = d.v[ed.x], vb = d.v[ed.y];
+        ra = xa.x + ua.x + 0.0 * va.x;
+        rb = xb.x + ub.x + 0.0 * vb.x;
+        za = xa.y + ua.y + 0.0 * va.y;
+        zb = xb.y + ub.y + 0.0 * vb.y;
+    } else {
+        ra = xa.x; rb = xb.x; za = xa.y; zb = xb.y;
+    }
+    const double tr = rb - ra;
+    const double tz = zb - za;
+    const double len = __builtin_sqrt(tr * tr + tz * tz);
+    if (len == 0.0) {
+        d.slot[s] = make_double2(0.0, 0.0);
+        d.slot[s + 1] = make_double2(0.0, 0.0);
+        return;
+    }
+    const double nr = tz / len;
+    const double nz = -tr / len;
+    double wa, wb;
+    if (d.axi) {
+        const double circumference = 2.0 * NFD_PI * len;
+        wa = circumference * (ra / 3.0 + rb / 6.0);
+        wb = circumference * (ra / 6.0 + rb / 3.0);
+    } else {
+        wa = d.thickness * len * 0.5;
+        wb = wa;
+    }
+    d.slot[s] = make_double2(p * nr * wa, p * nz * wa);
+    d.slot[s + 1] = make_double2(p * nr * wb, p * nz * wb);
+}
+
+template <bool FULL, bool DAMAGE>
+__global__ void __launch_bounds__(BLOCK) k_element(Dev d, int mode) {
+    Ctl* c = d.ctl;
+    if (mode == MODE_STEP && !c->go) return;
+    const double strainDt = mode == MODE_INITIAL ? 0.0 : (mode == MODE_STEP ? c->dtStep : c->dt);
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    double edge = NFD_MAX_VALUE;
+    bool sawNaN = false;
+    double eEnergy = 0.0;
+    int eCount = 0, iCount = 0;
+    bool stale = false;
+    if (e < d.E) {
+        const int4 n = d.conn[e];
+        bool gone = false;
+        if (d.gone) {
+            gone = mode == MODE_STEP ? erode_element(d, e, n, eEnergy, eCount, iCount, stale) : d.gone[e] != 0;
+        }
+        if (gone) {
+            write_zero_slots(d, e);
+        } else {
+            edge = FULL ? full_element<DAMAGE>(d, e, n, strainDt) : reduced_element<DAMAGE>(d, e, n, strainDt);
+            sawNaN = edge != edge;
+        }
+    } else if (e < d.E + d.edges) {
+        const double t = mode == MODE_STEP ? c->timeNext : c->time;
+        pressure_edge(d, e - d.E, pressureAt(d, t));
+    }
+
+    // Shortest edge: exact minimum over positive doubles by their bit patterns.
+    unsigned long long bits = sawNaN ? 0x7FEFFFFFFFFFFFFFULL : (unsigned long long)nfm::bits_of(edge);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o < bits ? o : bits;
+    }
+    const unsigned long long anyNaN = __ballot(sawNaN);
+    __shared__ unsigned long long ldsMin[BLOCK / 64];
+    __shared__ int ldsNaN;
+    if (threadIdx.x == 0) ldsNaN = 0;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        ldsMin[threadIdx.x >> 6] = bits;
+        if (anyNaN) ldsNaN = 1;
+    }
+    if (d.gone && mode == MODE_STEP) {
+        __shared__ double ldsE[4 * 1];
+        double ev[1] = {eEnergy};
+        block_sum<1>(ev, ldsE);
+        int a = eCount, b = iCount, s = stale ? 1 : 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            b += __shfl_xor(b, off, 64);
+            s |= __shfl_xor(s, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0 && (a | b | s)) {
+            atomicAdd(&c->erodedCount, a);
+            atomicAdd(&c->invertedCount, b);
+            if (s) c->surfaceStale = 1;
+        }
+        if (threadIdx.x == 0) {
+            d.erodePartial[blockIdx.x] = ev[0];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = ldsMin[0];
+        for (int w = 1; w < BLOCK / 64; w++) m = ldsMin[w] < m ? ldsMin[w] : m;
+        if (m != 0x7FEFFFFFFFFFFFFFULL) atomicMin(&c->minEdgeBits, m);
+        if (ldsNaN) atomicOr(&c->nanEdge, 1);
+    }
+}
+
+// ------------------------------------------------------------------ gather
+// solver/ExplicitSolver.java:1147-1165 and the step's tail :973-990.
+__global__ void __launch_bounds__(BLOCK) k_gather(Dev d, int mode, int erodeBlocks) {
+    Ctl* c = d.ctl;
+    if (mode == MODE_STEP && !c->go) return;
+    const int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n < d.N) {
+        double sr = 0.0, sz = 0.0;
+        const int end = d.slotStart[n + 1];
+        for (int k = d.slotStart[n]; k < end; k++) {
+            const double2 s = d.slot[d.slotOf[k]];
+            sr += s.x;
+            sz += s.y;
+        }
+        const double m = d.mass[n];
+        d.f[n] = make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double minEdge = c->nanEdge ? NFD_MAX_VALUE : nfm::from_bits(c->minEdgeBits);
+        c->minEdgeBits = 0x7FEFFFFFFFFFFFFFULL;  // Double.MAX_VALUE
+        c->nanEdge = 0;
+        if (mode == MODE_STEP) {
+            c->time = c->timeNext;
+            c->steps = c->steps + 1;
+            if (d.finite) {
+                c->dtPrev = c->dtStep;
+                c->dt = d.cfl * __builtin_sqrt(minEdge * d.massScale) / d.fastestWave;
+            }
+            if (d.gone) {
+                double s = 0.0;
+                for (int b = 0; b < erodeBlocks; b++) s += d.erodePartial[b];
+                c->erodedEnergy += s;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ setup and small kernels
+
+// Row-sum lumped mass (solver/ExplicitSolver.java:1559-1597) as a per-node gather over the
+// CSR, whose element order reproduces the reference's accumulation order (element by element,
+// Gauss point by Gauss point).
+__global__ void k_lump_mass(Dev d, const double* densityOfElement, double* massOut) {
+    const int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= d.N) return;
+    const double G = 0.5773502691896258;
+    double m = 0.0;
+    for (int k = d.slotStart[n]; k < d.slotStart[n + 1]; k++) {
+        const int s = d.slotOf[k];
+        if (s >= 4 * d.E) continue;  // pressure edges carry no mass
+        const int e = s >> 2, corner = s & 3;
+        const int4 c4 = d.conn[e];
+        const double rho = densityOfElement[e];
+        const double2 q0 = d.x0[c4.x], q1 = d.x0[c4.y], q2 = d.x0[c4.z], q3 = d.x0[c4.w];
+        for (int g = 0; g < 4; g++) {
+            const double xi = (g == 0 || g == 3) ? -G : G;
+            const double eta = g < 2 ? -G : G;
+            const double xm = 1.0 - xi, xp = 1.0 + xi;
+            const double em = 1.0 - eta, ep = 1.0 + eta;
+            const double N0 = 0.25 * xm * em, N1 = 0.25 * xp * em;
+            const double N2 = 0.25 * xp * ep, N3 = 0.25 * xm * ep;
+            const double dN0x = -0.25 * em, dN1x = 0.25 * em, dN2x = 0.25 * ep, dN3x = -0.25 * ep;
+            const double dN0e = -0.25 * xm, dN1e = -0.25 * xp, dN2e = 0.25 * xp, dN3e = 0.25 * xm;
+            const double J00 = dN0x * q0.x + dN1x * q1.x + dN2x * q2.x + dN3x * q3.x;
+            const double J01 = dN0x * q0.y + dN1x * q1.y + dN2x * q2.y + dN3x * q3.y;
+            const double J10 = dN0e * q0.x + dN1e * q1.x + dN2e * q2.x + dN3e * q3.x;
+            const double J11 = dN0e * q0.y + dN1e * q1.y + dN2e * q2.y + dN3e * q3.y;
+            const double det = J00 * J11 - J01 * J10;
+            const double rg = N0 * q0.x + N1 * q1.x + N2 * q2.x + N3 * q3.x;
+            const double dV = d.axi ? 2.0 * NFD_PI * rg * det : d.thickness * det;
+            const double Nc = corner == 0 ? N0 : corner == 1 ? N1 : corner == 2 ? N2 : N3;
+            m += rho * Nc * dV;
+        }
+    }
+    massOut[n] = m;
+}
+
+__global__ void k_scale_mass(double* mass, int N, double factor) {
+    const int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n < N) mass[n] *= factor;
+}
+
+__global__ void k_fill2(double2* a, int N, double x, double y) {
+    const int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n < N) a[n] = make_double2(x, y);
+}
+
+// advance() :1069-1075, the displacement half.
+__global__ void k_advance_disp(Dev d) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= d.N) return;
+    const double dt = d.ctl->dt;
+    const uint8_t fl = d.flags[i];
+    double2 ui = d.u[i];
+    const double2 vi = d.v[i];
+    if (!(fl & FIXED_R)) ui.x += vi.x * dt;
+    if (!(fl & FIXED_Z)) ui.y += vi.y * dt;
+    d.u[i] = ui;
+}
+
+// Device self-check of the shared transcendental functions (nf_math.h).
+__global__ void k_math_eval(int fn, const double* x, const double* y, double* out, int n) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double r;
+    switch (fn) {
+        case 0: r = nfm::log(x[i]); break;
+        case 1: r = nfm::exp(x[i]); break;
+        case 2: r = nfm::pow(x[i], y[i]); break;
+        case 3: r = nfm::atan2(x[i], y[i]); break;
+        case 4: r = nfm::sin(x[i]); break;
+        case 5: r = nfm::cos(x[i]); break;
+        case 6: r = nfm::hypot(x[i], y[i]); break;
+        case 7: r = __builtin_sqrt(x[i]); break;
+        case 8: r = x[i] / y[i]; break;
+        default: r = 0.0;
+    }
+    out[i] = r;
+}
+
+inline int blocks_for(int n) { return (n + BLOCK - 1) / BLOCK; }
+
+template <typename T>
+int dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return NF_OK;
+    HIP_TRY(hipMalloc((void**)p, sizeof(T) * count));
+    HIP_TRY(hipMemset(*p, 0, sizeof(T) * count));
+    return NF_OK;
+}
+
+#define TRY(expr)               \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_) return rc_;    \
+    } while (0)
+
+struct HostMaterial {
+    nf_material raw;
+    double lambda() const {
+        const double e = raw.youngs_modulus, nu = raw.poisson_ratio;
+        return e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    }
+    double mu() const { return raw.youngs_modulus / (2.0 * (1.0 + raw.poisson_ratio)); }
+    double wave() const { return std::sqrt((lambda() + 2.0 * mu()) / raw.density); }
+};
+
+int check_material(const nf_material* m) {
+    if (!m) return fail(NF_E_ARGUMENT, "material must not be null");
+    if (m->youngs_modulus <= 0) return fail(NF_E_ARGUMENT, "E must be positive");
+    if (m->poisson_ratio <= -1.0 || m->poisson_ratio >= 0.5)
+        return fail(NF_E_ARGUMENT, "nu must lie in (-1, 0.5)");
+    if (m->density <= 0) return fail(NF_E_ARGUMENT, "density must be positive");
+    if (m->yield_stress <= 0) return fail(NF_E_ARGUMENT, "yield stress must be positive");
+    if (m->isotropic_hardening < 0 || m->kinematic_hardening < 0)
+        return fail(NF_E_ARGUMENT, "hardening moduli must be non-negative");
+    if (m->has_johnson_cook) {
+        const nf_johnson_cook& j = m->johnson_cook;
+        if (j.a <= 0) return fail(NF_E_ARGUMENT, "A must be positive");
+        if (j.b < 0) return fail(NF_E_ARGUMENT, "B must be non-negative");
+        if (j.n <= 0 || j.n > 1) return fail(NF_E_ARGUMENT, "n must lie in (0, 1]");
+        if (j.c < 0) return fail(NF_E_ARGUMENT, "C must be non-negative");
+        if (j.m <= 0) return fail(NF_E_ARGUMENT, "m must be positive");
+        if (j.reference_strain_rate <= 0) return fail(NF_E_ARGUMENT, "reference strain rate must be positive");
+        if (j.melt_temperature <= j.room_temperature)
+            return fail(NF_E_ARGUMENT, "melt temperature must exceed room temperature");
+        if (j.specific_heat <= 0) return fail(NF_E_ARGUMENT, "specific heat must be positive");
+        if (j.taylor_quinney < 0 || j.taylor_quinney > 1)
+            return fail(NF_E_ARGUMENT, "Taylor-Quinney fraction must lie in [0, 1]");
+        if (m->yield_stress != j.a) return fail(NF_E_ARGUMENT, "yield stress disagrees with the Johnson-Cook A");
+    }
+    return NF_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ the handle
+
+struct nf_solver {
+    hipStream_t stream = nullptr;
+    Dev d{};
+    Ctl* ctl = nullptr;
+    int N = 0, E = 0, P = 0, edges = 0, ppe = 1;
+    int formulation = 0, integration = 0, kinematics = 0;
+    double thickness = 1.0, cfl = 0.5;
+    std::vector<HostMaterial> materials;
+    std::vector<int> materialOf;
+    std::vector<double> r, z;
+    std::vector<int> conn, pedges;
+    std::vector<int> slotStart, slotOf;
+    std::vector<uint8_t> flagsHost;  // fixed flags mirror (onWall lives on the device)
+    double fastestWave = 0.0, dt0 = 0.0, massScale = 1.0;
+    bool hasWall = false, hasDamage = false, hasErosion = false, hasContact = false;
+    bool anyJC = false;
+    double wallPlane = 0.0, wallFriction = 0.0;
+    // device buffers owned here
+    std::vector<void*> owned;
+    double* densityOfElement = nullptr;
+
+    ~nf_solver() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* p : owned) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    int alloc(T** p, size_t count) {
+        TRY(dev_alloc(p, count));
+        if (*p) owned.push_back((void*)*p);
+        return NF_OK;
+    }
+
+    double matE(int e) const {
+        return materials[materialOf.empty() ? 0 : materialOf[e]].raw.youngs_modulus;
+    }
+
+    int sync() {
+        HIP_TRY(hipStreamSynchronize(stream));
+        return NF_OK;
+    }
+
+    int readCtl(Ctl& out) {
+        HIP_TRY(hipMemcpyAsync(&out, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return NF_OK;
+    }
+
+    int writeCtl(const Ctl& in) {
+        HIP_TRY(hipMemcpyAsync(ctl, &in, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return NF_OK;
+    }
+
+    int launchElement(int mode) {
+        const int grid = blocks_for(E + edges);
+        const bool full = integration == NF_FULL;
+        if (full && hasDamage) hipLaunchKernelGGL((k_element<true, true>), dim3(grid), dim3(BLOCK), 0, stream, d, mode);
+        else if (full) hipLaunchKernelGGL((k_element<true, false>), dim3(grid), dim3(BLOCK), 0, stream, d, mode);
+        else if (hasDamage) hipLaunchKernelGGL((k_element<false, true>), dim3(grid), dim3(BLOCK), 0, stream, d, mode);
+        else hipLaunchKernelGGL((k_element<false, false>), dim3(grid), dim3(BLOCK), 0, stream, d, mode);
+        HIP_TRY(hipGetLastError());
+        return NF_OK;
+    }
+
+    int launchGather(int mode) {
+        hipLaunchKernelGGL(k_gather, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, mode,
+                           blocks_for(E + edges));
+        HIP_TRY(hipGetLastError());
+        return NF_OK;
+    }
+
+    int launchNodal() {
+        if (hasWall) hipLaunchKernelGGL(k_nodal<true>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d);
+        else hipLaunchKernelGGL(k_nodal<false>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d);
+        HIP_TRY(hipGetLastError());
+        return NF_OK;
+    }
+
+    // advanceMaterialAndAssemble(strainDt) :1081-1115 for the initial and advance modes.
+    int assemble(int mode) {
+        TRY(launchElement(mode));
+        TRY(launchGather(mode));
+        if (hasContact) return fail(NF_E_STATE, "penalty contact is not available in this build");
+        return NF_OK;
+    }
+
+    // Enqueues one full step; the device decides whether it executes.
+    int enqueueStep() {
+        TRY(launchNodal());
+        TRY(launchElement(MODE_STEP));
+        TRY(launchGather(MODE_STEP));
+        return NF_OK;
+    }
+
+    // run steps while time < endTime and fewer than maxSteps have been taken.
+    int runSteps(double endTime, long long maxSteps, long long* taken) {
+        Ctl h;
+        TRY(readCtl(h));
+        const long long start = h.steps;
+        if (taken) *taken = 0;
+        if (!(h.time < endTime) || maxSteps == 0) return NF_OK;
+        if (hasContact) return fail(NF_E_STATE, "penalty contact is not available in this build");
+        if (h.steps == 0) TRY(assemble(MODE_INITIAL));  // :945
+        h.endTime = endTime;
+        h.stepLimit = maxSteps > 0 ? start + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
+        // Only the two request fields change; the rest of the control block is device state.
+        HIP_TRY(hipMemcpyAsync(&ctl->endTime, &h.endTime, sizeof(double), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(&ctl->stepLimit, &h.stepLimit, sizeof(long long), hipMemcpyHostToDevice, stream));
+        const int chunk = 64;
+        while (true) {
+            long long budget = maxSteps > 0 ? (start + maxSteps - h.steps) : chunk;
+            const int n = (int)(budget < chunk ? budget : chunk);
+            for (int k = 0; k < n; k++) TRY(enqueueStep());
+            TRY(readCtl(h));
+            if (!(h.time < endTime)) break;
+            if (maxSteps > 0 && h.steps >= start + maxSteps) break;
+        }
+        if (taken) *taken = h.steps - start;
+        return NF_OK;
+    }
+};
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char* nf_last_error(void) { return g_error.c_str(); }
+
+int nf_material_check(const nf_material* m) {
+    g_error.clear();
+    return check_material(m);
+}
+
+int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const double* z,
+                     int32_t element_count, const int32_t* conn, int32_t pressure_edge_count,
+                     const int32_t* pressure_edges, int32_t material_count,
+                     const nf_material* materials, const int32_t* material_of,
+                     int32_t formulation, int32_t integration, int32_t kinematics,
+                     double thickness, double cfl_safety) {
+    g_error.clear();
+    if (!out) return fail(NF_E_ARGUMENT, "out must not be null");
+    *out = nullptr;
+    if (node_count < 1 || element_count < 1 || !r || !z || !conn)
+        return fail(NF_E_ARGUMENT, "mesh arrays must be non-empty");
+    if (pressure_edge_count < 0 || (pressure_edge_count > 0 && !pressure_edges))
+        return fail(NF_E_ARGUMENT, "bad pressure edges");
+    if (material_count < 1 || !materials) return fail(NF_E_ARGUMENT, "need at least one material");
+    if (!material_of && material_count != 1) return fail(NF_E_ARGUMENT, "a uniform mesh takes exactly one material");
+    if (formulation < 0 || formulation > 2 || integration < 0 || integration > 1 || kinematics < 0 || kinematics > 1)
+        return fail(NF_E_ARGUMENT, "unknown formulation, integration or kinematics");
+    if ((long long)element_count * 4 + 2LL * pressure_edge_count > 0x7FFFFFFFLL)
+        return fail(NF_E_ARGUMENT, "mesh too large for 32-bit slot indices");
+    for (long long i = 0; i < 4LL * element_count; i++)
+        if (conn[i] < 0 || conn[i] >= node_count) return fail(NF_E_ARGUMENT, "connectivity out of range");
+    for (long long i = 0; i < 2LL * pressure_edge_count; i++)
+        if (pressure_edges[i] < 0 || pressure_edges[i] >= node_count) return fail(NF_E_ARGUMENT, "pressure edge out of range");
+    if (material_of)
+        for (int e = 0; e < element_count; e++)
+            if (material_of[e] < 0 || material_of[e] >= material_count) return fail(NF_E_ARGUMENT, "material index out of range");
+    for (int m = 0; m < material_count; m++) {
+        TRY(check_material(&materials[m]));
+        if (formulation == NF_PLANE_STRESS && !std::isinf(materials[m].yield_stress))
+            return fail(NF_E_ARGUMENT, "plane stress plasticity needs its own return map; not implemented");
+    }
+    // QuadMesh.checkOrientation mesh/QuadMesh.java:232-240
+    for (int e = 0; e < element_count; e++) {
+        double a = 0.0;
+        for (int k = 0; k < 4; k++) {
+            const int i = conn[e * 4 + k], j = conn[e * 4 + (k + 1) % 4];
+            a += r[i] * z[j] - r[j] * z[i];
+        }
+        if (0.5 * a <= 0.0)
+            return fail(NF_E_STATE, "element " + std::to_string(e) +
+                                        " has non-positive area; node ordering must be counter-clockwise in (r, z)");
+    }
+
+    nf_solver* s = new nf_solver();
+    auto bail = [&](int rc) {
+        delete s;
+        return rc;
+    };
+    int rc;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(NF_E_DEVICE, "hipStreamCreate failed: no usable GPU"));
+    const int N = node_count, E = element_count;
+    s->N = N;
+    s->E = E;
+    s->edges = pressure_edge_count;
+    s->ppe = integration == NF_FULL ? 4 : 1;
+    s->P = E * s->ppe;
+    s->formulation = formulation;
+    s->integration = integration;
+    s->kinematics = kinematics;
+    s->thickness = thickness;
+    s->cfl = cfl_safety;
+    s->r.assign(r, r + N);
+    s->z.assign(z, z + N);
+    s->conn.assign(conn, conn + 4 * (size_t)E);
+    if (pressure_edge_count) s->pedges.assign(pressure_edges, pressure_edges + 2 * (size_t)pressure_edge_count);
+    for (int m = 0; m < material_count; m++) s->materials.push_back({materials[m]});
+    if (material_of) s->materialOf.assign(material_of, material_of + E);
+    s->flagsHost.assign(N, 0);
+
+    // Material table (solver/ExplicitSolver.java:289-299, J2.Flow.of).
+    std::vector<Mat> table(material_count);
+    for (int m = 0; m < material_count; m++) {
+        const HostMaterial& hm = s->materials[m];
+        Mat& t = table[m];
+        std::memset(&t, 0, sizeof t);
+        const double lam = formulation == NF_PLANE_STRESS
+                               ? 2.0 * hm.lambda() * hm.mu() / (hm.lambda() + 2.0 * hm.mu())
+                               : hm.lambda();
+        t.lambda = lam;
+        t.mu = hm.mu();
+        t.bulk = lam + 2.0 * hm.mu() / 3.0;
+        t.density = hm.raw.density;
+        t.sy0 = hm.raw.yield_stress;
+        t.hIso = hm.raw.isotropic_hardening;
+        t.hKin = hm.raw.kinematic_hardening;
+        t.youngs = hm.raw.youngs_modulus;
+        t.hasJC = hm.raw.has_johnson_cook;
+        if (t.hasJC) {
+            const nf_johnson_cook& j = hm.raw.johnson_cook;
+            t.a = j.a; t.b = j.b; t.n = j.n; t.c = j.c; t.m = j.m;
+            t.eps0 = j.reference_strain_rate; t.troom = j.room_temperature;
+            t.tmelt = j.melt_temperature; t.cp = j.specific_heat; t.beta = j.taylor_quinney;
+            t.thermalCoefficient = j.taylor_quinney / (hm.raw.density * j.specific_heat);
+            s->anyJC = true;
+        }
+    }
+    // Materials.fastestWave core/Materials.java:120-128
+    if (s->materialOf.empty()) {
+        s->fastestWave = s->materials[0].wave();
+    } else {
+        double fastest = 0.0;
+        for (int e = 0; e < E; e++) {
+            const double c = s->materials[s->materialOf[e]].wave();
+            if (c > fastest) fastest = c;
+        }
+        s->fastestWave = fastest;
+    }
+
+    // CSR of force slots per node (buildNodeSlots :347-366): element corners in ascending
+    // element order, then pressure edges. Slot index = 4 e + corner, or 4 (E + k) + end.
+    s->slotStart.assign(N + 1, 0);
+    for (size_t k = 0; k < s->conn.size(); k++) s->slotStart[s->conn[k] + 1]++;
+    for (size_t k = 0; k < s->pedges.size(); k++) s->slotStart[s->pedges[k] + 1]++;
+    for (int i = 0; i < N; i++) s->slotStart[i + 1] += s->slotStart[i];
+    s->slotOf.assign(s->conn.size() + s->pedges.size(), 0);
+    {
+        std::vector<int> cursor(s->slotStart.begin(), s->slotStart.end() - 1);
+        for (int e = 0; e < E; e++)
+            for (int c = 0; c < 4; c++) s->slotOf[cursor[s->conn[e * 4 + c]]++] = 4 * e + c;
+        for (int k = 0; k < pressure_edge_count; k++) {
+            s->slotOf[cursor[s->pedges[2 * k]]++] = 4 * (E + k);
+            s->slotOf[cursor[s->pedges[2 * k + 1]]++] = 4 * (E + k) + 1;
+        }
+    }
+
+    Dev& d = s->d;
+    d.N = N;
+    d.E = E;
+    d.P = s->P;
+    d.edges = pressure_edge_count;
+    d.ppe = s->ppe;
+    d.axi = formulation == NF_AXISYMMETRIC;
+    d.finite = kinematics == NF_FINITE_STRAIN;
+    d.thickness = thickness;
+    d.qhg = 0.05;  // DEFAULT_HOURGLASS_COEFFICIENT :59
+    d.cfl = cfl_safety;
+    d.fastestWave = s->fastestWave;
+    d.massScale = 1.0;
+    d.erosionDamage = HUGE_VAL;
+
+    double2* x0 = nullptr;
+    double* mass = nullptr;
+    int4* conn4 = nullptr;
+    Mat* mats = nullptr;
+    int* slotStart = nullptr;
+    int* slotOf = nullptr;
+    if ((rc = s->alloc(&s->ctl, 1)) || (rc = s->alloc(&x0, N)) || (rc = s->alloc(&d.u, N)) ||
+        (rc = s->alloc(&d.v, N)) || (rc = s->alloc(&d.f, N)) || (rc = s->alloc(&mass, N)) ||
+        (rc = s->alloc(&d.flags, N)) || (rc = s->alloc(&conn4, E)) ||
+        (rc = s->alloc(&mats, material_count)) || (rc = s->alloc(&d.sig0, s->P)) ||
+        (rc = s->alloc(&d.sig1, s->P)) || (rc = s->alloc(&d.sig2, s->P)) || (rc = s->alloc(&d.sig3, s->P)) ||
+        (rc = s->alloc(&d.back0, s->P)) || (rc = s->alloc(&d.back1, s->P)) ||
+        (rc = s->alloc(&d.back2, s->P)) || (rc = s->alloc(&d.back3, s->P)) ||
+        (rc = s->alloc(&d.epsP, s->P)) || (rc = s->alloc(&d.work, s->P)) ||
+        (rc = s->alloc(&d.hgR, E)) || (rc = s->alloc(&d.hgZ, E)) || (rc = s->alloc(&d.hgK, E)) ||
+        (rc = s->alloc(&d.slot, 4 * ((size_t)E + pressure_edge_count))) ||
+        (rc = s->alloc(&slotStart, N + 1)) || (rc = s->alloc(&slotOf, s->slotOf.size())) ||
+        (rc = s->alloc(&d.wallPartial, 3 * (size_t)blocks_for(N))) ||
+        (rc = s->alloc(&d.wallCountPartial, 2 * (size_t)blocks_for(N))) ||
+        (rc = s->alloc(&d.erodePartial, (size_t)blocks_for(E + pressure_edge_count))) ||
+        (rc = s->alloc(&s->densityOfElement, E)))
+        return bail(rc);
+    d.ctl = s->ctl;
+    d.x0 = x0;
+    d.mass = mass;
+    d.conn = conn4;
+    d.mats = mats;
+    d.slotStart = slotStart;
+    d.slotOf = slotOf;
+    if (pressure_edge_count) {
+        int2* pe = nullptr;
+        if ((rc = s->alloc(&pe, pressure_edge_count))) return bail(rc);
+        d.pedges = pe;
+        if (hipMemcpy(pe, s->pedges.data(), sizeof(int2) * pressure_edge_count, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(NF_E_DEVICE, "upload failed"));
+    }
+    if (material_of) {
+        int* mi = nullptr;
+        if ((rc = s->alloc(&mi, E))) return bail(rc);
+        d.matIndex = mi;
+        if (hipMemcpy(mi, material_of, sizeof(int) * E, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(NF_E_DEVICE, "upload failed"));
+    }
+    std::vector<double2> x0h(N);
+    for (int i = 0; i < N; i++) x0h[i] = make_double2(r[i], z[i]);
+    std::vector<double> dens(E);
+    for (int e = 0; e < E; e++) dens[e] = table[s->materialOf.empty() ? 0 : s->materialOf[e]].density;
+    if (hipMemcpy(x0, x0h.data(), sizeof(double2) * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(conn4, conn, sizeof(int4) * E, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(mats, table.data(), sizeof(Mat) * material_count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(slotStart, s->slotStart.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(slotOf, s->slotOf.data(), sizeof(int) * s->slotOf.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->densityOfElement, dens.data(), sizeof(double) * E, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(NF_E_DEVICE, "upload failed"));
+    hipLaunchKernelGGL(k_lump_mass, dim3(blocks_for(N)), dim3(BLOCK), 0, s->stream, d,
+                       (const double*)s->densityOfElement, mass);
+    if (hipGetLastError() != hipSuccess) return bail(fail(NF_E_DEVICE, "mass lumping launch failed"));
+
+    // dt0 = cfl * minEdge / c (:334-338), QuadMesh.minimumEdgeLength :61-75
+    double minEdge = 1.7976931348623157e308;
+    for (int e = 0; e < E; e++)
+        for (int k = 0; k < 4; k++) {
+            const int i = conn[e * 4 + k], j = conn[e * 4 + (k + 1) % 4];
+            const double dr = r[j] - r[i], dz = z[j] - z[i];
+            const double len = std::sqrt(dr * dr + dz * dz);
+            if (len < minEdge) minEdge = len;
+        }
+    Ctl h;
+    std::memset(&h, 0, sizeof h);
+    h.dt = cfl_safety * minEdge / s->fastestWave;
+    h.dtPrev = h.dt;
+    h.minEdgeBits = 0x7FEFFFFFFFFFFFFFULL;
+    h.stepLimit = 0;
+    s->dt0 = h.dt;
+    if ((rc = s->writeCtl(h))) return bail(rc);
+    *out = s;
+    return NF_OK;
+}
+
+int nf_solver_destroy(nf_solver* s) {
+    delete s;
+    return NF_OK;
+}
+
+int nf_set_threads(nf_solver* s, int32_t n) {
+    g_error.clear();
+    if (n < 1) return fail(NF_E_ARGUMENT, "need at least one thread");
+    return NF_OK;  // one GPU does the work; the answer never depended on the count
+}
+
+static int upload_flags(nf_solver* s) {
+    // Keep the device's ON_WALL bits; replace the fixed bits.
+    std::vector<uint8_t> cur(s->N);
+    HIP_TRY(hipMemcpyAsync(cur.data(), s->d.flags, s->N, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < s->N; i++) cur[i] = (uint8_t)((cur[i] & ON_WALL) | s->flagsHost[i]);
+    HIP_TRY(hipMemcpyAsync(s->d.flags, cur.data(), s->N, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return NF_OK;
+}
+
+int nf_fix_radial(nf_solver* s, int32_t node) {
+    g_error.clear();
+    if (node < 0 || node >= s->N) return fail(NF_E_ARGUMENT, "node out of range");
+    s->flagsHost[node] |= FIXED_R;
+    return upload_flags(s);
+}
+int nf_fix_axial(nf_solver* s, int32_t node) {
+    g_error.clear();
+    if (node < 0 || node >= s->N) return fail(NF_E_ARGUMENT, "node out of range");
+    s->flagsHost[node] |= FIXED_Z;
+    return upload_flags(s);
+}
+int nf_fix_all_axial(nf_solver* s) {
+    for (auto& f : s->flagsHost) f |= FIXED_Z;
+    return upload_flags(s);
+}
+int nf_set_pressure_ramp(nf_solver* s, double pascals, double ramp) {
+    s->d.targetPressure = pascals;
+    s->d.rampDuration = ramp;
+    return NF_OK;
+}
+int nf_set_relaxation_damping(nf_solver* s, double fraction, double omega) {
+    g_error.clear();
+    if (omega <= 0.0) return fail(NF_E_ARGUMENT, "angular frequency must be positive");
+    s->d.damping = 2.0 * fraction * omega;
+    return NF_OK;
+}
+int nf_set_hourglass_coefficient(nf_solver* s, double q) {
+    g_error.clear();
+    if (q < 0.0) return fail(NF_E_ARGUMENT, "hourglass coefficient must be >= 0");
+    s->d.qhg = q;
+    return NF_OK;
+}
+int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double friction) {
+    g_error.clear();
+    if (!enabled) {
+        s->hasWall = false;
+        s->d.wall = 0;
+        return NF_OK;
+    }
+    if (friction < 0.0 || friction != friction) return fail(NF_E_ARGUMENT, "friction coefficient must be >= 0");
+    s->hasWall = true;
+    s->d.wall = 1;
+    s->d.plane = plane_z;
+    s->d.friction = friction;
+    s->d.wallFrictional = friction > 0.0;
+    // A new RigidWall starts with zero audits (solver/RigidWall.java:103-107).
+    Ctl h;
+    TRY(s->readCtl(h));
+    h.wallImpulse = h.wallEnergyLoss = h.wallFriction = 0.0;
+    h.wallContact = h.wallStuck = 0;
+    return s->writeCtl(h);
+}
+int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double damping, double scale) {
+    g_error.clear();
+    if (!enabled) {
+        s->hasContact = false;
+        return NF_OK;
+    }
+    if (friction < 0.0) return fail(NF_E_ARGUMENT, "friction must not be negative");
+    if (damping < 0.0 || damping > 1.0) return fail(NF_E_ARGUMENT, "contact damping is a fraction of critical, so it lies in [0, 1]");
+    if (!(scale > 0.0) || scale >= 2.0) return fail(NF_E_ARGUMENT, "the stiffness scale must lie in (0, 2)");
+    return fail(NF_E_STATE, "penalty contact is not available in this build");
+}
+int nf_set_gravity(nf_solver* s, double ar, double az) {
+    s->d.gravityR = ar;
+    s->d.gravityZ = az;
+    return NF_OK;
+}
+int nf_set_damage(nf_solver* s, const double* gf, const double* eps) {  // :729-791
+    g_error.clear();
+    if (!gf || !eps) return fail(NF_E_ARGUMENT, "fracture energy and failure strain are required");
+    for (const HostMaterial& m : s->materials) {
+        const double worst = 2.0 * m.mu() + (2.0 / 3.0) * (m.raw.kinematic_hardening - m.raw.youngs_modulus);
+        if (!(worst > 0.0)) return fail(NF_E_ARGUMENT, "a fully clamped softening modulus would make the return map singular");
+    }
+    const int E = s->E, ppe = s->ppe, P = s->P;
+    std::vector<double> fe(P), ym(P), fs(P), geom(5 * (size_t)P);
+    for (int e = 0; e < E; e++) {
+        const int* c = &s->conn[4 * e];
+        const double uR = 0.5 * (s->r[c[1]] - s->r[c[0]] + s->r[c[2]] - s->r[c[3]]);
+        const double uZ = 0.5 * (s->z[c[1]] - s->z[c[0]] + s->z[c[2]] - s->z[c[3]]);
+        const double vR = 0.5 * (s->r[c[3]] - s->r[c[0]] + s->r[c[2]] - s->r[c[1]]);
+        const double vZ = 0.5 * (s->z[c[3]] - s->z[c[0]] + s->z[c[2]] - s->z[c[1]]);
+        double a = 0.0;
+        for (int k = 0; k < 4; k++) {
+            const int i = c[k], j = c[(k + 1) % 4];
+            a += s->r[i] * s->z[j] - s->r[j] * s->z[i];
+        }
+        const double area = std::fabs(0.5 * a);
+        for (int g = 0; g < ppe; g++) {
+            const int p = e * ppe + g;
+            fs[p] = eps[e];
+            fe[p] = gf[e];
+            ym[p] = s->matE(e);
+            geom[5 * (size_t)p] = area;
+            geom[5 * (size_t)p + 1] = uR;
+            geom[5 * (size_t)p + 2] = uZ;
+            geom[5 * (size_t)p + 3] = vR;
+            geom[5 * (size_t)p + 4] = vZ;
+        }
+    }
+    for (int p = 0; p < P; p++) {
+        if (!(fe[p] > 0.0)) return fail(NF_E_ARGUMENT, "fracture energy must be positive");
+        if (!(ym[p] > 0.0)) return fail(NF_E_ARGUMENT, "Young's modulus must be positive");
+    }
+    for (int p = 0; p < P; p++) {
+        if (!(fs[p] > 0.0)) return fail(NF_E_ARGUMENT, "failure strain must be positive");
+        if (!(geom[5 * (size_t)p] > 0.0)) return fail(NF_E_ARGUMENT, "area must be positive");
+    }
+    Dev& d = s->d;
+    if (!d.dStrength) {
+        double *a, *b, *c2, *f1, *f2, *f3, *g;
+        TRY(s->alloc(&a, P));
+        TRY(s->alloc(&b, P));
+        TRY(s->alloc(&c2, P));
+        TRY(s->alloc(&f1, P));
+        TRY(s->alloc(&f2, P));
+        TRY(s->alloc(&f3, P));
+        TRY(s->alloc(&g, 5 * (size_t)P));
+        d.dStrength = a; d.dSoftening = b; d.dWidth = c2;
+        d.dFailure = f1; d.dGf = f2; d.dYoungs = f3; d.dGeom = g;
+    } else {
+        HIP_TRY(hipMemsetAsync(d.dStrength, 0, sizeof(double) * P, s->stream));
+        HIP_TRY(hipMemsetAsync(d.dSoftening, 0, sizeof(double) * P, s->stream));
+        HIP_TRY(hipMemsetAsync(d.dWidth, 0, sizeof(double) * P, s->stream));
+    }
+    HIP_TRY(hipMemcpyAsync((void*)d.dFailure, fs.data(), sizeof(double) * P, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync((void*)d.dGf, fe.data(), sizeof(double) * P, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync((void*)d.dYoungs, ym.data(), sizeof(double) * P, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync((void*)d.dGeom, geom.data(), sizeof(double) * 5 * P, hipMemcpyHostToDevice, s->stream));
+    TRY(s->sync());
+    s->hasDamage = true;
+    return NF_OK;
+}
+int nf_set_erosion(nf_solver* s, double threshold) {  // :549-561
+    g_error.clear();
+    if (!s->hasDamage) return fail(NF_E_STATE, "erosion needs a failure model; call setDamage first");
+    if (!(threshold > 0.0) || threshold > 1.0)
+        return fail(NF_E_ARGUMENT, "the erosion threshold is a damage, so it lies in (0, 1]");
+    s->d.erosionDamage = threshold;
+    if (!s->d.gone) {
+        uint8_t* g = nullptr;
+        TRY(s->alloc(&g, s->E));
+        s->d.gone = g;
+    }
+    s->hasErosion = true;
+    return NF_OK;
+}
+int nf_set_mass_scaling(nf_solver* s, double target) {  // :601-619
+    g_error.clear();
+    if (!(target > 0.0)) return fail(NF_E_ARGUMENT, "the target step must be positive");
+    Ctl h;
+    TRY(s->readCtl(h));
+    if (target < h.dt) return fail(NF_E_ARGUMENT, "mass scaling adds mass and cannot shorten a step");
+    const double factor = (target / h.dt) * (target / h.dt);
+    hipLaunchKernelGGL(k_scale_mass, dim3(blocks_for(s->N)), dim3(BLOCK), 0, s->stream,
+                       (double*)s->d.mass, s->N, factor);
+    HIP_TRY(hipGetLastError());
+    s->massScale *= factor;
+    s->d.massScale = s->massScale;
+    h.dt = target;
+    h.dtPrev = target;
+    s->dt0 = target;
+    return s->writeCtl(h);
+}
+int nf_set_velocity(nf_solver* s, int32_t node, double vr, double vz) {
+    g_error.clear();
+    if (node < 0 || node >= s->N) return fail(NF_E_ARGUMENT, "node out of range");
+    const double2 v = make_double2(vr, vz);
+    HIP_TRY(hipMemcpyAsync(s->d.v + node, &v, sizeof v, hipMemcpyHostToDevice, s->stream));
+    return s->sync();
+}
+int nf_set_uniform_velocity(nf_solver* s, double vr, double vz) {
+    hipLaunchKernelGGL(k_fill2, dim3(blocks_for(s->N)), dim3(BLOCK), 0, s->stream, s->d.v, s->N, vr, vz);
+    HIP_TRY(hipGetLastError());
+    return s->sync();
+}
+int nf_run(nf_solver* s, int64_t steps) {
+    g_error.clear();
+    if (steps <= 0) return NF_OK;
+    long long taken = 0;
+    return s->runSteps(HUGE_VAL, steps, &taken);
+}
+int nf_run_until(nf_solver* s, double end_time, int64_t max_steps, int64_t* steps_taken) {
+    g_error.clear();
+    long long taken = 0;
+    TRY(s->runSteps(end_time, max_steps > 0 ? max_steps : -1, &taken));
+    if (steps_taken) *steps_taken = taken;
+    return NF_OK;
+}
+int nf_advance(nf_solver* s) {
+    g_error.clear();
+    hipLaunchKernelGGL(k_advance_disp, dim3(blocks_for(s->N)), dim3(BLOCK), 0, s->stream, s->d);
+    HIP_TRY(hipGetLastError());
+    TRY(s->assemble(MODE_ADVANCE));
+    return s->sync();
+}
+
+int nf_get_node_field(nf_solver* s, int32_t field, double* out) {
+    g_error.clear();
+    const int N = s->N;
+    if (field < 0 || field >= NF_NODE_FIELD_COUNT) return fail(NF_E_ARGUMENT, "unknown node field");
+    if (field == NF_NODE_R || field == NF_NODE_Z) {
+        std::memcpy(out, field == NF_NODE_R ? s->r.data() : s->z.data(), sizeof(double) * N);
+        return NF_OK;
+    }
+    if (field == NF_NODE_MASS) {
+        HIP_TRY(hipMemcpyAsync(out, s->d.mass, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+        return s->sync();
+    }
+    if (field >= NF_NODE_ON_WALL) {
+        std::vector<uint8_t> fl(N);
+        HIP_TRY(hipMemcpyAsync(fl.data(), s->d.flags, N, hipMemcpyDeviceToHost, s->stream));
+        TRY(s->sync());
+        const uint8_t bit = field == NF_NODE_ON_WALL ? ON_WALL : field == NF_NODE_FIXED_R ? FIXED_R : FIXED_Z;
+        for (int i = 0; i < N; i++) out[i] = (fl[i] & bit) ? 1.0 : 0.0;
+        return NF_OK;
+    }
+    const double2* src = field <= NF_NODE_UZ ? s->d.u : field <= NF_NODE_VZ ? s->d.v : s->d.f;
+    std::vector<double2> tmp(N);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), src, sizeof(double2) * N, hipMemcpyDeviceToHost, s->stream));
+    TRY(s->sync());
+    const bool second = (field & 1) != 0;  // UR/VR/FR even, UZ/VZ/FZ odd
+    for (int i = 0; i < N; i++) out[i] = second ? tmp[i].y : tmp[i].x;
+    return NF_OK;
+}
+
+int nf_put_node_field(nf_solver* s, int32_t field, const double* in) {
+    g_error.clear();
+    if (field < NF_NODE_UR || field > NF_NODE_VZ) return fail(NF_E_ARGUMENT, "only UR, UZ, VR and VZ may be written");
+    double2* dst = field <= NF_NODE_UZ ? s->d.u : s->d.v;
+    std::vector<double2> tmp(s->N);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), dst, sizeof(double2) * s->N, hipMemcpyDeviceToHost, s->stream));
+    TRY(s->sync());
+    const bool second = (field & 1) != 0;
+    for (int i = 0; i < s->N; i++) (second ? tmp[i].y : tmp[i].x) = in[i];
+    HIP_TRY(hipMemcpyAsync(dst, tmp.data(), sizeof(double2) * s->N, hipMemcpyHostToDevice, s->stream));
+    return s->sync();
+}
+
+int nf_get_point_field(nf_solver* s, int32_t field, double* out) {
+    g_error.clear();
+    const int P = s->P;
+    const Dev& d = s->d;
+    auto copy = [&](const double* src, double* dst) -> int {
+        HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * P, hipMemcpyDeviceToHost, s->stream));
+        return NF_OK;
+    };
+    if (field == NF_POINT_STRESS || field == NF_POINT_BACK) {
+        const bool st = field == NF_POINT_STRESS;
+        std::vector<double> c0(P), c1(P), c2(P), c3(P);
+        TRY(copy(st ? d.sig0 : d.back0, c0.data()));
+        TRY(copy(st ? d.sig1 : d.back1, c1.data()));
+        TRY(copy(st ? d.sig2 : d.back2, c2.data()));
+        TRY(copy(st ? d.sig3 : d.back3, c3.data()));
+        TRY(s->sync());
+        for (int p = 0; p < P; p++) {
+            out[4 * p] = c0[p];
+            out[4 * p + 1] = c1[p];
+            out[4 * p + 2] = c2[p];
+            out[4 * p + 3] = c3[p];
+        }
+        return NF_OK;
+    }
+    const double* src = nullptr;
+    switch (field) {
+        case NF_POINT_PLASTIC_STRAIN: src = d.epsP; break;
+        case NF_POINT_PLASTIC_WORK: src = d.work; break;
+        case NF_POINT_DAMAGE_STRENGTH: src = d.dStrength; break;
+        case NF_POINT_DAMAGE_SOFTENING: src = d.dSoftening; break;
+        case NF_POINT_DAMAGE_WIDTH: src = d.dWidth; break;
+        default: return fail(NF_E_ARGUMENT, "unknown point field");
+    }
+    if (!src) {
+        std::memset(out, 0, sizeof(double) * P);
+        return NF_OK;
+    }
+    TRY(copy(src, out));
+    return s->sync();
+}
+
+int nf_get_element_field(nf_solver* s, int32_t field, double* out) {
+    g_error.clear();
+    const int E = s->E;
+    const double* src = nullptr;
+    switch (field) {
+        case NF_ELEMENT_HG_R: src = s->d.hgR; break;
+        case NF_ELEMENT_HG_Z: src = s->d.hgZ; break;
+        case NF_ELEMENT_HG_STIFFNESS: src = s->d.hgK; break;
+        case NF_ELEMENT_ERODED: {
+            if (!s->d.gone) {
+                std::memset(out, 0, sizeof(double) * E);
+                return NF_OK;
+            }
+            std::vector<uint8_t> g(E);
+            HIP_TRY(hipMemcpyAsync(g.data(), s->d.gone, E, hipMemcpyDeviceToHost, s->stream));
+            TRY(s->sync());
+            for (int e = 0; e < E; e++) out[e] = g[e] ? 1.0 : 0.0;
+            return NF_OK;
+        }
+        default: return fail(NF_E_ARGUMENT, "unknown element field");
+    }
+    HIP_TRY(hipMemcpyAsync(out, src, sizeof(double) * E, hipMemcpyDeviceToHost, s->stream));
+    return s->sync();
+}
+
+int nf_get_state(nf_solver* s, nf_state* o) {
+    g_error.clear();
+    Ctl h;
+    TRY(s->readCtl(h));
+    std::memset(o, 0, sizeof *o);
+    o->time = h.time;
+    o->dt = h.dt;
+    o->dt_previous = h.dtPrev;
+    o->dt0 = s->dt0;
+    o->mass_scale = s->massScale;
+    o->steps = h.steps;
+    o->node_count = s->N;
+    o->element_count = s->E;
+    o->points_per_element = s->ppe;
+    o->eroded_elements = h.erodedCount;
+    o->inverted_elements = h.invertedCount;
+    o->eroded_energy = h.erodedEnergy;
+    if (s->hasWall) {
+        o->wall_contact_count = h.wallContact;
+        o->wall_stuck_count = h.wallStuck;
+        o->wall_impulse = h.wallImpulse;
+        o->wall_energy_loss = h.wallEnergyLoss;
+        o->wall_friction_dissipation = h.wallFriction;
+    }
+    return NF_OK;
+}
+
+// Audits from the device state, in the reference's summation order
+// (solver/ExplicitSolver.java:875-886, :1610-1860).
+int nf_get_audits(nf_solver* s, nf_audits* o) {
+    g_error.clear();
+    std::memset(o, 0, sizeof *o);
+    const int N = s->N, E = s->E, ppe = s->ppe, P = s->P;
+    std::vector<double> mass(N), ur(N), uz(N), vr(N), vz(N), fr(N), fz(N), onw(N), fxr(N), fxz(N);
+    TRY(nf_get_node_field(s, NF_NODE_MASS, mass.data()));
+    TRY(nf_get_node_field(s, NF_NODE_VR, vr.data()));
+    TRY(nf_get_node_field(s, NF_NODE_VZ, vz.data()));
+    TRY(nf_get_node_field(s, NF_NODE_FR, fr.data()));
+    TRY(nf_get_node_field(s, NF_NODE_FZ, fz.data()));
+    TRY(nf_get_node_field(s, NF_NODE_ON_WALL, onw.data()));
+    TRY(nf_get_node_field(s, NF_NODE_FIXED_R, fxr.data()));
+    TRY(nf_get_node_field(s, NF_NODE_FIXED_Z, fxz.data()));
+    std::vector<double> sig(4 * (size_t)P), work(P), epsP(P), hgR(E), hgZ(E), hgK(E), gone(E);
+    TRY(nf_get_point_field(s, NF_POINT_STRESS, sig.data()));
+    TRY(nf_get_point_field(s, NF_POINT_PLASTIC_WORK, work.data()));
+    TRY(nf_get_point_field(s, NF_POINT_PLASTIC_STRAIN, epsP.data()));
+    TRY(nf_get_element_field(s, NF_ELEMENT_HG_R, hgR.data()));
+    TRY(nf_get_element_field(s, NF_ELEMENT_HG_Z, hgZ.data()));
+    TRY(nf_get_element_field(s, NF_ELEMENT_HG_STIFFNESS, hgK.data()));
+    TRY(nf_get_element_field(s, NF_ELEMENT_ERODED, gone.data()));
+    Ctl h;
+    TRY(s->readCtl(h));
+
+    double p = 0.0;
+    for (int i = 0; i < N; i++) p += mass[i] * vz[i];
+    o->axial_momentum = p;
+    p = 0.0;
+    for (int i = 0; i < N; i++) p += mass[i] * vr[i];
+    o->radial_momentum = p;
+    double ke = 0.0;
+    for (int i = 0; i < N; i++) ke += mass[i] * (vr[i] * vr[i] + vz[i] * vz[i]);
+    o->kinetic_energy = 0.5 * ke;
+    ke = 0.0;
+    const double hdt = 0.5 * h.dt;
+    for (int i = 0; i < N; i++) {
+        const double invM = 1.0 / mass[i];
+        const bool held = s->hasWall && onw[i] != 0.0 && fz[i] <= 0.0;
+        const double cvr = fxr[i] != 0.0 ? vr[i] : vr[i] + hdt * fr[i] * invM;
+        const double cvz = fxz[i] != 0.0 || held ? vz[i] : vz[i] + hdt * fz[i] * invM;
+        ke += 0.5 * mass[i] * (cvr * cvr + cvz * cvz);
+    }
+    o->centred_kinetic_energy = ke;
+
+    auto matOf = [&](int e) -> const HostMaterial& { return s->materials[s->materialOf.empty() ? 0 : s->materialOf[e]]; };
+    auto gaussVolume = [&](int e, int g) {
+        const int* c = &s->conn[4 * e];
+        const double r0 = s->r[c[0]], r1 = s->r[c[1]], r2 = s->r[c[2]], r3 = s->r[c[3]];
+        const double z0 = s->z[c[0]], z1 = s->z[c[1]], z2 = s->z[c[2]], z3 = s->z[c[3]];
+        const double G = 0.5773502691896258;
+        const double GX[4] = {-G, G, G, -G}, GE[4] = {-G, -G, G, G};
+        const double xi = ppe == 1 ? 0.0 : GX[g];
+        const double eta = ppe == 1 ? 0.0 : GE[g];
+        const double weight = ppe == 1 ? 4.0 : 1.0;
+        const double xm = 1.0 - xi, xp = 1.0 + xi, em = 1.0 - eta, ep = 1.0 + eta;
+        const double N0 = 0.25 * xm * em, N1 = 0.25 * xp * em, N2 = 0.25 * xp * ep, N3 = 0.25 * xm * ep;
+        const double dN0x = -0.25 * em, dN1x = 0.25 * em, dN2x = 0.25 * ep, dN3x = -0.25 * ep;
+        const double dN0e = -0.25 * xm, dN1e = -0.25 * xp, dN2e = 0.25 * xp, dN3e = 0.25 * xm;
+        const double J00 = dN0x * r0 + dN1x * r1 + dN2x * r2 + dN3x * r3;
+        const double J01 = dN0x * z0 + dN1x * z1 + dN2x * z2 + dN3x * z3;
+        const double J10 = dN0e * r0 + dN1e * r1 + dN2e * r2 + dN3e * r3;
+        const double J11 = dN0e * z0 + dN1e * z1 + dN2e * z2 + dN3e * z3;
+        const double det = J00 * J11 - J01 * J10;
+        const double rg = N0 * r0 + N1 * r1 + N2 * r2 + N3 * r3;
+        return weight * (s->formulation == NF_AXISYMMETRIC ? 2.0 * 3.141592653589793 * rg * det : s->thickness * det);
+    };
+    double se = 0.0;
+    for (int e = 0; e < E; e++) {
+        if (gone[e] != 0.0) continue;
+        const HostMaterial& m = matOf(e);
+        const double lam = s->formulation == NF_PLANE_STRESS ? 2.0 * m.lambda() * m.mu() / (m.lambda() + 2.0 * m.mu()) : m.lambda();
+        const double bulk = lam + 2.0 * m.mu() / 3.0;
+        for (int g = 0; g < ppe; g++) {
+            const size_t q = 4 * (size_t)(e * ppe + g);
+            const double press = (sig[q] + sig[q + 1] + sig[q + 2]) / 3.0;
+            const double a = sig[q] - press, bb = sig[q + 1] - press, c = sig[q + 2] - press, dd = sig[q + 3];
+            const double density = (a * a + bb * bb + c * c + 2.0 * dd * dd) / (4.0 * m.mu()) + press * press / (2.0 * bulk);
+            se += density * gaussVolume(e, g);
+        }
+    }
+    o->strain_energy = se;
+    double w = 0.0;
+    for (int e = 0; e < E; e++)
+        for (int g = 0; g < ppe; g++) w += work[e * ppe + g] * gaussVolume(e, g);
+    o->plastic_dissipation = w;
+    double hg = 0.0;
+    if (!(s->integration == NF_FULL || s->d.qhg == 0.0))
+        for (int e = 0; e < E; e++)
+            if (hgK[e] > 0.0) hg += 0.5 * (hgR[e] * hgR[e] + hgZ[e] * hgZ[e]) / hgK[e];
+    o->hourglass_energy = hg;
+    double mx = 0.0;
+    int yielded = 0;
+    for (int q = 0; q < P; q++) {
+        if (epsP[q] > mx) mx = epsP[q];
+        if (epsP[q] > 0.0) yielded++;
+    }
+    o->max_plastic_strain = mx;
+    o->yielded_fraction = (double)yielded / P;
+    if (s->anyJC) {
+        double t = -HUGE_VAL;
+        bool ok = true;
+        for (int q = 0; q < P && ok; q++) {
+            const HostMaterial& m = matOf(q / ppe);
+            if (!m.raw.has_johnson_cook) {
+                ok = false;
+                break;
+            }
+            const nf_johnson_cook& j = m.raw.johnson_cook;
+            const double temp = j.room_temperature + j.taylor_quinney * work[q] / (m.raw.density * j.specific_heat);
+            if (temp != temp) t = temp;
+            else if (t == t && temp > t) t = temp;
+        }
+        o->max_temperature = ok ? t : NAN;
+    } else {
+        o->max_temperature = NAN;
+    }
+    if (s->hasDamage) {
+        std::vector<double> st(P), so(P), fs(P), gf(P);
+        TRY(nf_get_point_field(s, NF_POINT_DAMAGE_STRENGTH, st.data()));
+        TRY(nf_get_point_field(s, NF_POINT_DAMAGE_SOFTENING, so.data()));
+        HIP_TRY(hipMemcpyAsync(fs.data(), s->d.dFailure, sizeof(double) * P, hipMemcpyDeviceToHost, s->stream));
+        TRY(s->sync());
+        double fd = 0.0, md = 0.0;
+        int begun = 0, failed = 0;
+        for (int e = 0; e < E; e++)
+            for (int g = 0; g < ppe; g++) {
+                const int q = e * ppe + g;
+                const double sf = st[q];
+                double diss = 0.0;
+                if (sf > 0.0) {  // Damage.dissipation :363-372
+                    const double slope = so[q];
+                    const double full = -sf / slope;
+                    double excess = epsP[q] - fs[q];
+                    if (excess > 0.0) {
+                        if (excess > full) excess = full;
+                        diss = excess * (sf + 0.5 * slope * excess);
+                    }
+                }
+                fd += diss * gaussVolume(e, g);
+            }
+        for (int q = 0; q < P; q++) {
+            double dmg = 0.0;
+            if (st[q] > 0.0) {
+                const double dd = -so[q] * (epsP[q] - fs[q]) / st[q];
+                dmg = dd <= 0.0 ? 0.0 : (dd >= 1.0 ? 1.0 : dd);
+            }
+            if (dmg > md) md = dmg;
+            if (st[q] > 0.0) begun++;
+            if (dmg >= 1.0) failed++;
+        }
+        o->fracture_dissipation = fd;
+        o->max_damage = md;
+        o->damaged_fraction = (double)begun / P;
+        o->failed_fraction = (double)failed / P;
+    }
+    return NF_OK;
+}
+
+// Evaluates the device implementations of the shared transcendental functions (and the
+// device's division and square root) on n inputs: a self-check that the GPU computes exactly
+// what the host computes. fn: 0 log, 1 exp, 2 pow, 3 atan2(x, y), 4 sin, 5 cos, 6 hypot,
+// 7 sqrt, 8 x / y.
+int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n) {
+    g_error.clear();
+    if (n <= 0) return NF_OK;
+    if (fn < 0 || fn > 8) return fail(NF_E_ARGUMENT, "unknown function");
+    double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    TRY(dev_alloc(&dx, n));
+    TRY(dev_alloc(&dy, n));
+    TRY(dev_alloc(&dout, n));
+    hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (y) hipMemcpy(dy, y, sizeof(double) * n, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_math_eval, dim3(blocks_for(n)), dim3(BLOCK), 0, 0, fn, (const double*)dx,
+                       (const double*)dy, dout, n);
+    hipError_t err = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
+    hipFree(dx);
+    hipFree(dy);
+    hipFree(dout);
+    if (err != hipSuccess) return fail(NF_E_DEVICE, hipGetErrorString(err));
+    return NF_OK;
+}
+
+}  // extern "C"

@@ -1918,3 +1918,24 @@ int nfo_contact_pairs(nfo_solver* s, int32_t capacity, int32_t* out, int32_t* co
 }
 
 }  // extern "C"
+
+// The shared transcendental functions evaluated on the host, for the math self-checks.
+extern "C" int nfo_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n) {
+    for (int i = 0; i < n; i++) {
+        double r;
+        switch (fn) {
+            case 0: r = nfm::log(x[i]); break;
+            case 1: r = nfm::exp(x[i]); break;
+            case 2: r = nfm::pow(x[i], y[i]); break;
+            case 3: r = nfm::atan2(x[i], y[i]); break;
+            case 4: r = nfm::sin(x[i]); break;
+            case 5: r = nfm::cos(x[i]); break;
+            case 6: r = nfm::hypot(x[i], y[i]); break;
+            case 7: r = std::sqrt(x[i]); break;
+            case 8: r = x[i] / y[i]; break;
+            default: return fail(NF_E_ARGUMENT, "unknown function");
+        }
+        out[i] = r;
+    }
+    return NF_OK;
+}
