@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 element-updates per second of the explicit step loop on one MI355X.
+
+Workload (the HBM-roofline case): Taylor impact of a 25.4 mm x 7.62 mm copper rod at
+190 m/s on a rigid anvil with Coulomb friction mu = 0.09, Johnson-Cook flow stress
+(A 90 MPa, B 292 MPa, n 0.31, C 0.025, m 1.09, Tm 1356 K, beta 0.9), axisymmetric, reduced
+integration with hourglass control, finite strain, CFL 0.5, meshed 128 x 1280
+(163,840 elements, 165,249 nodes). One step is one explicit time step of the whole mesh:
+nodal update, element pass, ordered gather, adaptive dt.
+
+`python bench.py [--gpus 1] [--steps K] [--warmup W]` runs W untimed steps from t = 0, then
+times exactly K steps with the device synchronised on both sides, and prints one JSON line.
+Inputs are resident in HBM before the timed region starts.
+
+Extra objects on the line:
+  roofline      the element kernel's algorithmic bytes per launch over its mean launch
+                duration (HIP events on the library's stream, a separate profiled pass of
+                the same steps), against the 8 TB/s HBM peak; traffic from the committed
+                rocprofv3 PMC summary when present.
+  cpu_baseline  the oracle (the C++ restatement, threaded like the reference's Parallel
+                pool) on the host's cores over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "element_kernel_traffic.json")
+
+
+def element_kernel_bytes(E, N, P):
+    """Algorithmic bytes one launch of the reduced-integration element kernel must move.
+
+    Per element: connectivity 16; point state read and written 2 x 96 (stress 32, back
+    stress 32, plastic strain 8, plastic work 8, hourglass force 16); hourglass stiffness
+    written 8; four force slots written 64. Per node, read once: reference coordinates 16,
+    displacement 16, velocity 16.
+    """
+    return E * (16 + 2 * 96 + 8 + 64) + N * 48
+
+
+def step_bytes(E, N):
+    """Algorithmic bytes of one whole step (element + gather + nodal), the survey's
+    B = 352 + 160 N/E per element-step for reduced integration with a rigid wall."""
+    return 352.0 * E + 160.0 * N
+
+
+def cpu_baseline(nr, nz, seconds, threads):
+    """The oracle on the host's cores: steps from t = 0 of the same rod until `seconds` pass."""
+    from neofiz_amd import taylor
+    from oracle.pyoracle import OracleSolver
+    mesh, solver, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER,
+                                   taylor.ROD_SPEED, nr, nz, friction=taylor.ROD_JC_FRICTION,
+                                   solver_cls=OracleSolver)
+    solver.setThreads(threads)
+    solver.run(2)  # warm the pool and the caches
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        solver.run(5)
+        steps += 5
+    dt = time.perf_counter() - t0
+    solver.shutdown()
+    return mesh.elementCount * steps / dt, steps
+
+
+def stream_copy_gbps():
+    """Device-to-device copy bandwidth of a 2 GiB buffer (read + write bytes / time), by the
+    library's plain copy kernel."""
+    import ctypes as C
+    from neofiz_amd import _native
+    b = _native.binding()
+    gbps, ms = C.c_double(), C.c_double()
+    rc = b.copy_bandwidth(1 << 31, 10, C.byref(gbps), C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(b.last_error().decode())
+    return gbps.value
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--nr", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=1280)
+    ap.add_argument("--profile-steps", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-copy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; a multi-GPU figure is not "
+                                   "produced (ensembles shard across GPUs with no "
+                                   "communication; see DESIGN.md)"}))
+        sys.exit(2)
+
+    from neofiz_amd import taylor
+    from neofiz_amd.solver import ExplicitSolver
+
+    print(f"# building the {args.nr}x{args.nz} rod", file=sys.stderr, flush=True)
+    mesh, solver, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER,
+                                   taylor.ROD_SPEED, args.nr, args.nz,
+                                   friction=taylor.ROD_JC_FRICTION, solver_cls=ExplicitSolver)
+    E, N = mesh.elementCount, mesh.nodeCount
+    solver.run(args.warmup)  # synchronous: returns with the device idle
+    print(f"# warmup {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    solver.run(args.steps)   # synchronous
+    elapsed = time.perf_counter() - t0
+    st = solver.state()
+    assert st.steps == args.warmup + args.steps, "step count mismatch"
+
+    # Profiled pass over the next steps: per-kernel mean durations from HIP events.
+    import ctypes as C
+    b = solver._b
+    ms = (C.c_double * 3)()
+    counts = (C.c_int64 * 3)()
+    b.profile(solver._h, 1)
+    b.profile_read(solver._h, ms, counts)
+    solver.run(args.profile_steps)
+    b.profile_read(solver._h, ms, counts)
+    b.profile(solver._h, 0)
+    kernel_ms = [ms[k] / max(1, counts[k]) for k in range(3)]
+    solver.shutdown()
+
+    elem_bytes = element_kernel_bytes(E, N, E)
+    achieved = elem_bytes / (kernel_ms[1] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("nr") == args.nr and t.get("nz") == args.nz:
+            traffic = t.get("bytes_per_launch")
+
+    value = E * args.steps / elapsed
+    line = {
+        "metric": "FP64 element-updates/s (Taylor rod, JC + Coulomb anvil)",
+        "value": value,
+        "unit": "element-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded-free: uniform initial velocity on a generated mesh)",
+        "config": {"workload": f"taylor-rod-jc-{args.nr}x{args.nz}", "elements": E,
+                   "nodes": N, "integration": "reduced+hourglass", "friction": 0.09,
+                   "steps_from_t0": [args.warmup, args.warmup + args.steps],
+                   "parallelism": "single-gpu"},
+        "hbm_fraction_step": value * (step_bytes(E, N) / E) / (HBM_PEAK_GBPS * 1e9),
+        "kernel_ms": {"gather_nodal": kernel_ms[0], "element": kernel_ms[1]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic},
+    }
+    if not args.no_copy:
+        try:
+            line["stream_copy_GBps"] = stream_copy_gbps()
+        except Exception as exc:  # torch missing or no device: report, do not hide
+            line["stream_copy_GBps"] = None
+            line["stream_copy_error"] = str(exc)
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, 16)
+        print(f"# CPU baseline: oracle on {threads} threads for ~{args.cpu_seconds} s",
+              file=sys.stderr, flush=True)
+        rate, cpu_steps = cpu_baseline(args.nr, args.nz, args.cpu_seconds, threads)
+        line["cpu_baseline"] = {
+            "value": rate, "unit": "element-updates/s", "cores": threads, "kind": "port",
+            "sample": f"the same {args.nr}x{args.nz} rod, steps 2..{2 + cpu_steps} from t = 0, "
+                      f"oracle (C++ restatement) threaded with the reference's cursor schedule"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -184,6 +184,35 @@ typedef struct nf_audits {
 } nf_audits;
 int nf_get_audits(nf_solver* s, nf_audits* out);
 
+/* Per-kernel timing of the step loop (no counterpart in the reference, which times whole
+ * runs with System.nanoTime, validate/TaylorImpactCase.java:254-260). When enabled, each
+ * launch is bracketed by HIP events on the handle's stream; nf_profile_read returns summed
+ * milliseconds and launch counts for {nodal, element, gather} and resets them. */
+int nf_profile(nf_solver* s, int32_t enable);
+int nf_profile_read(nf_solver* s, double* ms, int64_t* counts);
+
+/* J2.update (solver/J2.java:136-296) at `paths` independent material points of one material:
+ * increments holds, per path and step, {dRR, dZZ, dTT, dGamRZ (engineering), strainDt};
+ * damage is NULL or, per path, {failureStrain, fractureEnergy, area, uR, uZ, vR, vZ} (the
+ * Damage point data of solver/Damage.java:222-263 and ExplicitSolver.setDamage :767-789);
+ * out receives per path and step {sig rr, zz, tt, rz, back rr, zz, tt, rz, epsP, plastic
+ * work, dGamma, damage strength, softening, band width}. formulation selects the plane-stress
+ * lambda of :291-293. Points start from zero stress. */
+int nf_point_path(const nf_material* material, int32_t formulation, int32_t paths,
+                  int32_t steps, const double* increments, const double* damage, double* out);
+
+/* Device-to-device copy bandwidth probe (no counterpart in the reference): copies `bytes`
+ * reps times with a plain 16-byte-per-thread kernel; reports GB/s (read + write) and ms per
+ * copy. It is the measured stream-copy peak the step loop's HBM fraction is also quoted
+ * against, and the known-size kernel that hardware counters are calibrated with. */
+int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_copy);
+
+/* Device self-check: evaluates the GPU's log (0), exp (1), pow (2), atan2 (3), sin (4),
+ * cos (5), hypot (6), sqrt (7) or division (8) on n inputs. These are the functions the
+ * kernels use in place of java.lang.Math's (core/JohnsonCook.java:139, :182, :196;
+ * solver/Damage.java:315-322), evaluated where the kernels run. */
+int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,7 +140,7 @@ struct PointState {
 
 // solver/J2.java:163-296. `dmg` is null when the solver has no damage model.
 template <bool DAMAGE>
-__device__ void j2_update(PointState& q, double dRR, double dZZ, double dTT, double dGamRZ,
+__device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, double dGamRZ,
                           const Mat& fl, double strainDt, DamagePoint* dmg) {
     const double lambda = fl.lambda, mu = fl.mu;
     const double lt = lambda * (dRR + dZZ + dTT);
@@ -182,7 +182,7 @@ __device__ void j2_update(PointState& q, double dRR, double dZZ, double dTT, dou
         q.s1 = tZZ;
         q.s2 = tTT;
         q.s3 = tRZ;
-        return;
+        return 0.0;
     }
 
     const double stiffness = 2.0 * mu + NFD_TWO_THIRDS * fl.hKin;
@@ -219,6 +219,7 @@ __device__ void j2_update(PointState& q, double dRR, double dZZ, double dTT, dou
         damage_begin(*dmg, !jc ? fl.sy0 + fl.hIso * q.epsP : jc_quasiStatic(fl, q.epsP, temperature),
                      nRR, nZZ, nRZ);
     }
+    return dGamma;
 }
 
 // solver/Corotational.java:59-97, applied to one tensor held in registers.

@@ -57,17 +57,19 @@ constexpr int BLOCK = 256;
 constexpr uint8_t FIXED_R = 1, FIXED_Z = 2, ON_WALL = 4;
 constexpr int MODE_STEP = 0, MODE_INITIAL = 1, MODE_ADVANCE = 2;
 
-// Device-resident control block. Each field has one writer per launch (block 0, thread 0)
-// and is read only by later launches, or is written only through atomics.
-struct Ctl {
-    double time, dt, dtPrev;
-    double endTime;
+// Device-resident control. The per-step scalars are double-buffered: a launch reads buffer
+// q and block 0 writes buffer q ^ 1, so no block ever reads a field that another block of
+// the same launch writes. The accumulators below them have one writer per launch.
+struct Step {
+    double time, dt, dtPrev, endTime;
     long long steps, stepLimit;
-    double timeNext, dtStep;
-    int go;
-    int nanEdge;
-    unsigned long long minEdgeBits;
-    unsigned int wallTicket, erodeTicket;
+    int go;       // the element pass that reads this buffer executes
+    int nanEdge;  // some element's shortest edge was NaN
+    unsigned long long minEdgeBits;  // exact minimum of the squared shortest edge, as bits
+};
+
+struct Ctl {
+    Step s[2];
     int wallContact, wallStuck;
     double wallImpulse, wallEnergyLoss, wallFriction;
     int erodedCount, invertedCount;
@@ -75,6 +77,8 @@ struct Ctl {
     int surfaceStale;
     int pad;
 };
+
+constexpr unsigned long long MAX_BITS = 0x7FEFFFFFFFFFFFFFULL;  // Double.MAX_VALUE
 
 // Everything a kernel needs, passed by value.
 struct Dev {
@@ -112,6 +116,7 @@ struct Dev {
     double gravityR, gravityZ;
     double plane, friction;  // friction may be +inf
     double targetPressure, rampDuration;
+    int nodalBlocks, elementBlocks;
 };
 
 __device__ __forceinline__ double pressureAt(const Dev& d, double t) {  // :1203-1207
@@ -144,125 +149,188 @@ __device__ void block_sum(double (&vals)[K], double* lds) {
 }
 
 // ------------------------------------------------------------------ nodal update
-// solver/ExplicitSolver.java:947-974, :998-1062 with solver/RigidWall.java:153-279.
+// One node of the central-difference update, solver/ExplicitSolver.java:998-1062, with the
+// rigid wall of solver/RigidWall.java:163-279. acc receives this node's contribution to the
+// wall's impulse, arrival energy loss and friction dissipation.
 template <bool WALL>
-__global__ void __launch_bounds__(BLOCK) k_nodal(Dev d) {
-    Ctl* c = d.ctl;
-    const bool go = c->time < c->endTime && c->steps < c->stepLimit;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c->go = go ? 1 : 0;
-        if (go) {
-            c->timeNext = c->time + c->dt;
-            c->dtStep = c->dt;
+__device__ __forceinline__ void nodal_node(const Dev& d, int i, double2 fi, double dt,
+                                           double dtAvg, double decay, double gain,
+                                           double (&acc)[3], int& contacts, int& stuck) {
+    const double m = d.mass[i];
+    const double invM = 1.0 / m;
+    uint8_t fl = d.flags[i];
+    double2 vi = d.v[i];
+    double2 ui = d.u[i];
+    const bool resting = WALL && (fl & ON_WALL) && fi.y <= 0.0;
+    if (fl & FIXED_R) {
+        vi.x = 0.0;
+    } else if (WALL && resting && d.wallFrictional) {
+        // RigidWall.slide :252-279
+        const double normal = -fi.y;
+        const double trial = -m * decay * vi.x / gain - fi.x;
+        const double limit = __builtin_isinf(d.friction) ? HUGE_VAL : d.friction * normal;
+        if (nfd::jabs(trial) <= limit) {
+            acc[1] += 0.5 * m * vi.x * vi.x;
+            stuck++;
+            vi.x = 0.0;
+        } else {
+            const double tangential = __builtin_copysign(limit, trial);
+            const double slid = decay * vi.x + gain * (fi.x + tangential) / m;
+            acc[2] += -tangential * slid * dt;
+            vi.x = slid;
+        }
+        ui.x += vi.x * dt;
+    } else {
+        vi.x = decay * vi.x + gain * fi.x * invM;
+        ui.x += vi.x * dt;
+    }
+    if (fl & FIXED_Z) {
+        vi.y = 0.0;
+    } else if (WALL && resting) {
+        acc[0] += -fi.y * dtAvg;  // RigidWall.hold :175-178
+        contacts++;
+    } else {
+        const double trial = decay * vi.y + gain * fi.y * invM;
+        const double zc = d.x0[i].y + ui.y;
+        if (WALL && zc + trial * dt < d.plane) {
+            acc[0] += -m * trial;  // RigidWall.arrive :201-205
+            acc[1] += 0.5 * m * trial * trial;
+            contacts++;
+            ui.y += d.plane - zc;
+            vi.y = 0.0;
+            fl |= ON_WALL;
+        } else {
+            vi.y = trial;
+            ui.y += vi.y * dt;
+            if (WALL) fl &= (uint8_t)~ON_WALL;
         }
     }
-    if (!go) return;
-    const double dt = c->dt;
-    const double dtAvg = 0.5 * (c->dtPrev + dt);
+    d.v[i] = vi;
+    d.u[i] = ui;
+    if (WALL) d.flags[i] = fl;
+}
+
+// Gather of one node, solver/ExplicitSolver.java:1147-1165: the node's force slots in the
+// reference's order (elements ascending, then pressure edges), plus gravity.
+__device__ __forceinline__ double2 gather_node(const Dev& d, int n) {
+    double sr = 0.0, sz = 0.0;
+    const int end = d.slotStart[n + 1];
+    for (int k = d.slotStart[n]; k < end; k++) {
+        const double2 s = d.slot[d.slotOf[k]];
+        sr += s.x;
+        sz += s.y;
+    }
+    const double m = d.mass[n];
+    return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
+}
+
+constexpr int PHASE_STEP = 0, PHASE_START = 1, PHASE_ASSEMBLE_END = 2;
+
+// The node-centred launch that closes step n and opens step n + 1 (the reference's
+// advanceMaterialAndAssemble tail, the adaptive step, and the next step's nodal update,
+// solver/ExplicitSolver.java:947-990, :1092, :1111-1114):
+//   PHASE_STEP   gather step n's forces, take the CFL step from step n's shortest edge, test
+//                while-time-below-end, and if the next step runs, integrate every node;
+//   PHASE_START  the first nodal update of a run call (gathering first after the initial
+//                zero-interval assembly when gatherFirst is set);
+//   PHASE_ASSEMBLE_END  gather only, after advance().
+// Reads control buffer q; block 0 writes buffer q ^ 1.
+template <bool WALL>
+__global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst) {
+    Ctl* c = d.ctl;
+    const Step& in = c->s[q];
+    Step& out = c->s[q ^ 1];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (phase == PHASE_STEP && !in.go) {  // nothing ran: carry the state over
+        if (lead) out = in;
+        return;
+    }
+    const bool gather = phase == PHASE_STEP || gatherFirst;
+    double dt = in.dt, dtPrev = in.dtPrev;
+    bool next = true;
+    if (phase == PHASE_STEP) {
+        if (d.finite) {
+            const double minEdge = in.nanEdge ? NFD_MAX_VALUE : nfm::from_bits(in.minEdgeBits);
+            dtPrev = in.dt;
+            dt = d.cfl * __builtin_sqrt(minEdge * d.massScale) / d.fastestWave;
+        }
+        next = in.time < in.endTime && in.steps < in.stepLimit;
+    } else if (phase == PHASE_ASSEMBLE_END) {
+        next = false;
+    }
+    if (lead) {
+        Step o = in;
+        o.dt = dt;
+        o.dtPrev = dtPrev;
+        o.go = next ? 1 : 0;
+        if (next) {
+            o.time = in.time + dt;
+            o.steps = in.steps + 1;
+        }
+        o.nanEdge = 0;
+        o.minEdgeBits = MAX_BITS;
+        out = o;
+        if (phase == PHASE_STEP && d.gone) {
+            // eroded energy of step n, summed over the element launch's blocks in order
+            double sum = 0.0;
+            for (int b = 0; b < d.elementBlocks; b++) sum += d.erodePartial[b];
+            c->erodedEnergy += sum;
+        }
+    }
+
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    double2 fi = make_double2(0.0, 0.0);
+    if (i < d.N) {
+        if (gather) {
+            fi = gather_node(d, i);
+            d.f[i] = fi;
+        } else {
+            fi = d.f[i];
+        }
+    }
+    if (!next) return;
+    const double dtAvg = 0.5 * (dtPrev + dt);
     const double half = 0.5 * d.damping * dtAvg;
     const double decay = (1.0 - half) / (1.0 + half);
     const double gain = dtAvg / (1.0 + half);
-
-    double acc[3] = {0.0, 0.0, 0.0};  // impulse, energy loss, friction dissipation
+    double acc[3] = {0.0, 0.0, 0.0};
     int contacts = 0, stuck = 0;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < d.N) {
-        const double m = d.mass[i];
-        const double invM = 1.0 / m;
-        uint8_t fl = d.flags[i];
-        double2 vi = d.v[i];
-        double2 ui = d.u[i];
-        const double2 fi = d.f[i];
-        const bool resting = WALL && (fl & ON_WALL) && fi.y <= 0.0;
-        if (fl & FIXED_R) {
-            vi.x = 0.0;
-        } else if (WALL && resting && d.wallFrictional) {
-            // RigidWall.slide :252-279
-            const double normal = -fi.y;
-            const double trial = -m * decay * vi.x / gain - fi.x;
-            const double limit = __builtin_isinf(d.friction) ? HUGE_VAL : d.friction * normal;
-            if (nfd::jabs(trial) <= limit) {
-                acc[1] += 0.5 * m * vi.x * vi.x;
-                stuck++;
-                vi.x = 0.0;
-            } else {
-                const double tangential = __builtin_copysign(limit, trial);
-                const double slid = decay * vi.x + gain * (fi.x + tangential) / m;
-                acc[2] += -tangential * slid * dt;
-                vi.x = slid;
-            }
-            ui.x += vi.x * dt;
-        } else {
-            vi.x = decay * vi.x + gain * fi.x * invM;
-            ui.x += vi.x * dt;
-        }
-        if (fl & FIXED_Z) {
-            vi.y = 0.0;
-        } else if (WALL && resting) {
-            acc[0] += -fi.y * dtAvg;  // RigidWall.hold :175-178
-            contacts++;
-        } else {
-            const double trial = decay * vi.y + gain * fi.y * invM;
-            const double zc = d.x0[i].y + ui.y;
-            if (WALL && zc + trial * dt < d.plane) {
-                acc[0] += -m * trial;  // RigidWall.arrive :201-205
-                acc[1] += 0.5 * m * trial * trial;
-                contacts++;
-                ui.y += d.plane - zc;
-                vi.y = 0.0;
-                fl |= ON_WALL;
-            } else {
-                vi.y = trial;
-                ui.y += vi.y * dt;
-                if (WALL) fl &= (uint8_t)~ON_WALL;
-            }
-        }
-        d.v[i] = vi;
-        d.u[i] = ui;
-        if (WALL) d.flags[i] = fl;
-    }
+    if (i < d.N) nodal_node<WALL>(d, i, fi, dt, dtAvg, decay, gain, acc, contacts, stuck);
     if (!WALL) return;
 
-    __shared__ double lds[4 * 5];
+    // Per-block wall audit partials (fixed-order block sum); the next element launch adds
+    // them up in block order.
+    __shared__ double lds[4 * 3];
     __shared__ int ldsi[2 * 4];
-    __shared__ bool last;
     block_sum<3>(acc, lds);
-    // integer counts: plain wave sums
-    {
-        int a = contacts, b = stuck;
-        for (int off = 32; off > 0; off >>= 1) {
-            a += __shfl_xor(a, off, 64);
-            b += __shfl_xor(b, off, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            ldsi[(threadIdx.x >> 6) * 2] = a;
-            ldsi[(threadIdx.x >> 6) * 2 + 1] = b;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            contacts = ldsi[0] + ldsi[2] + ldsi[4] + ldsi[6];
-            stuck = ldsi[1] + ldsi[3] + ldsi[5] + ldsi[7];
-        }
+    int a = contacts, b = stuck;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
     }
+    if ((threadIdx.x & 63) == 0) {
+        ldsi[(threadIdx.x >> 6) * 2] = a;
+        ldsi[(threadIdx.x >> 6) * 2 + 1] = b;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         double* p = d.wallPartial + 3 * blockIdx.x;
         p[0] = acc[0];
         p[1] = acc[1];
         p[2] = acc[2];
-        d.wallCountPartial[2 * blockIdx.x] = contacts;
-        d.wallCountPartial[2 * blockIdx.x + 1] = stuck;
-        __threadfence();
-        const unsigned t = atomicAdd(&c->wallTicket, 1u);
-        last = (t == gridDim.x - 1);
+        d.wallCountPartial[2 * blockIdx.x] = ldsi[0] + ldsi[2] + ldsi[4] + ldsi[6];
+        d.wallCountPartial[2 * blockIdx.x + 1] = ldsi[1] + ldsi[3] + ldsi[5] + ldsi[7];
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // Last block: every thread sums a fixed strided subset of the block partials, then the
-    // block tree adds those in a fixed order, so the per-step totals are reproducible.
+}
+
+// Adds the wall partials of the preceding nodal update to the running totals, in block
+// order through a fixed tree (run by block 0 of the element launch that follows it).
+__device__ void reduce_wall_partials(const Dev& d) {
+    __shared__ double lds[4 * 3];
+    __shared__ int ldsi[2 * 4];
     double tot[3] = {0.0, 0.0, 0.0};
     int cc = 0, cs = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK) {
+    for (int b = threadIdx.x; b < d.nodalBlocks; b += BLOCK) {
         tot[0] += d.wallPartial[3 * b];
         tot[1] += d.wallPartial[3 * b + 1];
         tot[2] += d.wallPartial[3 * b + 2];
@@ -280,12 +348,12 @@ __global__ void __launch_bounds__(BLOCK) k_nodal(Dev d) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        Ctl* c = d.ctl;
         c->wallImpulse += tot[0];
         c->wallEnergyLoss += tot[1];
         c->wallFriction += tot[2];
         c->wallContact = ldsi[0] + ldsi[2] + ldsi[4] + ldsi[6];
         c->wallStuck = ldsi[1] + ldsi[3] + ldsi[5] + ldsi[7];
-        c->wallTicket = 0u;
     }
 }
 
@@ -653,10 +721,11 @@ __device__ void pressure_edge(const Dev& d, int k, double p) {
 }
 
 template <bool FULL, bool DAMAGE>
-__global__ void __launch_bounds__(BLOCK) k_element(Dev d, int mode) {
+__global__ void __launch_bounds__(BLOCK) k_element(Dev d, int q, int mode) {
     Ctl* c = d.ctl;
-    if (mode == MODE_STEP && !c->go) return;
-    const double strainDt = mode == MODE_INITIAL ? 0.0 : (mode == MODE_STEP ? c->dtStep : c->dt);
+    Step& st = c->s[q];
+    if (mode == MODE_STEP && !st.go) return;
+    const double strainDt = mode == MODE_INITIAL ? 0.0 : st.dt;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     double edge = NFD_MAX_VALUE;
     bool sawNaN = false;
@@ -676,12 +745,11 @@ __global__ void __launch_bounds__(BLOCK) k_element(Dev d, int mode) {
             sawNaN = edge != edge;
         }
     } else if (e < d.E + d.edges) {
-        const double t = mode == MODE_STEP ? c->timeNext : c->time;
-        pressure_edge(d, e - d.E, pressureAt(d, t));
+        pressure_edge(d, e - d.E, pressureAt(d, st.time));
     }
 
-    // Shortest edge: exact minimum over positive doubles by their bit patterns.
-    unsigned long long bits = sawNaN ? 0x7FEFFFFFFFFFFFFFULL : (unsigned long long)nfm::bits_of(edge);
+    // Shortest edge: exact minimum over non-negative doubles by their bit patterns.
+    unsigned long long bits = sawNaN ? MAX_BITS : (unsigned long long)nfm::bits_of(edge);
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(bits, off, 64);
         bits = o < bits ? o : bits;
@@ -699,65 +767,27 @@ __global__ void __launch_bounds__(BLOCK) k_element(Dev d, int mode) {
         __shared__ double ldsE[4 * 1];
         double ev[1] = {eEnergy};
         block_sum<1>(ev, ldsE);
-        int a = eCount, b = iCount, s = stale ? 1 : 0;
+        int a = eCount, b = iCount, sflag = stale ? 1 : 0;
         for (int off = 32; off > 0; off >>= 1) {
             a += __shfl_xor(a, off, 64);
             b += __shfl_xor(b, off, 64);
-            s |= __shfl_xor(s, off, 64);
+            sflag |= __shfl_xor(sflag, off, 64);
         }
-        if ((threadIdx.x & 63) == 0 && (a | b | s)) {
+        if ((threadIdx.x & 63) == 0 && (a | b | sflag)) {
             atomicAdd(&c->erodedCount, a);
             atomicAdd(&c->invertedCount, b);
-            if (s) c->surfaceStale = 1;
+            if (sflag) c->surfaceStale = 1;
         }
-        if (threadIdx.x == 0) {
-            d.erodePartial[blockIdx.x] = ev[0];
-        }
+        if (threadIdx.x == 0) d.erodePartial[blockIdx.x] = ev[0];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long m = ldsMin[0];
         for (int w = 1; w < BLOCK / 64; w++) m = ldsMin[w] < m ? ldsMin[w] : m;
-        if (m != 0x7FEFFFFFFFFFFFFFULL) atomicMin(&c->minEdgeBits, m);
-        if (ldsNaN) atomicOr(&c->nanEdge, 1);
+        if (m != MAX_BITS) atomicMin(&st.minEdgeBits, m);
+        if (ldsNaN) atomicOr(&st.nanEdge, 1);
     }
-}
-
-// ------------------------------------------------------------------ gather
-// solver/ExplicitSolver.java:1147-1165 and the step's tail :973-990.
-__global__ void __launch_bounds__(BLOCK) k_gather(Dev d, int mode, int erodeBlocks) {
-    Ctl* c = d.ctl;
-    if (mode == MODE_STEP && !c->go) return;
-    const int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n < d.N) {
-        double sr = 0.0, sz = 0.0;
-        const int end = d.slotStart[n + 1];
-        for (int k = d.slotStart[n]; k < end; k++) {
-            const double2 s = d.slot[d.slotOf[k]];
-            sr += s.x;
-            sz += s.y;
-        }
-        const double m = d.mass[n];
-        d.f[n] = make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const double minEdge = c->nanEdge ? NFD_MAX_VALUE : nfm::from_bits(c->minEdgeBits);
-        c->minEdgeBits = 0x7FEFFFFFFFFFFFFFULL;  // Double.MAX_VALUE
-        c->nanEdge = 0;
-        if (mode == MODE_STEP) {
-            c->time = c->timeNext;
-            c->steps = c->steps + 1;
-            if (d.finite) {
-                c->dtPrev = c->dtStep;
-                c->dt = d.cfl * __builtin_sqrt(minEdge * d.massScale) / d.fastestWave;
-            }
-            if (d.gone) {
-                double s = 0.0;
-                for (int b = 0; b < erodeBlocks; b++) s += d.erodePartial[b];
-                c->erodedEnergy += s;
-            }
-        }
-    }
+    if (mode == MODE_STEP && d.wall && blockIdx.x == 0) reduce_wall_partials(d);
 }
 
 // ------------------------------------------------------------------ setup and small kernels
@@ -811,16 +841,49 @@ __global__ void k_fill2(double2* a, int N, double x, double y) {
 }
 
 // advance() :1069-1075, the displacement half.
-__global__ void k_advance_disp(Dev d) {
+__global__ void k_advance_disp(Dev d, int q) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
-    const double dt = d.ctl->dt;
+    const double dt = d.ctl->s[q].dt;
     const uint8_t fl = d.flags[i];
     double2 ui = d.u[i];
     const double2 vi = d.v[i];
     if (!(fl & FIXED_R)) ui.x += vi.x * dt;
     if (!(fl & FIXED_Z)) ui.y += vi.y * dt;
     d.u[i] = ui;
+}
+
+// Single material points driven through J2.update (solver/J2.java:163-296), one thread per
+// path, for the constitutive self-checks.
+template <bool DAMAGE>
+__global__ void k_point_path(Mat m, int paths, int steps, const double* inc, const double* dmgIn,
+                             double* out) {
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= paths) return;
+    nfd::PointState q{};
+    nfd::DamagePoint dp{};
+    if (DAMAGE) {
+        const double* g = dmgIn + 7 * (size_t)p;
+        dp.failureStrain = g[0];
+        dp.fractureEnergy = g[1];
+        dp.youngs = m.youngs;
+        dp.area = g[2]; dp.uR = g[3]; dp.uZ = g[4]; dp.vR = g[5]; dp.vZ = g[6];
+    }
+    for (int k = 0; k < steps; k++) {
+        const double* x = inc + 5 * ((size_t)p * steps + k);
+        const double dg = nfd::j2_update<DAMAGE>(q, x[0], x[1], x[2], x[3], m, x[4], &dp);
+        double* o = out + 14 * ((size_t)p * steps + k);
+        o[0] = q.s0; o[1] = q.s1; o[2] = q.s2; o[3] = q.s3;
+        o[4] = q.b0; o[5] = q.b1; o[6] = q.b2; o[7] = q.b3;
+        o[8] = q.epsP; o[9] = q.work; o[10] = dg;
+        o[11] = dp.strength; o[12] = dp.softening; o[13] = dp.width;
+    }
+}
+
+// Plain device copy, 16 bytes per thread per iteration, grid-stride: the bandwidth probe and
+// the known-size kernel the HBM counters are calibrated against.
+__global__ void __launch_bounds__(BLOCK) k_copy(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) b[i] = a[i];
 }
 
 // Device self-check of the shared transcendental functions (nf_math.h).
@@ -870,6 +933,34 @@ struct HostMaterial {
     double wave() const { return std::sqrt((lambda() + 2.0 * mu()) / raw.density); }
 };
 
+// Unrolls one material into the kernels' constants (solver/ExplicitSolver.java:289-299,
+// J2.Flow.of solver/J2.java:108-113).
+Mat make_mat(const nf_material& raw, int formulation) {
+    HostMaterial hm{raw};
+    Mat t;
+    std::memset(&t, 0, sizeof t);
+    const double lam = formulation == NF_PLANE_STRESS
+                           ? 2.0 * hm.lambda() * hm.mu() / (hm.lambda() + 2.0 * hm.mu())
+                           : hm.lambda();
+    t.lambda = lam;
+    t.mu = hm.mu();
+    t.bulk = lam + 2.0 * hm.mu() / 3.0;
+    t.density = raw.density;
+    t.sy0 = raw.yield_stress;
+    t.hIso = raw.isotropic_hardening;
+    t.hKin = raw.kinematic_hardening;
+    t.youngs = raw.youngs_modulus;
+    t.hasJC = raw.has_johnson_cook;
+    if (t.hasJC) {
+        const nf_johnson_cook& j = raw.johnson_cook;
+        t.a = j.a; t.b = j.b; t.n = j.n; t.c = j.c; t.m = j.m;
+        t.eps0 = j.reference_strain_rate; t.troom = j.room_temperature;
+        t.tmelt = j.melt_temperature; t.cp = j.specific_heat; t.beta = j.taylor_quinney;
+        t.thermalCoefficient = j.taylor_quinney / (raw.density * j.specific_heat);
+    }
+    return t;
+}
+
 int check_material(const nf_material* m) {
     if (!m) return fail(NF_E_ARGUMENT, "material must not be null");
     if (m->youngs_modulus <= 0) return fail(NF_E_ARGUMENT, "E must be positive");
@@ -904,6 +995,7 @@ int check_material(const nf_material* m) {
 struct nf_solver {
     hipStream_t stream = nullptr;
     Dev d{};
+    int q = 0;  // control buffer holding the current state
     Ctl* ctl = nullptr;
     int N = 0, E = 0, P = 0, edges = 0, ppe = 1;
     int formulation = 0, integration = 0, kinematics = 0;
@@ -918,14 +1010,60 @@ struct nf_solver {
     bool hasWall = false, hasDamage = false, hasErosion = false, hasContact = false;
     bool anyJC = false;
     double wallPlane = 0.0, wallFriction = 0.0;
+    // Optional per-kernel timing with HIP events on the handle's stream (nf_profile).
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    std::vector<std::pair<int, std::pair<int, int>>> evPending;  // kernel id, (start, stop)
+    double kernelMs[3] = {0.0, 0.0, 0.0};
+    long long kernelCount[3] = {0, 0, 0};
     // device buffers owned here
     std::vector<void*> owned;
     double* densityOfElement = nullptr;
 
     ~nf_solver() {
-        if (stream) hipStreamSynchronize(stream);
-        for (void* p : owned) hipFree(p);
-        if (stream) hipStreamDestroy(stream);
+        if (stream) (void)hipStreamSynchronize(stream);
+        dropGraphs();
+        for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
+        for (void* p : owned) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int event(int index, hipEvent_t* out) {
+        while ((int)evPool.size() <= index) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            evPool.push_back(e);
+        }
+        *out = evPool[index];
+        return NF_OK;
+    }
+
+    // Records an event pair around the launch `fn` when profiling.
+    template <typename F>
+    int timed(int kernel, F fn) {
+        if (!profiling) return fn();
+        const int base = 2 * (int)evPending.size();
+        hipEvent_t a, b;
+        TRY(event(base, &a));
+        TRY(event(base + 1, &b));
+        HIP_TRY(hipEventRecord(a, stream));
+        TRY(fn());
+        HIP_TRY(hipEventRecord(b, stream));
+        evPending.push_back({kernel, {base, base + 1}});
+        return NF_OK;
+    }
+
+    int harvest() {
+        if (evPending.empty()) return NF_OK;
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (auto& p : evPending) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, evPool[p.second.first], evPool[p.second.second]));
+            kernelMs[p.first] += ms;
+            kernelCount[p.first] += 1;
+        }
+        evPending.clear();
+        return NF_OK;
     }
 
     template <typename T>
@@ -956,69 +1094,133 @@ struct nf_solver {
         return NF_OK;
     }
 
-    int launchElement(int mode) {
+    // The step buffer that holds the solver's current state.
+    int readStep(Step& out) {
+        Ctl h;
+        TRY(readCtl(h));
+        out = h.s[q];
+        return NF_OK;
+    }
+
+    int writeStep(const Step& in) {
+        HIP_TRY(hipMemcpyAsync(&ctl->s[q], &in, sizeof(Step), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return NF_OK;
+    }
+
+    int launchElement(int mode, int parity) {
         const int grid = blocks_for(E + edges);
         const bool full = integration == NF_FULL;
-        if (full && hasDamage) hipLaunchKernelGGL((k_element<true, true>), dim3(grid), dim3(BLOCK), 0, stream, d, mode);
-        else if (full) hipLaunchKernelGGL((k_element<true, false>), dim3(grid), dim3(BLOCK), 0, stream, d, mode);
-        else if (hasDamage) hipLaunchKernelGGL((k_element<false, true>), dim3(grid), dim3(BLOCK), 0, stream, d, mode);
-        else hipLaunchKernelGGL((k_element<false, false>), dim3(grid), dim3(BLOCK), 0, stream, d, mode);
+        if (full && hasDamage) hipLaunchKernelGGL((k_element<true, true>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
+        else if (full) hipLaunchKernelGGL((k_element<true, false>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
+        else if (hasDamage) hipLaunchKernelGGL((k_element<false, true>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
+        else hipLaunchKernelGGL((k_element<false, false>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
 
-    int launchGather(int mode) {
-        hipLaunchKernelGGL(k_gather, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, mode,
-                           blocks_for(E + edges));
+    int launchGN(int phase, int parity, int gatherFirst) {
+        if (hasWall) hipLaunchKernelGGL(k_gn<true>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst);
+        else hipLaunchKernelGGL(k_gn<false>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
 
-    int launchNodal() {
-        if (hasWall) hipLaunchKernelGGL(k_nodal<true>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d);
-        else hipLaunchKernelGGL(k_nodal<false>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d);
-        HIP_TRY(hipGetLastError());
-        return NF_OK;
-    }
-
-    // advanceMaterialAndAssemble(strainDt) :1081-1115 for the initial and advance modes.
-    int assemble(int mode) {
-        TRY(launchElement(mode));
-        TRY(launchGather(mode));
-        if (hasContact) return fail(NF_E_STATE, "penalty contact is not available in this build");
-        return NF_OK;
-    }
-
-    // Enqueues one full step; the device decides whether it executes.
+    // One step on the stream: the element pass of the open step, then the launch that
+    // gathers it and opens the next. The device decides whether either does anything.
     int enqueueStep() {
-        TRY(launchNodal());
-        TRY(launchElement(MODE_STEP));
-        TRY(launchGather(MODE_STEP));
+        TRY(timed(1, [&] { return launchElement(MODE_STEP, q); }));
+        TRY(timed(0, [&] { return launchGN(PHASE_STEP, q, 0); }));
+        q ^= 1;
+        return NF_OK;
+    }
+
+    // A hipGraph of GRAPH_STEPS steps starting at parity p (an even count, so the parity is
+    // the same at both ends). Captured on first use and whenever a setter changes a uniform.
+    static constexpr int GRAPH_STEPS = 32;
+    hipGraphExec_t graphExec[2] = {nullptr, nullptr};
+    bool graphDirty = true;
+    bool useGraphs = true;
+
+    void dropGraphs() {
+        for (int k = 0; k < 2; k++)
+            if (graphExec[k]) {
+                (void)hipGraphExecDestroy(graphExec[k]);
+                graphExec[k] = nullptr;
+            }
+        graphDirty = false;
+    }
+
+    int captureGraph(int p) {
+        hipGraph_t g = nullptr;
+        const int q0 = q;
+        q = p;
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = NF_OK;
+        for (int k = 0; k < GRAPH_STEPS && rc == NF_OK; k++) rc = enqueueStep();
+        hipError_t err = hipStreamEndCapture(stream, &g);
+        q = q0;
+        if (rc) return rc;
+        if (err != hipSuccess) return fail(NF_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(err));
+        err = hipGraphInstantiate(&graphExec[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) return fail(NF_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(err));
+        return NF_OK;
+    }
+
+    int launchSteps(int n) {
+        if (useGraphs && !profiling && n == GRAPH_STEPS) {
+            if (graphDirty) dropGraphs();
+            if (!graphExec[q]) TRY(captureGraph(q));
+            HIP_TRY(hipGraphLaunch(graphExec[q], stream));
+            return NF_OK;  // GRAPH_STEPS is even: the parity is unchanged
+        }
+        for (int k = 0; k < n; k++) TRY(enqueueStep());
+        return NF_OK;
+    }
+
+    // advanceMaterialAndAssemble(strainDt) :1081-1115 outside a step: the initial
+    // zero-interval assembly is followed by the first nodal update (PHASE_START with
+    // gatherFirst); advance() ends with a bare gather.
+    int assembleAdvance() {
+        if (hasContact) return fail(NF_E_STATE, "penalty contact is not available in this build");
+        TRY(launchElement(MODE_ADVANCE, q));
+        TRY(launchGN(PHASE_ASSEMBLE_END, q, 1));
+        q ^= 1;
         return NF_OK;
     }
 
     // run steps while time < endTime and fewer than maxSteps have been taken.
     int runSteps(double endTime, long long maxSteps, long long* taken) {
-        Ctl h;
-        TRY(readCtl(h));
+        Step h;
+        TRY(readStep(h));
         const long long start = h.steps;
         if (taken) *taken = 0;
         if (!(h.time < endTime) || maxSteps == 0) return NF_OK;
         if (hasContact) return fail(NF_E_STATE, "penalty contact is not available in this build");
-        if (h.steps == 0) TRY(assemble(MODE_INITIAL));  // :945
         h.endTime = endTime;
         h.stepLimit = maxSteps > 0 ? start + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
-        // Only the two request fields change; the rest of the control block is device state.
-        HIP_TRY(hipMemcpyAsync(&ctl->endTime, &h.endTime, sizeof(double), hipMemcpyHostToDevice, stream));
-        HIP_TRY(hipMemcpyAsync(&ctl->stepLimit, &h.stepLimit, sizeof(long long), hipMemcpyHostToDevice, stream));
-        const int chunk = 64;
+        TRY(writeStep(h));
+        // Open the first step of this call.
+        if (h.steps == 0) {
+            TRY(launchElement(MODE_INITIAL, q));  // :945
+            TRY(timed(0, [&] { return launchGN(PHASE_START, q, 1); }));
+        } else {
+            TRY(timed(0, [&] { return launchGN(PHASE_START, q, 0); }));
+        }
+        q ^= 1;
+        long long done = 0;  // steps enqueued in this call
         while (true) {
-            long long budget = maxSteps > 0 ? (start + maxSteps - h.steps) : chunk;
-            const int n = (int)(budget < chunk ? budget : chunk);
-            for (int k = 0; k < n; k++) TRY(enqueueStep());
-            TRY(readCtl(h));
-            if (!(h.time < endTime)) break;
-            if (maxSteps > 0 && h.steps >= start + maxSteps) break;
+            int n = GRAPH_STEPS;
+            if (maxSteps > 0) {
+                const long long left = maxSteps - done;
+                n = (int)(left < GRAPH_STEPS ? left : GRAPH_STEPS);
+            }
+            TRY(launchSteps(n));
+            done += n;
+            TRY(readStep(h));
+            TRY(harvest());
+            if (!h.go) break;
         }
         if (taken) *taken = h.steps - start;
         return NF_OK;
@@ -1109,29 +1311,8 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
     // Material table (solver/ExplicitSolver.java:289-299, J2.Flow.of).
     std::vector<Mat> table(material_count);
     for (int m = 0; m < material_count; m++) {
-        const HostMaterial& hm = s->materials[m];
-        Mat& t = table[m];
-        std::memset(&t, 0, sizeof t);
-        const double lam = formulation == NF_PLANE_STRESS
-                               ? 2.0 * hm.lambda() * hm.mu() / (hm.lambda() + 2.0 * hm.mu())
-                               : hm.lambda();
-        t.lambda = lam;
-        t.mu = hm.mu();
-        t.bulk = lam + 2.0 * hm.mu() / 3.0;
-        t.density = hm.raw.density;
-        t.sy0 = hm.raw.yield_stress;
-        t.hIso = hm.raw.isotropic_hardening;
-        t.hKin = hm.raw.kinematic_hardening;
-        t.youngs = hm.raw.youngs_modulus;
-        t.hasJC = hm.raw.has_johnson_cook;
-        if (t.hasJC) {
-            const nf_johnson_cook& j = hm.raw.johnson_cook;
-            t.a = j.a; t.b = j.b; t.n = j.n; t.c = j.c; t.m = j.m;
-            t.eps0 = j.reference_strain_rate; t.troom = j.room_temperature;
-            t.tmelt = j.melt_temperature; t.cp = j.specific_heat; t.beta = j.taylor_quinney;
-            t.thermalCoefficient = j.taylor_quinney / (hm.raw.density * j.specific_heat);
-            s->anyJC = true;
-        }
+        table[m] = make_mat(s->materials[m].raw, formulation);
+        if (table[m].hasJC) s->anyJC = true;
     }
     // Materials.fastestWave core/Materials.java:120-128
     if (s->materialOf.empty()) {
@@ -1246,11 +1427,14 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         }
     Ctl h;
     std::memset(&h, 0, sizeof h);
-    h.dt = cfl_safety * minEdge / s->fastestWave;
-    h.dtPrev = h.dt;
-    h.minEdgeBits = 0x7FEFFFFFFFFFFFFFULL;
-    h.stepLimit = 0;
-    s->dt0 = h.dt;
+    for (int k = 0; k < 2; k++) {
+        h.s[k].dt = cfl_safety * minEdge / s->fastestWave;
+        h.s[k].dtPrev = h.s[k].dt;
+        h.s[k].minEdgeBits = MAX_BITS;
+    }
+    s->dt0 = h.s[0].dt;
+    s->d.nodalBlocks = blocks_for(N);
+    s->d.elementBlocks = blocks_for(E + pressure_edge_count);
     if ((rc = s->writeCtl(h))) return bail(rc);
     *out = s;
     return NF_OK;
@@ -1297,18 +1481,21 @@ int nf_fix_all_axial(nf_solver* s) {
 int nf_set_pressure_ramp(nf_solver* s, double pascals, double ramp) {
     s->d.targetPressure = pascals;
     s->d.rampDuration = ramp;
+    s->graphDirty = true;
     return NF_OK;
 }
 int nf_set_relaxation_damping(nf_solver* s, double fraction, double omega) {
     g_error.clear();
     if (omega <= 0.0) return fail(NF_E_ARGUMENT, "angular frequency must be positive");
     s->d.damping = 2.0 * fraction * omega;
+    s->graphDirty = true;
     return NF_OK;
 }
 int nf_set_hourglass_coefficient(nf_solver* s, double q) {
     g_error.clear();
     if (q < 0.0) return fail(NF_E_ARGUMENT, "hourglass coefficient must be >= 0");
     s->d.qhg = q;
+    s->graphDirty = true;
     return NF_OK;
 }
 int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double friction) {
@@ -1316,6 +1503,7 @@ int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double fric
     if (!enabled) {
         s->hasWall = false;
         s->d.wall = 0;
+        s->graphDirty = true;
         return NF_OK;
     }
     if (friction < 0.0 || friction != friction) return fail(NF_E_ARGUMENT, "friction coefficient must be >= 0");
@@ -1324,6 +1512,7 @@ int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double fric
     s->d.plane = plane_z;
     s->d.friction = friction;
     s->d.wallFrictional = friction > 0.0;
+    s->graphDirty = true;
     // A new RigidWall starts with zero audits (solver/RigidWall.java:103-107).
     Ctl h;
     TRY(s->readCtl(h));
@@ -1345,6 +1534,7 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
 int nf_set_gravity(nf_solver* s, double ar, double az) {
     s->d.gravityR = ar;
     s->d.gravityZ = az;
+    s->graphDirty = true;
     return NF_OK;
 }
 int nf_set_damage(nf_solver* s, const double* gf, const double* eps) {  // :729-791
@@ -1411,6 +1601,7 @@ int nf_set_damage(nf_solver* s, const double* gf, const double* eps) {  // :729-
     HIP_TRY(hipMemcpyAsync((void*)d.dGeom, geom.data(), sizeof(double) * 5 * P, hipMemcpyHostToDevice, s->stream));
     TRY(s->sync());
     s->hasDamage = true;
+    s->graphDirty = true;
     return NF_OK;
 }
 int nf_set_erosion(nf_solver* s, double threshold) {  // :549-561
@@ -1425,13 +1616,14 @@ int nf_set_erosion(nf_solver* s, double threshold) {  // :549-561
         s->d.gone = g;
     }
     s->hasErosion = true;
+    s->graphDirty = true;
     return NF_OK;
 }
 int nf_set_mass_scaling(nf_solver* s, double target) {  // :601-619
     g_error.clear();
     if (!(target > 0.0)) return fail(NF_E_ARGUMENT, "the target step must be positive");
-    Ctl h;
-    TRY(s->readCtl(h));
+    Step h;
+    TRY(s->readStep(h));
     if (target < h.dt) return fail(NF_E_ARGUMENT, "mass scaling adds mass and cannot shorten a step");
     const double factor = (target / h.dt) * (target / h.dt);
     hipLaunchKernelGGL(k_scale_mass, dim3(blocks_for(s->N)), dim3(BLOCK), 0, s->stream,
@@ -1442,7 +1634,8 @@ int nf_set_mass_scaling(nf_solver* s, double target) {  // :601-619
     h.dt = target;
     h.dtPrev = target;
     s->dt0 = target;
-    return s->writeCtl(h);
+    s->graphDirty = true;
+    return s->writeStep(h);
 }
 int nf_set_velocity(nf_solver* s, int32_t node, double vr, double vz) {
     g_error.clear();
@@ -1471,9 +1664,9 @@ int nf_run_until(nf_solver* s, double end_time, int64_t max_steps, int64_t* step
 }
 int nf_advance(nf_solver* s) {
     g_error.clear();
-    hipLaunchKernelGGL(k_advance_disp, dim3(blocks_for(s->N)), dim3(BLOCK), 0, s->stream, s->d);
+    hipLaunchKernelGGL(k_advance_disp, dim3(blocks_for(s->N)), dim3(BLOCK), 0, s->stream, s->d, s->q);
     HIP_TRY(hipGetLastError());
-    TRY(s->assemble(MODE_ADVANCE));
+    TRY(s->assembleAdvance());
     return s->sync();
 }
 
@@ -1589,13 +1782,14 @@ int nf_get_state(nf_solver* s, nf_state* o) {
     g_error.clear();
     Ctl h;
     TRY(s->readCtl(h));
+    const Step& st = h.s[s->q];
     std::memset(o, 0, sizeof *o);
-    o->time = h.time;
-    o->dt = h.dt;
-    o->dt_previous = h.dtPrev;
+    o->time = st.time;
+    o->dt = st.dt;
+    o->dt_previous = st.dtPrev;
     o->dt0 = s->dt0;
     o->mass_scale = s->massScale;
-    o->steps = h.steps;
+    o->steps = st.steps;
     o->node_count = s->N;
     o->element_count = s->E;
     o->points_per_element = s->ppe;
@@ -1635,8 +1829,8 @@ int nf_get_audits(nf_solver* s, nf_audits* o) {
     TRY(nf_get_element_field(s, NF_ELEMENT_HG_Z, hgZ.data()));
     TRY(nf_get_element_field(s, NF_ELEMENT_HG_STIFFNESS, hgK.data()));
     TRY(nf_get_element_field(s, NF_ELEMENT_ERODED, gone.data()));
-    Ctl h;
-    TRY(s->readCtl(h));
+    Step h;
+    TRY(s->readStep(h));
 
     double p = 0.0;
     for (int i = 0; i < N; i++) p += mass[i] * vz[i];
@@ -1772,6 +1966,86 @@ int nf_get_audits(nf_solver* s, nf_audits* o) {
     return NF_OK;
 }
 
+// Per-kernel timing of the step loop: when enabled, every launch of the nodal, element and
+// gather kernels is bracketed by HIP events on the handle's stream. nf_profile_read returns
+// the summed milliseconds and launch counts (indices 0 nodal, 1 element, 2 gather) and
+// resets them.
+int nf_profile(nf_solver* s, int32_t enable) {
+    g_error.clear();
+    TRY(s->harvest());
+    s->profiling = enable != 0;
+    return NF_OK;
+}
+
+int nf_profile_read(nf_solver* s, double* ms, int64_t* counts) {
+    g_error.clear();
+    TRY(s->harvest());
+    for (int k = 0; k < 3; k++) {
+        ms[k] = s->kernelMs[k];
+        counts[k] = s->kernelCount[k];
+        s->kernelMs[k] = 0.0;
+        s->kernelCount[k] = 0;
+    }
+    return NF_OK;
+}
+
+// Drives `paths` independent material points of one material through `steps` strain
+// increments each (J2.update, solver/J2.java:163-296).
+int nf_point_path(const nf_material* m, int32_t formulation, int32_t paths, int32_t steps,
+                  const double* increments, const double* damage, double* out) {
+    g_error.clear();
+    TRY(check_material(m));
+    if (paths < 1 || steps < 1 || !increments || !out) return fail(NF_E_ARGUMENT, "empty path");
+    const Mat mat = make_mat(*m, formulation);
+    double *di = nullptr, *dd = nullptr, *dout = nullptr;
+    const size_t ni = 5 * (size_t)paths * steps, no = 14 * (size_t)paths * steps;
+    TRY(dev_alloc(&di, ni));
+    TRY(dev_alloc(&dout, no));
+    if (damage) TRY(dev_alloc(&dd, 7 * (size_t)paths));
+    HIP_TRY(hipMemcpy(di, increments, sizeof(double) * ni, hipMemcpyHostToDevice));
+    if (damage) HIP_TRY(hipMemcpy(dd, damage, sizeof(double) * 7 * paths, hipMemcpyHostToDevice));
+    if (damage) hipLaunchKernelGGL(k_point_path<true>, dim3(blocks_for(paths)), dim3(BLOCK), 0, 0, mat, paths, steps, (const double*)di, (const double*)dd, dout);
+    else hipLaunchKernelGGL(k_point_path<false>, dim3(blocks_for(paths)), dim3(BLOCK), 0, 0, mat, paths, steps, (const double*)di, (const double*)dd, dout);
+    hipError_t err = hipMemcpy(out, dout, sizeof(double) * no, hipMemcpyDeviceToHost);
+    (void)hipFree(di);
+    (void)hipFree(dd);
+    (void)hipFree(dout);
+    if (err != hipSuccess) return fail(NF_E_DEVICE, hipGetErrorString(err));
+    return NF_OK;
+}
+
+// Device-to-device copy bandwidth: `bytes` copied `reps` times by k_copy; returns read plus
+// written bytes per second in GB/s and the mean kernel time.
+int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_copy) {
+    g_error.clear();
+    if (bytes < 16 || reps < 1) return fail(NF_E_ARGUMENT, "need at least 16 bytes and one repetition");
+    const size_t n = (size_t)bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    TRY(dev_alloc(&a, n));
+    if (int rc = dev_alloc(&b, n)) {
+        (void)hipFree(a);
+        return rc;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int grid = 256 * 16;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(BLOCK), 0, 0, (const double2*)a, b, n);
+    HIP_TRY(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(BLOCK), 0, 0, (const double2*)a, b, n);
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    *ms_per_copy = ms / reps;
+    *gbps = 2.0 * 16.0 * n / (*ms_per_copy * 1e-3) / 1e9;
+    return NF_OK;
+}
+
 // Evaluates the device implementations of the shared transcendental functions (and the
 // device's division and square root) on n inputs: a self-check that the GPU computes exactly
 // what the host computes. fn: 0 log, 1 exp, 2 pow, 3 atan2(x, y), 4 sin, 5 cos, 6 hypot,
@@ -1784,14 +2058,14 @@ int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int3
     TRY(dev_alloc(&dx, n));
     TRY(dev_alloc(&dy, n));
     TRY(dev_alloc(&dout, n));
-    hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
-    if (y) hipMemcpy(dy, y, sizeof(double) * n, hipMemcpyHostToDevice);
+    HIP_TRY(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (y) HIP_TRY(hipMemcpy(dy, y, sizeof(double) * n, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_math_eval, dim3(blocks_for(n)), dim3(BLOCK), 0, 0, fn, (const double*)dx,
                        (const double*)dy, dout, n);
     hipError_t err = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
-    hipFree(dx);
-    hipFree(dy);
-    hipFree(dout);
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    (void)hipFree(dout);
     if (err != hipSuccess) return fail(NF_E_DEVICE, hipGetErrorString(err));
     return NF_OK;
 }

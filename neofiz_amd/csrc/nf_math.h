@@ -189,9 +189,10 @@ NF_HD double exp_dd(double a, double b) {
 NF_HD double exp(double x) { return exp_dd(x, 0.0); }
 
 // x^y for the cases the solver needs: x > 0 finite, y finite. Special values follow IEEE pow
-// for x = 0, x = 1, y = 0, NaN; a negative base yields NaN.
+// for x = 0, x = 1, y = 0, y = 1 (exactly x), NaN; a negative base yields NaN.
 NF_HD double pow(double x, double y) {
     if (y == 0.0) return 1.0;
+    if (y == 1.0) return x;
     if (is_nan(x) || is_nan(y)) return qnan();
     if (x == 1.0) return 1.0;
     if (x == 0.0) return y > 0.0 ? 0.0 : inf();

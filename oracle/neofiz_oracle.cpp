@@ -1939,3 +1939,56 @@ extern "C" int nfo_math_eval(int32_t fn, const double* x, const double* y, doubl
     }
     return NF_OK;
 }
+
+// J2.update at independent material points (see nf_point_path in include/neofiz.h).
+extern "C" int nfo_point_path(const nf_material* m, int32_t formulation, int32_t paths,
+                              int32_t steps, const double* inc, const double* dmgIn, double* out) {
+    g_error.clear();
+    int rc = check_material(m);
+    if (rc) return rc;
+    if (paths < 1 || steps < 1 || !inc || !out) return fail(NF_E_ARGUMENT, "empty path");
+    const Material mat = to_material(*m);
+    const double lam = formulation == NF_PLANE_STRESS
+                           ? 2.0 * mat.lambda() * mat.mu() / (mat.lambda() + 2.0 * mat.mu())
+                           : mat.lambda();
+    Flow fl;
+    fl.sy0 = mat.sy;
+    fl.hIso = mat.hiso;
+    fl.hKin = mat.hkin;
+    fl.jc = mat.has_jc ? &mat.jc : nullptr;
+    fl.thermalCoefficient = mat.has_jc ? mat.jc.thermalCoefficient(mat.rho) : 0.0;
+    Damage d;
+    if (dmgIn) {
+        d.fractureEnergy.resize(paths);
+        d.youngsModulus.assign(paths, mat.E);
+        d.failureStrain.resize(paths);
+        d.geometry.resize(5 * (size_t)paths);
+        for (int p = 0; p < paths; p++) {
+            const double* g = dmgIn + 7 * (size_t)p;
+            d.failureStrain[p] = g[0];
+            d.fractureEnergy[p] = g[1];
+            for (int k = 0; k < 5; k++) d.geometry[5 * (size_t)p + k] = g[2 + k];
+        }
+        d.strength.assign(paths, 0.0);
+        d.softening.assign(paths, 0.0);
+        d.width.assign(paths, 0.0);
+    }
+    std::vector<double> sig(4 * (size_t)paths, 0.0), back(4 * (size_t)paths, 0.0), epsP(paths, 0.0), work(paths, 0.0);
+    for (int p = 0; p < paths; p++) {
+        for (int k = 0; k < steps; k++) {
+            const double* x = inc + 5 * ((size_t)p * steps + k);
+            const double dg = j2_update(sig.data(), back.data(), epsP.data(), work.data(), p, x[0], x[1], x[2],
+                                        x[3], lam, mat.mu(), fl, x[4], dmgIn ? &d : nullptr);
+            double* o = out + 14 * ((size_t)p * steps + k);
+            for (int c = 0; c < 4; c++) o[c] = sig[4 * (size_t)p + c];
+            for (int c = 0; c < 4; c++) o[4 + c] = back[4 * (size_t)p + c];
+            o[8] = epsP[p];
+            o[9] = work[p];
+            o[10] = dg;
+            o[11] = dmgIn ? d.strength[p] : 0.0;
+            o[12] = dmgIn ? d.softening[p] : 0.0;
+            o[13] = dmgIn ? d.width[p] : 0.0;
+        }
+    }
+    return NF_OK;
+}

@@ -176,3 +176,27 @@ def test_run_until_matches_step_count_at_largest_mesh():
     (g, _, gs), (r, _, rs) = run_pair(rod_builder(taylor.ROD_JC, 128, 1280, 0.09), steps=30)
     assert gs.steps == rs.steps == 30
     assert_parity(g, r, "JC 128x1280 prefix")
+
+
+@pytest.mark.parametrize("with_damage", [False, True])
+def test_point_paths_bitwise(hip, oracle, with_damage):
+    """J2 / Johnson-Cook / damage at 4096 random material points: device == oracle."""
+    import pointpath
+    rng = np.random.default_rng(11 if with_damage else 10)
+    paths, steps = 4096, 200
+    inc = np.zeros((paths, steps, 5))
+    inc[..., :4] = rng.normal(0.0, 3e-4, (paths, steps, 4))
+    inc[..., 1] += 2e-4  # a stretching bias so most points yield and many soften
+    inc[..., 4] = rng.uniform(1e-9, 1e-7, (paths, steps))
+    mat = Material("cu", 117e9, 0.35, 8930.0).withJohnsonCook(JohnsonCook.COPPER_OFHC)
+    damage = None
+    if with_damage:
+        damage = np.column_stack([rng.uniform(0.01, 0.1, paths), rng.uniform(1e3, 1e5, paths),
+                                  rng.uniform(0.5e-6, 2e-6, paths), rng.uniform(-1e-3, 1e-3, paths),
+                                  rng.uniform(-1e-3, 1e-3, paths), rng.uniform(-1e-3, 1e-3, paths),
+                                  rng.uniform(-1e-3, 1e-3, paths)])
+    got = pointpath.run(hip, mat, inc, damage)
+    want = pointpath.run(oracle.binding(), mat, inc, damage)
+    assert first_difference(got.reshape(-1), want.reshape(-1)) is None
+    if with_damage:
+        assert (want[:, -1, 11] > 0.0).sum() > 100  # onsets were exercised
