@@ -11,12 +11,14 @@
 // file must be compiled with floating-point contraction off (-ffp-contract=off) on every
 // target, so that no multiply-add is fused on one side and not the other.
 //
-// Methods (all from the mathematics, no library constants):
-//  - log: x = 2^k m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1),
-//    with s carried as a double-double (Dekker's exact product by Veltkamp splitting,
-//    Knuth's two-sum) and the odd series 2 s^3 (1/3 + s^2/5 + ...) to 1/23 in double.
-//  - exp: Cody-Waite reduction by ln 2 (split constant), Taylor series of expm1 on
-//    |r| <= ln2/2 to degree 15, recombined in double-double.
+// Methods (all from the mathematics; the tables in nf_math_tables.h are correctly rounded
+// values computed by gen_math_tables.py with mpmath):
+//  - log: Tang's table method (128 points per octave around 1, Taylor series of log1p to
+//    degree 8 on |r| <= 2^-7.5), double-double recombination with Knuth's two-sum and an
+//    exact product by Veltkamp splitting. No division: the dependency chain is short, which
+//    is what the Johnson-Cook Newton loop waits on.
+//  - exp: table of 2^(j/64) and Cody-Waite reduction by ln2/64 (split constant), Taylor
+//    series of expm1 to degree 7 on |r| <= ln2/128.
 //  - pow(x, y) for x > 0: exp(y log x) with log x carried as a double-double.
 //  - atan: reduction atan t = pi/4 + atan((t-1)/(t+1)) and one half-angle step, then the
 //    Gregory series; atan2 by quadrant.
@@ -26,6 +28,8 @@
 #pragma once
 
 #include <stdint.h>
+
+#include "nf_math_tables.h"
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define NF_HD __host__ __device__ __forceinline__
@@ -77,11 +81,12 @@ NF_HD void two_prod(double a, double b, double& p, double& e) {
     e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
 }
 
-// ln 2 split so that k * LN2_HI is exact for |k| < 2^11 (LN2_HI has 32 significant bits).
-#define NFM_LN2_HI 0x1.62e42fefa0000p-1
-#define NFM_LN2_LO 0x1.cf79abc9e3b3ap-40  // ln2 - LN2_HI, rounded
-
-// log(x) as an unevaluated sum hi + lo (|lo| <= ulp(hi)). x must be positive and finite.
+// log(x) as an unevaluated sum hi + lo (|lo| <= ulp(hi)) for positive finite x.
+// Tang's table method: x = 2^k m, m in [3/4, 3/2); F = j/128 nearest to m (j = 96..192);
+// f = m - F exactly; r = f/F carried as r + r_lo (r = f * (1/F), r_lo from the exact
+// remainder f - r F, which needs no split of F: F has at most 8 significant bits);
+// log(1 + r) = r + r^2 (-1/2 + r/3 - ... - r^6/8) by Estrin's scheme, |r| <= 2^-7.5;
+// log x = k ln2 + log F + log(1 + r), summed as a double-double. No division.
 NF_HD void log_dd(double x, double& hi, double& lo) {
     uint64_t b = bits_of(x);
     int k = 0;
@@ -92,43 +97,37 @@ NF_HD void log_dd(double x, double& hi, double& lo) {
     }
     k += (int)(b >> 52) - 1023;
     double m = from_bits((b & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);
-    if (m > 1.4142135623730951) {
+    if (m >= 1.5) {
         m = m * 0.5;
         k += 1;
     }
-    const double f = m - 1.0;  // exact (Sterbenz)
-
-    // s = f / (2 + f) as a double-double.
-    double d_hi, d_lo;
-    fast_two_sum(2.0, f, d_hi, d_lo);
-    const double s_hi = f / d_hi;
-    double p_hi, p_lo;
-    two_prod(s_hi, d_hi, p_hi, p_lo);
-    const double s_lo = (((f - p_hi) - p_lo) - s_hi * d_lo) / d_hi;
-
-    // Tail 2 s^3 (1/3 + z/5 + ... + z^10/23), z = s^2.
-    const double z = s_hi * s_hi;
-    double t = 1.0 / 23.0;
-    t = 1.0 / 21.0 + z * t;
-    t = 1.0 / 19.0 + z * t;
-    t = 1.0 / 17.0 + z * t;
-    t = 1.0 / 15.0 + z * t;
-    t = 1.0 / 13.0 + z * t;
-    t = 1.0 / 11.0 + z * t;
-    t = 1.0 / 9.0 + z * t;
-    t = 1.0 / 7.0 + z * t;
-    t = 1.0 / 5.0 + z * t;
-    t = 1.0 / 3.0 + z * t;
-    const double tail = 2.0 * (s_hi * z) * t;
-
-    // log m = 2 s_hi + (2 s_lo + tail), then add k ln2.
-    const double lm_hi = 2.0 * s_hi;
-    const double lm_lo = 2.0 * s_lo + tail;
+    const double shifter = 0x1.8p52;
+    const double jd = (m * 128.0 + shifter) - shifter;  // nearest integer, 96..192
+    const int j = (int)jd - 96;
+    const double F = jd * 0x1p-7;
+    const double f = m - F;  // exact (Sterbenz)
+    const double invF = NFM_LOG_TBL[j][0];
+    const double r = f * invF;
+    // exact product r * F = p + e (Veltkamp split of r; F needs none)
+    const double p = r * F;
+    double rh, rl;
+    split(r, rh, rl);
+    const double e = (rh * F - p) + rl * F;
+    const double r_lo = ((f - p) - e) * invF;
+    // log(1 + r) - r, Estrin
+    const double r2 = r * r;
+    const double r4 = r2 * r2;
+    const double a = -0.5 + r * (1.0 / 3.0);
+    const double bq = -0.25 + r * (1.0 / 5.0);
+    const double c = -1.0 / 6.0 + r * (1.0 / 7.0);
+    const double q = (a + bq * r2) + (c + (-0.125) * r2) * r4;
+    const double P = r2 * q;
     const double kd = (double)k;
-    double h, e;
-    two_sum(kd * NFM_LN2_HI, lm_hi, h, e);
-    e = e + (lm_lo + kd * NFM_LN2_LO);
-    fast_two_sum(h, e, hi, lo);
+    double s1, e1, s2, e2;
+    two_sum(kd * NFM_LN2_HI, NFM_LOG_TBL[j][1], s1, e1);
+    two_sum(s1, r, s2, e2);
+    const double l = (e1 + e2) + ((kd * NFM_LN2_LO + NFM_LOG_TBL[j][2]) + (r_lo + P));
+    fast_two_sum(s2, l, hi, lo);
 }
 
 NF_HD double log(double x) {
@@ -139,42 +138,30 @@ NF_HD double log(double x) {
     return hi + lo;
 }
 
-// exp(a + b) with |b| <= ulp(a).
+// exp(a + b) with |b| <= ulp(a). Table method: a + b = (64 k + j) ln2/64 + r, |r| <= ln2/128;
+// exp = 2^k 2^(j/64) (1 + r + r^2 (1/2 + r/6 + ... + r^5/5040)), the polynomial by Estrin's
+// scheme and the table value carried as hi + lo.
 NF_HD double exp_dd(double a, double b) {
     if (is_nan(a)) return a;
     if (a > 709.782712893384) return inf();
     if (a < -745.2) return 0.0;
-    // k = round(a / ln2), by the 1.5 * 2^52 shift (round to nearest even, exact here).
     const double shifter = 0x1.8p52;
-    const double kd = (a * 1.4426950408889634 + shifter) - shifter;
-    const int k = (int)kd;
-    // r = a - k ln2 + b, as a double-double.
-    double r_hi, r_lo;
-    two_sum(a - kd * NFM_LN2_HI, b - kd * NFM_LN2_LO, r_hi, r_lo);
-
-    // expm1(r_hi) = r + r^2 (1/2! + r/3! + ... + r^13/15!).
-    const double r = r_hi;
-    double q = 1.0 / 1307674368000.0;   // 1/15!
-    q = 1.0 / 87178291200.0 + r * q;    // 1/14!
-    q = 1.0 / 6227020800.0 + r * q;     // 1/13!
-    q = 1.0 / 479001600.0 + r * q;      // 1/12!
-    q = 1.0 / 39916800.0 + r * q;       // 1/11!
-    q = 1.0 / 3628800.0 + r * q;        // 1/10!
-    q = 1.0 / 362880.0 + r * q;         // 1/9!
-    q = 1.0 / 40320.0 + r * q;          // 1/8!
-    q = 1.0 / 5040.0 + r * q;           // 1/7!
-    q = 1.0 / 720.0 + r * q;            // 1/6!
-    q = 1.0 / 120.0 + r * q;            // 1/5!
-    q = 1.0 / 24.0 + r * q;             // 1/4!
-    q = 1.0 / 6.0 + r * q;              // 1/3!
-    q = 0.5 + r * q;                    // 1/2!
-    double em_hi, em_lo;
-    two_sum(r, (r * r) * q, em_hi, em_lo);
-    // exp(r_hi + r_lo) = 1 + em + r_lo (1 + em)
-    double s, e;
-    fast_two_sum(1.0, em_hi, s, e);
-    const double y = s + (e + (em_lo + r_lo * (1.0 + em_hi)));
-
+    const double nd = (a * NFM_INV_L64 + shifter) - shifter;
+    const int n = (int)nd;
+    const int j = n & 63;
+    const int k = (n - j) / 64;
+    const double t = a - nd * NFM_L64_HI;  // exact
+    double r, rl;
+    two_sum(t, b - nd * NFM_L64_LO, r, rl);
+    const double r2 = r * r;
+    const double qa = 0.5 + r * (1.0 / 6.0);
+    const double qb = 1.0 / 24.0 + r * (1.0 / 120.0);
+    const double qc = 1.0 / 720.0 + r * (1.0 / 5040.0);
+    const double Q = (qa + qb * r2) + qc * (r2 * r2);
+    const double E = rl + r2 * Q;  // exp(r + rl) = 1 + r + E
+    const double p = r + E;
+    const double th = NFM_EXP_TBL[j][0], tl = NFM_EXP_TBL[j][1];
+    const double y = th + (tl + (th * r + (th * E + tl * p)));
     // Scale by 2^k exactly where the result is normal.
     if (k > 1023) {
         return (y * from_bits((uint64_t)(1023 + 1023) << 52)) *
