@@ -52,7 +52,8 @@ __device__ __forceinline__ double jc_strainTerm(const Mat& j, double e) {
 }
 __device__ __forceinline__ double jc_rateTerm(const Mat& j, double rate) {
     if (rate <= j.eps0) return 1.0;
-    return 1.0 + j.c * nfm::log(rate / j.eps0);
+    // x / 1.0 is x exactly, so the division is skipped at the usual reference rate of 1/s.
+    return 1.0 + j.c * nfm::log(j.eps0 == 1.0 ? rate : rate / j.eps0);
 }
 __device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t) {
     const double homologous = (t - j.troom) / (j.tmelt - j.troom);
@@ -65,9 +66,10 @@ __device__ __forceinline__ double jc_quasiStatic(const Mat& j, double e, double 
 }
 
 // solver/J2.java:332-375, the bracketed Newton solve for the Johnson-Cook multiplier.
+// `thermal` is thermalTerm(temperature), which the caller has already evaluated for the
+// yield check on the same temperature (J2.java:221 and :334 compute the same value twice).
 __device__ double j2_solve(double norm, double stiffness, double plasticStrain,
-                           double temperature, double dt, const Mat& j) {
-    const double thermal = jc_thermalTerm(j, temperature);
+                           double thermal, double dt, const Mat& j) {
     const double tolerance = 1.0e-10 * j.a;
     double lo = 0.0;
     double hi = norm / stiffness;
@@ -138,10 +140,13 @@ struct PointState {
     double epsP, work;
 };
 
-// solver/J2.java:163-296. `dmg` is null when the solver has no damage model.
-template <bool DAMAGE>
+// solver/J2.java:163-296. `dmg` is null when the solver has no damage model. KIN is false
+// when no material has kinematic hardening: the back stress is then +0 forever (its
+// co-rotation and its update only ever add signed zeros to +0), so it is neither loaded nor
+// stored, and subtracting it or adding alpha : n changes no bit of any result.
+template <bool DAMAGE, bool KIN>
 __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, double dGamRZ,
-                          const Mat& fl, double strainDt, DamagePoint* dmg) {
+                            const Mat& fl, double strainDt, DamagePoint* dmg) {
     const double lambda = fl.lambda, mu = fl.mu;
     const double lt = lambda * (dRR + dZZ + dTT);
     const double tRR = q.s0 + lt + 2.0 * mu * dRR;
@@ -150,10 +155,10 @@ __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, d
     const double tRZ = q.s3 + mu * dGamRZ;
 
     const double p = (tRR + tZZ + tTT) / 3.0;
-    const double xRR = tRR - p - q.b0;
-    const double xZZ = tZZ - p - q.b1;
-    const double xTT = tTT - p - q.b2;
-    const double xRZ = tRZ - q.b3;
+    const double xRR = KIN ? tRR - p - q.b0 : tRR - p;
+    const double xZZ = KIN ? tZZ - p - q.b1 : tZZ - p;
+    const double xTT = KIN ? tTT - p - q.b2 : tTT - p;
+    const double xRZ = KIN ? tRZ - q.b3 : tRZ;
     const double norm = __builtin_sqrt(xRR * xRR + xZZ * xZZ + xTT * xTT + 2.0 * xRZ * xRZ);
 
     const bool jc = fl.hasJC != 0;
@@ -162,6 +167,7 @@ __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, d
     const bool softening = DAMAGE && dmg->strength > 0.0;
     double hardening = 0.0;
     double flowStress;
+    double thermal = 1.0;
     if (softening) {
         flowStress = dmg->strength + dmg->softening * (q.epsP - dmg->failureStrain);
         if (flowStress > 0.0) {
@@ -173,7 +179,8 @@ __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, d
         flowStress = fl.sy0 + fl.hIso * q.epsP;
         hardening = fl.hIso;
     } else {
-        flowStress = jc_quasiStatic(fl, q.epsP, temperature);
+        thermal = jc_thermalTerm(fl, temperature);
+        flowStress = jc_strainTerm(fl, q.epsP) * thermal;  // quasiStaticFlowStress
     }
     const double radius = NFD_SQRT_2_3 * flowStress;
 
@@ -193,7 +200,7 @@ __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, d
             dGamma = norm / stiffness;
         }
     } else {
-        dGamma = j2_solve(norm, stiffness, q.epsP, temperature, strainDt, fl);
+        dGamma = j2_solve(norm, stiffness, q.epsP, thermal, strainDt, fl);
     }
     const double inv = 1.0 / norm;
     const double nRR = xRR * inv, nZZ = xZZ * inv, nTT = xTT * inv, nRZ = xRZ * inv;
@@ -204,19 +211,23 @@ __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, d
     q.s2 = tTT - pull * nTT;
     q.s3 = tRZ - pull * nRZ;
 
-    const double alphaDotN = q.b0 * nRR + q.b1 * nZZ + q.b2 * nTT + 2.0 * q.b3 * nRZ;
-    q.work += dGamma * (norm - pull + alphaDotN);
-
-    const double shift = NFD_TWO_THIRDS * fl.hKin * dGamma;
-    q.b0 += shift * nRR;
-    q.b1 += shift * nZZ;
-    q.b2 += shift * nTT;
-    q.b3 += shift * nRZ;
+    if (KIN) {
+        const double alphaDotN = q.b0 * nRR + q.b1 * nZZ + q.b2 * nTT + 2.0 * q.b3 * nRZ;
+        q.work += dGamma * (norm - pull + alphaDotN);
+        const double shift = NFD_TWO_THIRDS * fl.hKin * dGamma;
+        q.b0 += shift * nRR;
+        q.b1 += shift * nZZ;
+        q.b2 += shift * nTT;
+        q.b3 += shift * nRZ;
+    } else {
+        q.work += dGamma * (norm - pull);
+    }
 
     q.epsP += NFD_SQRT_2_3 * dGamma;
 
     if (DAMAGE && !softening && q.epsP >= dmg->failureStrain) {
-        damage_begin(*dmg, !jc ? fl.sy0 + fl.hIso * q.epsP : jc_quasiStatic(fl, q.epsP, temperature),
+        // the onset flow stress: the same temperature, so the same thermal factor
+        damage_begin(*dmg, !jc ? fl.sy0 + fl.hIso * q.epsP : jc_strainTerm(fl, q.epsP) * thermal,
                      nRR, nZZ, nRZ);
     }
     return dGamma;

@@ -22,6 +22,7 @@
 //   elements  conn[E] int4, matIndex[E], gone[E]
 //   points    sig0..3[P], back0..3[P], epsP[P], work[P]; hgR, hgZ, hgK [E]
 //   slots     slot[4 E + 2 edges] (gr, gz); CSR slotStart[N + 1], slotOf[4 E + 2 edges]
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -85,6 +86,7 @@ struct Dev {
     Ctl* ctl;
     int N, E, P, edges, ppe;
     const double2* x0;
+    const double* z0;  // axial reference coordinate alone, for the wall test in the node pass
     double2* u;
     double2* v;
     double2* f;
@@ -191,7 +193,7 @@ __device__ __forceinline__ void nodal_node(const Dev& d, int i, double2 fi, doub
         contacts++;
     } else {
         const double trial = decay * vi.y + gain * fi.y * invM;
-        const double zc = d.x0[i].y + ui.y;
+        const double zc = d.z0[i] + ui.y;
         if (WALL && zc + trial * dt < d.plane) {
             acc[0] += -m * trial;  // RigidWall.arrive :201-205
             acc[1] += 0.5 * m * trial * trial;
@@ -376,18 +378,26 @@ __device__ __forceinline__ double currentArea(const Dev& d, int4 n) {
     return 0.5 * a;
 }
 
+template <bool KIN>
 __device__ __forceinline__ nfd::PointState load_point(const Dev& d, int p) {
     nfd::PointState q;
     q.s0 = d.sig0[p]; q.s1 = d.sig1[p]; q.s2 = d.sig2[p]; q.s3 = d.sig3[p];
-    q.b0 = d.back0[p]; q.b1 = d.back1[p]; q.b2 = d.back2[p]; q.b3 = d.back3[p];
+    if (KIN) {
+        q.b0 = d.back0[p]; q.b1 = d.back1[p]; q.b2 = d.back2[p]; q.b3 = d.back3[p];
+    } else {
+        q.b0 = q.b1 = q.b2 = q.b3 = 0.0;
+    }
     q.epsP = d.epsP[p];
     q.work = d.work[p];
     return q;
 }
 
+template <bool KIN>
 __device__ __forceinline__ void store_point(const Dev& d, int p, const nfd::PointState& q) {
     d.sig0[p] = q.s0; d.sig1[p] = q.s1; d.sig2[p] = q.s2; d.sig3[p] = q.s3;
-    d.back0[p] = q.b0; d.back1[p] = q.b1; d.back2[p] = q.b2; d.back3[p] = q.b3;
+    if (KIN) {
+        d.back0[p] = q.b0; d.back1[p] = q.b1; d.back2[p] = q.b2; d.back3[p] = q.b3;
+    }
     d.epsP[p] = q.epsP;
     d.work[p] = q.work;
 }
@@ -466,7 +476,7 @@ __device__ __forceinline__ void write_zero_slots(const Dev& d, int e) {
 }
 
 // One element under reduced integration, solver/ExplicitSolver.java:1373-1504.
-template <bool DAMAGE>
+template <bool DAMAGE, bool KIN>
 __device__ double reduced_element(const Dev& d, int e, int4 n, double strainDt) {
     const Mat fl = load_mat(d, e);
     const double mu = fl.mu;
@@ -514,19 +524,19 @@ __device__ double reduced_element(const Dev& d, int e, int4 n, double strainDt) 
         dEpsT = 0.25 * (a0 + a1 + a2 + a3) / rg;
     }
 
-    nfd::PointState q = load_point(d, e);
+    nfd::PointState q = load_point<KIN>(d, e);
     if (d.finite) {
         const double spinDt = 0.5 * (dZdR - dRdZ);
         if (spinDt != 0.0) {
             const double cs = nfd::corot_cos(spinDt), sn = nfd::corot_sin(spinDt);
             nfd::corot_rotate(q.s0, q.s1, q.s3, cs, sn);
-            nfd::corot_rotate(q.b0, q.b1, q.b3, cs, sn);
+            if (KIN) nfd::corot_rotate(q.b0, q.b1, q.b3, cs, sn);
         }
     }
     nfd::DamagePoint dp;
     if (DAMAGE) load_damage(d, e, dp);
-    nfd::j2_update<DAMAGE>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp);
-    store_point(d, e, q);
+    nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp);
+    store_point<KIN>(d, e, q);
     if (DAMAGE) {
         d.dStrength[e] = dp.strength;
         d.dSoftening[e] = dp.softening;
@@ -578,7 +588,7 @@ __device__ double reduced_element(const Dev& d, int e, int4 n, double strainDt) 
 }
 
 // One element under full 2x2 integration, solver/ExplicitSolver.java:1219-1339.
-template <bool DAMAGE>
+template <bool DAMAGE, bool KIN>
 __device__ double full_element(const Dev& d, int e, int4 n, double strainDt) {
     const Mat fl = load_mat(d, e);
     const double mid = -0.5 * strainDt;
@@ -637,19 +647,19 @@ __device__ double full_element(const Dev& d, int e, int4 n, double strainDt) {
             dEpsT = (N0 * a0 + N1 * a1 + N2 * a2 + N3 * a3) / rg;
         }
         const int p = e * 4 + g;
-        nfd::PointState q = load_point(d, p);
+        nfd::PointState q = load_point<KIN>(d, p);
         if (d.finite) {
             const double spinDt = 0.5 * (dZdR - dRdZ);
             if (spinDt != 0.0) {
                 const double cs = nfd::corot_cos(spinDt), sn = nfd::corot_sin(spinDt);
                 nfd::corot_rotate(q.s0, q.s1, q.s3, cs, sn);
-                nfd::corot_rotate(q.b0, q.b1, q.b3, cs, sn);
+                if (KIN) nfd::corot_rotate(q.b0, q.b1, q.b3, cs, sn);
             }
         }
         nfd::DamagePoint dp;
         if (DAMAGE) load_damage(d, p, dp);
-        nfd::j2_update<DAMAGE>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp);
-        store_point(d, p, q);
+        nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp);
+        store_point<KIN>(d, p, q);
         if (DAMAGE) {
             d.dStrength[p] = dp.strength;
             d.dSoftening[p] = dp.softening;
@@ -720,8 +730,8 @@ __device__ void pressure_edge(const Dev& d, int k, double p) {
     d.slot[s + 1] = make_double2(p * nr * wb, p * nz * wb);
 }
 
-template <bool FULL, bool DAMAGE>
-__global__ void __launch_bounds__(BLOCK) k_element(Dev d, int q, int mode) {
+template <bool FULL, bool DAMAGE, bool KIN>
+__global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev d, int q, int mode) {
     Ctl* c = d.ctl;
     Step& st = c->s[q];
     if (mode == MODE_STEP && !st.go) return;
@@ -741,7 +751,7 @@ __global__ void __launch_bounds__(BLOCK) k_element(Dev d, int q, int mode) {
         if (gone) {
             write_zero_slots(d, e);
         } else {
-            edge = FULL ? full_element<DAMAGE>(d, e, n, strainDt) : reduced_element<DAMAGE>(d, e, n, strainDt);
+            edge = FULL ? full_element<DAMAGE, KIN>(d, e, n, strainDt) : reduced_element<DAMAGE, KIN>(d, e, n, strainDt);
             sawNaN = edge != edge;
         }
     } else if (e < d.E + d.edges) {
@@ -871,7 +881,7 @@ __global__ void k_point_path(Mat m, int paths, int steps, const double* inc, con
     }
     for (int k = 0; k < steps; k++) {
         const double* x = inc + 5 * ((size_t)p * steps + k);
-        const double dg = nfd::j2_update<DAMAGE>(q, x[0], x[1], x[2], x[3], m, x[4], &dp);
+        const double dg = nfd::j2_update<DAMAGE, true>(q, x[0], x[1], x[2], x[3], m, x[4], &dp);
         double* o = out + 14 * ((size_t)p * steps + k);
         o[0] = q.s0; o[1] = q.s1; o[2] = q.s2; o[3] = q.s3;
         o[4] = q.b0; o[5] = q.b1; o[6] = q.b2; o[7] = q.b3;
@@ -1009,6 +1019,7 @@ struct nf_solver {
     double fastestWave = 0.0, dt0 = 0.0, massScale = 1.0;
     bool hasWall = false, hasDamage = false, hasErosion = false, hasContact = false;
     bool anyJC = false;
+    bool hasKin = false;  // some material has kinematic hardening
     double wallPlane = 0.0, wallFriction = 0.0;
     // Optional per-kernel timing with HIP events on the handle's stream (nf_profile).
     bool profiling = false;
@@ -1024,6 +1035,9 @@ struct nf_solver {
         if (stream) (void)hipStreamSynchronize(stream);
         dropGraphs();
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
+        for (hipEvent_t e : statusEvent)
+            if (e) (void)hipEventDestroy(e);
+        if (statusHost) (void)hipHostFree(statusHost);
         for (void* p : owned) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1038,17 +1052,15 @@ struct nf_solver {
         return NF_OK;
     }
 
-    // Records an event pair around the launch `fn` when profiling.
+    // Runs the launch `fn(start, stop)`, with a fresh event pair when profiling.
     template <typename F>
     int timed(int kernel, F fn) {
-        if (!profiling) return fn();
+        if (!profiling) return fn(nullptr, nullptr);
         const int base = 2 * (int)evPending.size();
         hipEvent_t a, b;
         TRY(event(base, &a));
         TRY(event(base + 1, &b));
-        HIP_TRY(hipEventRecord(a, stream));
-        TRY(fn());
-        HIP_TRY(hipEventRecord(b, stream));
+        TRY(fn(a, b));
         evPending.push_back({kernel, {base, base + 1}});
         return NF_OK;
     }
@@ -1108,29 +1120,49 @@ struct nf_solver {
         return NF_OK;
     }
 
-    int launchElement(int mode, int parity) {
+    // Launches with optional start/stop events recorded by the dispatch itself
+    // (hipExtLaunchKernelGGL), so a profiled duration is the kernel's own, not the gap
+    // around it.
+    template <bool FULL, bool DAMAGE, bool KIN>
+    int launchElementT(int mode, int parity, hipEvent_t a, hipEvent_t b) {
         const int grid = blocks_for(E + edges);
-        const bool full = integration == NF_FULL;
-        if (full && hasDamage) hipLaunchKernelGGL((k_element<true, true>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
-        else if (full) hipLaunchKernelGGL((k_element<true, false>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
-        else if (hasDamage) hipLaunchKernelGGL((k_element<false, true>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
-        else hipLaunchKernelGGL((k_element<false, false>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
+        if (a) hipExtLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, mode);
+        else hipLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
 
-    int launchGN(int phase, int parity, int gatherFirst) {
-        if (hasWall) hipLaunchKernelGGL(k_gn<true>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst);
-        else hipLaunchKernelGGL(k_gn<false>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst);
+    int launchElement(int mode, int parity, hipEvent_t a = nullptr, hipEvent_t b = nullptr) {
+        switch ((integration == NF_FULL ? 4 : 0) | (hasDamage ? 2 : 0) | (hasKin ? 1 : 0)) {
+            case 0: return launchElementT<false, false, false>(mode, parity, a, b);
+            case 1: return launchElementT<false, false, true>(mode, parity, a, b);
+            case 2: return launchElementT<false, true, false>(mode, parity, a, b);
+            case 3: return launchElementT<false, true, true>(mode, parity, a, b);
+            case 4: return launchElementT<true, false, false>(mode, parity, a, b);
+            case 5: return launchElementT<true, false, true>(mode, parity, a, b);
+            case 6: return launchElementT<true, true, false>(mode, parity, a, b);
+            default: return launchElementT<true, true, true>(mode, parity, a, b);
+        }
+    }
+
+    template <bool WALL>
+    int launchGNT(int phase, int parity, int gatherFirst, hipEvent_t a, hipEvent_t b) {
+        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst);
+        else hipLaunchKernelGGL(k_gn<WALL>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst);
         HIP_TRY(hipGetLastError());
         return NF_OK;
+    }
+
+    int launchGN(int phase, int parity, int gatherFirst, hipEvent_t a = nullptr, hipEvent_t b = nullptr) {
+        return hasWall ? launchGNT<true>(phase, parity, gatherFirst, a, b)
+                       : launchGNT<false>(phase, parity, gatherFirst, a, b);
     }
 
     // One step on the stream: the element pass of the open step, then the launch that
     // gathers it and opens the next. The device decides whether either does anything.
     int enqueueStep() {
-        TRY(timed(1, [&] { return launchElement(MODE_STEP, q); }));
-        TRY(timed(0, [&] { return launchGN(PHASE_STEP, q, 0); }));
+        TRY(timed(1, [&](hipEvent_t a, hipEvent_t b) { return launchElement(MODE_STEP, q, a, b); }));
+        TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_STEP, q, 0, a, b); }));
         q ^= 1;
         return NF_OK;
     }
@@ -1139,6 +1171,8 @@ struct nf_solver {
     // the same at both ends). Captured on first use and whenever a setter changes a uniform.
     static constexpr int GRAPH_STEPS = 32;
     hipGraphExec_t graphExec[2] = {nullptr, nullptr};
+    Step* statusHost = nullptr;  // pinned
+    hipEvent_t statusEvent[2] = {nullptr, nullptr};
     bool graphDirty = true;
     bool useGraphs = true;
 
@@ -1204,24 +1238,43 @@ struct nf_solver {
         // Open the first step of this call.
         if (h.steps == 0) {
             TRY(launchElement(MODE_INITIAL, q));  // :945
-            TRY(timed(0, [&] { return launchGN(PHASE_START, q, 1); }));
+            TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 1, a, b); }));
         } else {
-            TRY(timed(0, [&] { return launchGN(PHASE_START, q, 0); }));
+            TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 0, a, b); }));
         }
         q ^= 1;
-        long long done = 0;  // steps enqueued in this call
-        while (true) {
-            int n = GRAPH_STEPS;
-            if (maxSteps > 0) {
-                const long long left = maxSteps - done;
-                n = (int)(left < GRAPH_STEPS ? left : GRAPH_STEPS);
+        if (maxSteps > 0) {
+            // A known count: enqueue every step, synchronise once at the end.
+            for (long long left = maxSteps; left > 0;) {
+                const int n = (int)(left < GRAPH_STEPS ? left : GRAPH_STEPS);
+                TRY(launchSteps(n));
+                left -= n;
             }
-            TRY(launchSteps(n));
-            done += n;
-            TRY(readStep(h));
-            TRY(harvest());
-            if (!h.go) break;
+        } else {
+            // Open-ended: keep one block of steps in flight beyond the one whose status is
+            // being read, so the GPU never idles while the host decides. Blocks enqueued
+            // after the end test fails run as no-ops on the device.
+            if (!statusHost) {
+                HIP_TRY(hipHostMalloc((void**)&statusHost, 2 * sizeof(Step), hipHostMallocDefault));
+                HIP_TRY(hipEventCreateWithFlags(&statusEvent[0], hipEventDisableTiming));
+                HIP_TRY(hipEventCreateWithFlags(&statusEvent[1], hipEventDisableTiming));
+            }
+            int slot = 0;
+            bool pending = false;
+            while (true) {
+                TRY(launchSteps(GRAPH_STEPS));
+                HIP_TRY(hipMemcpyAsync(&statusHost[slot], &ctl->s[q], sizeof(Step), hipMemcpyDeviceToHost, stream));
+                HIP_TRY(hipEventRecord(statusEvent[slot], stream));
+                if (pending) {
+                    HIP_TRY(hipEventSynchronize(statusEvent[slot ^ 1]));
+                    if (!statusHost[slot ^ 1].go) break;
+                }
+                pending = true;
+                slot ^= 1;
+            }
         }
+        TRY(readStep(h));
+        TRY(harvest());
         if (taken) *taken = h.steps - start;
         return NF_OK;
     }
@@ -1313,6 +1366,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
     for (int m = 0; m < material_count; m++) {
         table[m] = make_mat(s->materials[m].raw, formulation);
         if (table[m].hasJC) s->anyJC = true;
+        if (table[m].hKin != 0.0) s->hasKin = true;
     }
     // Materials.fastestWave core/Materials.java:120-128
     if (s->materialOf.empty()) {
@@ -1359,12 +1413,13 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
     d.erosionDamage = HUGE_VAL;
 
     double2* x0 = nullptr;
+    double* z0 = nullptr;
     double* mass = nullptr;
     int4* conn4 = nullptr;
     Mat* mats = nullptr;
     int* slotStart = nullptr;
     int* slotOf = nullptr;
-    if ((rc = s->alloc(&s->ctl, 1)) || (rc = s->alloc(&x0, N)) || (rc = s->alloc(&d.u, N)) ||
+    if ((rc = s->alloc(&s->ctl, 1)) || (rc = s->alloc(&x0, N)) || (rc = s->alloc(&z0, N)) || (rc = s->alloc(&d.u, N)) ||
         (rc = s->alloc(&d.v, N)) || (rc = s->alloc(&d.f, N)) || (rc = s->alloc(&mass, N)) ||
         (rc = s->alloc(&d.flags, N)) || (rc = s->alloc(&conn4, E)) ||
         (rc = s->alloc(&mats, material_count)) || (rc = s->alloc(&d.sig0, s->P)) ||
@@ -1382,6 +1437,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         return bail(rc);
     d.ctl = s->ctl;
     d.x0 = x0;
+    d.z0 = z0;
     d.mass = mass;
     d.conn = conn4;
     d.mats = mats;
@@ -1406,6 +1462,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
     std::vector<double> dens(E);
     for (int e = 0; e < E; e++) dens[e] = table[s->materialOf.empty() ? 0 : s->materialOf[e]].density;
     if (hipMemcpy(x0, x0h.data(), sizeof(double2) * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(z0, z, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(conn4, conn, sizeof(int4) * E, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(mats, table.data(), sizeof(Mat) * material_count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(slotStart, s->slotStart.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice) != hipSuccess ||

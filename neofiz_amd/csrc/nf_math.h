@@ -83,10 +83,11 @@ NF_HD void two_prod(double a, double b, double& p, double& e) {
 
 // log(x) as an unevaluated sum hi + lo (|lo| <= ulp(hi)) for positive finite x.
 // Tang's table method: x = 2^k m, m in [3/4, 3/2); F = j/128 nearest to m (j = 96..192);
-// f = m - F exactly; r = f/F carried as r + r_lo (r = f * (1/F), r_lo from the exact
-// remainder f - r F, which needs no split of F: F has at most 8 significant bits);
-// log(1 + r) = r + r^2 (-1/2 + r/3 - ... - r^6/8) by Estrin's scheme, |r| <= 2^-7.5;
-// log x = k ln2 + log F + log(1 + r), summed as a double-double. No division.
+// f = m - F exactly; r = f/F carried as r + r_lo (r by one correctly rounded division, r_lo
+// from the exact remainder f - r F, which needs no split of F: F has at most 8 significant
+// bits); log(1 + r) = r + r^2 (-1/2 + r/3 - ... - r^6/8) by Estrin's scheme, |r| <= 2^-7.5;
+// log x = k ln2 + log F + log(1 + r), summed as a double-double. The table supplies only
+// log F, which is not needed until the end, so its load is off the dependency chain.
 NF_HD void log_dd(double x, double& hi, double& lo) {
     uint64_t b = bits_of(x);
     int k = 0;
@@ -106,14 +107,14 @@ NF_HD void log_dd(double x, double& hi, double& lo) {
     const int j = (int)jd - 96;
     const double F = jd * 0x1p-7;
     const double f = m - F;  // exact (Sterbenz)
-    const double invF = NFM_LOG_TBL[j][0];
-    const double r = f * invF;
-    // exact product r * F = p + e (Veltkamp split of r; F needs none)
+    // r = f / F correctly rounded, and its remainder exactly: r F = p + e (Veltkamp split of
+    // r; F has at most 8 significant bits and needs none), so f - r F = (f - p) - e.
+    const double r = f / F;
     const double p = r * F;
     double rh, rl;
     split(r, rh, rl);
     const double e = (rh * F - p) + rl * F;
-    const double r_lo = ((f - p) - e) * invF;
+    const double r_lo = ((f - p) - e) / F;
     // log(1 + r) - r, Estrin
     const double r2 = r * r;
     const double r4 = r2 * r2;
