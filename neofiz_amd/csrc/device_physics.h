@@ -47,29 +47,28 @@ struct Mat {
 };
 
 // core/JohnsonCook.java:138-197
-__device__ __forceinline__ double jc_strainTerm(const Mat& j, double e) {
-    return e <= 0.0 ? j.a : j.a + j.b * nfm::pow(e, j.n);
+// T: the log/exp tables, read from LDS in the kernels (nfm::Tables).
+__device__ __forceinline__ double jc_strainTerm(const Mat& j, double e, const nfm::Tables& T) {
+    return e <= 0.0 ? j.a : j.a + j.b * nfm::pow(e, j.n, T);
 }
-__device__ __forceinline__ double jc_rateTerm(const Mat& j, double rate) {
+__device__ __forceinline__ double jc_rateTerm(const Mat& j, double rate, const nfm::Tables& T) {
     if (rate <= j.eps0) return 1.0;
     // x / 1.0 is x exactly, so the division is skipped at the usual reference rate of 1/s.
-    return 1.0 + j.c * nfm::log(j.eps0 == 1.0 ? rate : rate / j.eps0);
+    return 1.0 + j.c * nfm::log(j.eps0 == 1.0 ? rate : rate / j.eps0, T);
 }
-__device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t) {
+__device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t, const nfm::Tables& T) {
     const double homologous = (t - j.troom) / (j.tmelt - j.troom);
     if (homologous <= 0.0) return 1.0;
     if (homologous >= 1.0) return 0.0;
-    return 1.0 - nfm::pow(homologous, j.m);
-}
-__device__ __forceinline__ double jc_quasiStatic(const Mat& j, double e, double t) {
-    return jc_strainTerm(j, e) * jc_thermalTerm(j, t);
+    return 1.0 - nfm::pow(homologous, j.m, T);
 }
 
 // solver/J2.java:332-375, the bracketed Newton solve for the Johnson-Cook multiplier.
 // `thermal` is thermalTerm(temperature), which the caller has already evaluated for the
 // yield check on the same temperature (J2.java:221 and :334 compute the same value twice).
-__device__ double j2_solve(double norm, double stiffness, double plasticStrain,
-                           double thermal, double dt, const Mat& j) {
+__device__ __forceinline__ double j2_solve(double norm, double stiffness, double plasticStrain,
+                                           double thermal, double dt, const Mat& j,
+                                           const nfm::Tables& T) {
     const double tolerance = 1.0e-10 * j.a;
     double lo = 0.0;
     double hi = norm / stiffness;
@@ -78,8 +77,8 @@ __device__ double j2_solve(double norm, double stiffness, double plasticStrain,
         const double dEps = NFD_SQRT_2_3 * dGamma;
         const double eps = plasticStrain + dEps;
         const double rate = dt > 0.0 ? dEps / dt : 0.0;
-        const double strain = jc_strainTerm(j, eps);
-        const double rateFactor = jc_rateTerm(j, rate);
+        const double strain = jc_strainTerm(j, eps, T);
+        const double rateFactor = jc_rateTerm(j, rate, T);
         const double g = norm - stiffness * dGamma - NFD_SQRT_2_3 * strain * rateFactor * thermal;
         if (g > 0.0) lo = dGamma;
         else hi = dGamma;
@@ -103,7 +102,7 @@ struct DamagePoint {
 };
 
 // solver/Damage.java:306-336
-__device__ void damage_begin(DamagePoint& d, double flowStress, double nRR, double nZZ,
+__device__ __forceinline__ void damage_begin(DamagePoint& d, double flowStress, double nRR, double nZZ,
                              double nRZ) {
     const double diff = nRR - nZZ;
     const double cross = 2.0 * nRZ;
@@ -133,6 +132,14 @@ __device__ __forceinline__ double damage_at(double strength, double softening,
     return d <= 0.0 ? 0.0 : (d >= 1.0 ? 1.0 : d);
 }
 
+// Copies the log/exp tables into the workgroup's LDS (call from every thread, before use).
+__device__ __forceinline__ nfm::Tables stage_tables(double (*sLog)[3], double (*sExp)[2]) {
+    for (int i = threadIdx.x; i < 97 * 3; i += blockDim.x) (&sLog[0][0])[i] = (&NFM_LOG_TBL[0][0])[i];
+    for (int i = threadIdx.x; i < 64 * 2; i += blockDim.x) (&sExp[0][0])[i] = (&NFM_EXP_TBL[0][0])[i];
+    __syncthreads();
+    return nfm::Tables{sLog, sExp};
+}
+
 // Point state carried through the update.
 struct PointState {
     double s0, s1, s2, s3;  // stress rr, zz, tt, rz
@@ -145,8 +152,9 @@ struct PointState {
 // co-rotation and its update only ever add signed zeros to +0), so it is neither loaded nor
 // stored, and subtracting it or adding alpha : n changes no bit of any result.
 template <bool DAMAGE, bool KIN>
-__device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, double dGamRZ,
-                            const Mat& fl, double strainDt, DamagePoint* dmg) {
+__device__ __forceinline__ double j2_update(PointState& q, double dRR, double dZZ, double dTT,
+                                            double dGamRZ, const Mat& fl, double strainDt,
+                                            DamagePoint* dmg, const nfm::Tables& T) {
     const double lambda = fl.lambda, mu = fl.mu;
     const double lt = lambda * (dRR + dZZ + dTT);
     const double tRR = q.s0 + lt + 2.0 * mu * dRR;
@@ -179,8 +187,8 @@ __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, d
         flowStress = fl.sy0 + fl.hIso * q.epsP;
         hardening = fl.hIso;
     } else {
-        thermal = jc_thermalTerm(fl, temperature);
-        flowStress = jc_strainTerm(fl, q.epsP) * thermal;  // quasiStaticFlowStress
+        thermal = jc_thermalTerm(fl, temperature, T);
+        flowStress = jc_strainTerm(fl, q.epsP, T) * thermal;  // quasiStaticFlowStress
     }
     const double radius = NFD_SQRT_2_3 * flowStress;
 
@@ -200,7 +208,7 @@ __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, d
             dGamma = norm / stiffness;
         }
     } else {
-        dGamma = j2_solve(norm, stiffness, q.epsP, thermal, strainDt, fl);
+        dGamma = j2_solve(norm, stiffness, q.epsP, thermal, strainDt, fl, T);
     }
     const double inv = 1.0 / norm;
     const double nRR = xRR * inv, nZZ = xZZ * inv, nTT = xTT * inv, nRZ = xRZ * inv;
@@ -227,7 +235,7 @@ __device__ double j2_update(PointState& q, double dRR, double dZZ, double dTT, d
 
     if (DAMAGE && !softening && q.epsP >= dmg->failureStrain) {
         // the onset flow stress: the same temperature, so the same thermal factor
-        damage_begin(*dmg, !jc ? fl.sy0 + fl.hIso * q.epsP : jc_strainTerm(fl, q.epsP) * thermal,
+        damage_begin(*dmg, !jc ? fl.sy0 + fl.hIso * q.epsP : jc_strainTerm(fl, q.epsP, T) * thermal,
                      nRR, nZZ, nRZ);
     }
     return dGamma;

@@ -95,6 +95,7 @@ struct Dev {
     const int4* conn;
     const int* matIndex;  // null when uniform
     const Mat* mats;
+    Mat mat0;             // the uniform material, by value (kernel arguments sit in SGPRs)
     uint8_t* gone;        // null when erosion is off
     double* sig0; double* sig1; double* sig2; double* sig3;
     double* back0; double* back1; double* back2; double* back3;
@@ -362,7 +363,8 @@ __device__ void reduce_wall_partials(const Dev& d) {
 // ------------------------------------------------------------------ element pass
 
 __device__ __forceinline__ Mat load_mat(const Dev& d, int e) {
-    return d.matIndex ? d.mats[d.matIndex[e]] : d.mats[0];
+    if (d.matIndex) return d.mats[d.matIndex[e]];
+    return d.mat0;
 }
 
 // Signed current area, solver/ExplicitSolver.java:681-691.
@@ -477,7 +479,8 @@ __device__ __forceinline__ void write_zero_slots(const Dev& d, int e) {
 
 // One element under reduced integration, solver/ExplicitSolver.java:1373-1504.
 template <bool DAMAGE, bool KIN>
-__device__ double reduced_element(const Dev& d, int e, int4 n, double strainDt) {
+__device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, double strainDt,
+                                       const nfm::Tables& T) {
     const Mat fl = load_mat(d, e);
     const double mu = fl.mu;
     const double mid = -0.5 * strainDt;
@@ -535,7 +538,7 @@ __device__ double reduced_element(const Dev& d, int e, int4 n, double strainDt) 
     }
     nfd::DamagePoint dp;
     if (DAMAGE) load_damage(d, e, dp);
-    nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp);
+    nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp, T);
     store_point<KIN>(d, e, q);
     if (DAMAGE) {
         d.dStrength[e] = dp.strength;
@@ -589,7 +592,8 @@ __device__ double reduced_element(const Dev& d, int e, int4 n, double strainDt) 
 
 // One element under full 2x2 integration, solver/ExplicitSolver.java:1219-1339.
 template <bool DAMAGE, bool KIN>
-__device__ double full_element(const Dev& d, int e, int4 n, double strainDt) {
+__device__ __forceinline__ double full_element(const Dev& d, int e, int4 n, double strainDt,
+                                       const nfm::Tables& T) {
     const Mat fl = load_mat(d, e);
     const double mid = -0.5 * strainDt;
     const double2 p0 = d.x0[n.x], p1 = d.x0[n.y], p2 = d.x0[n.z], p3 = d.x0[n.w];
@@ -658,7 +662,7 @@ __device__ double full_element(const Dev& d, int e, int4 n, double strainDt) {
         }
         nfd::DamagePoint dp;
         if (DAMAGE) load_damage(d, p, dp);
-        nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp);
+        nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp, T);
         store_point<KIN>(d, p, q);
         if (DAMAGE) {
             d.dStrength[p] = dp.strength;
@@ -735,6 +739,9 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
     Ctl* c = d.ctl;
     Step& st = c->s[q];
     if (mode == MODE_STEP && !st.go) return;
+    __shared__ double sLog[97][3];
+    __shared__ double sExp[64][2];
+    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
     const double strainDt = mode == MODE_INITIAL ? 0.0 : st.dt;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     double edge = NFD_MAX_VALUE;
@@ -751,7 +758,7 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
         if (gone) {
             write_zero_slots(d, e);
         } else {
-            edge = FULL ? full_element<DAMAGE, KIN>(d, e, n, strainDt) : reduced_element<DAMAGE, KIN>(d, e, n, strainDt);
+            edge = FULL ? full_element<DAMAGE, KIN>(d, e, n, strainDt, T) : reduced_element<DAMAGE, KIN>(d, e, n, strainDt, T);
             sawNaN = edge != edge;
         }
     } else if (e < d.E + d.edges) {
@@ -868,6 +875,9 @@ __global__ void k_advance_disp(Dev d, int q) {
 template <bool DAMAGE>
 __global__ void k_point_path(Mat m, int paths, int steps, const double* inc, const double* dmgIn,
                              double* out) {
+    __shared__ double sLog[97][3];
+    __shared__ double sExp[64][2];
+    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
     const int p = blockIdx.x * BLOCK + threadIdx.x;
     if (p >= paths) return;
     nfd::PointState q{};
@@ -881,7 +891,7 @@ __global__ void k_point_path(Mat m, int paths, int steps, const double* inc, con
     }
     for (int k = 0; k < steps; k++) {
         const double* x = inc + 5 * ((size_t)p * steps + k);
-        const double dg = nfd::j2_update<DAMAGE, true>(q, x[0], x[1], x[2], x[3], m, x[4], &dp);
+        const double dg = nfd::j2_update<DAMAGE, true>(q, x[0], x[1], x[2], x[3], m, x[4], &dp, T);
         double* o = out + 14 * ((size_t)p * steps + k);
         o[0] = q.s0; o[1] = q.s1; o[2] = q.s2; o[3] = q.s3;
         o[4] = q.b0; o[5] = q.b1; o[6] = q.b2; o[7] = q.b3;
@@ -1365,6 +1375,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
     std::vector<Mat> table(material_count);
     for (int m = 0; m < material_count; m++) {
         table[m] = make_mat(s->materials[m].raw, formulation);
+        if (m == 0) s->d.mat0 = table[0];
         if (table[m].hasJC) s->anyJC = true;
         if (table[m].hKin != 0.0) s->hasKin = true;
     }

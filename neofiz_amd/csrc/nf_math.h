@@ -88,7 +88,14 @@ NF_HD void two_prod(double a, double b, double& p, double& e) {
 // bits); log(1 + r) = r + r^2 (-1/2 + r/3 - ... - r^6/8) by Estrin's scheme, |r| <= 2^-7.5;
 // log x = k ln2 + log F + log(1 + r), summed as a double-double. The table supplies only
 // log F, which is not needed until the end, so its load is off the dependency chain.
-NF_HD void log_dd(double x, double& hi, double& lo) {
+// The tables are passed in so that a GPU kernel can read them from LDS (the same values,
+// copied there once per workgroup); the overloads below use the constant arrays directly.
+struct Tables {
+    const double (*log)[3];
+    const double (*exp)[2];
+};
+
+NF_HD void log_dd(double x, double& hi, double& lo, const double (*LT)[3]) {
     uint64_t b = bits_of(x);
     int k = 0;
     if (b < 0x0010000000000000ULL) {  // subnormal
@@ -125,24 +132,26 @@ NF_HD void log_dd(double x, double& hi, double& lo) {
     const double P = r2 * q;
     const double kd = (double)k;
     double s1, e1, s2, e2;
-    two_sum(kd * NFM_LN2_HI, NFM_LOG_TBL[j][1], s1, e1);
+    two_sum(kd * NFM_LN2_HI, LT[j][1], s1, e1);
     two_sum(s1, r, s2, e2);
-    const double l = (e1 + e2) + ((kd * NFM_LN2_LO + NFM_LOG_TBL[j][2]) + (r_lo + P));
+    const double l = (e1 + e2) + ((kd * NFM_LN2_LO + LT[j][2]) + (r_lo + P));
     fast_two_sum(s2, l, hi, lo);
 }
 
-NF_HD double log(double x) {
+NF_HD double log(double x, const Tables& T) {
     if (!(x > 0.0)) return x == 0.0 ? -inf() : qnan();
     if (x == inf()) return x;
     double hi, lo;
-    log_dd(x, hi, lo);
+    log_dd(x, hi, lo, T.log);
     return hi + lo;
 }
+
+NF_HD double log(double x) { return log(x, Tables{NFM_LOG_TBL, NFM_EXP_TBL}); }
 
 // exp(a + b) with |b| <= ulp(a). Table method: a + b = (64 k + j) ln2/64 + r, |r| <= ln2/128;
 // exp = 2^k 2^(j/64) (1 + r + r^2 (1/2 + r/6 + ... + r^5/5040)), the polynomial by Estrin's
 // scheme and the table value carried as hi + lo.
-NF_HD double exp_dd(double a, double b) {
+NF_HD double exp_dd(double a, double b, const double (*ET)[2]) {
     if (is_nan(a)) return a;
     if (a > 709.782712893384) return inf();
     if (a < -745.2) return 0.0;
@@ -161,7 +170,7 @@ NF_HD double exp_dd(double a, double b) {
     const double Q = (qa + qb * r2) + qc * (r2 * r2);
     const double E = rl + r2 * Q;  // exp(r + rl) = 1 + r + E
     const double p = r + E;
-    const double th = NFM_EXP_TBL[j][0], tl = NFM_EXP_TBL[j][1];
+    const double th = ET[j][0], tl = ET[j][1];
     const double y = th + (tl + (th * r + (th * E + tl * p)));
     // Scale by 2^k exactly where the result is normal.
     if (k > 1023) {
@@ -174,11 +183,11 @@ NF_HD double exp_dd(double a, double b) {
     return y * from_bits((uint64_t)(k + 1023) << 52);
 }
 
-NF_HD double exp(double x) { return exp_dd(x, 0.0); }
+NF_HD double exp(double x) { return exp_dd(x, 0.0, NFM_EXP_TBL); }
 
 // x^y for the cases the solver needs: x > 0 finite, y finite. Special values follow IEEE pow
 // for x = 0, x = 1, y = 0, y = 1 (exactly x), NaN; a negative base yields NaN.
-NF_HD double pow(double x, double y) {
+NF_HD double pow(double x, double y, const Tables& T) {
     if (y == 0.0) return 1.0;
     if (y == 1.0) return x;
     if (is_nan(x) || is_nan(y)) return qnan();
@@ -187,14 +196,16 @@ NF_HD double pow(double x, double y) {
     if (x < 0.0) return qnan();
     if (x == inf()) return y > 0.0 ? inf() : 0.0;
     double l_hi, l_lo;
-    log_dd(x, l_hi, l_lo);
+    log_dd(x, l_hi, l_lo, T.log);
     double p_hi, p_lo;
     two_prod(y, l_hi, p_hi, p_lo);
     p_lo = p_lo + y * l_lo;
     double t_hi, t_lo;
     fast_two_sum(p_hi, p_lo, t_hi, t_lo);
-    return exp_dd(t_hi, t_lo);
+    return exp_dd(t_hi, t_lo, T.exp);
 }
+
+NF_HD double pow(double x, double y) { return pow(x, y, Tables{NFM_LOG_TBL, NFM_EXP_TBL}); }
 
 #define NFM_PI 3.141592653589793
 #define NFM_PI_2_HI 0x1.921fb54442d18p0    // pi/2 rounded
