@@ -73,21 +73,21 @@ NF_HD void split(double a, double& hi, double& lo) {
     lo = a - hi;
 }
 
+// Exact product a b = p + e. The error of a rounded product is itself a double, and
+// fma(a, b, -p) computes it exactly: a fused multiply-add used deliberately, as an exact
+// operation (the compiler never contracts anything implicitly here), which gives the same
+// bits on every IEEE target -- the GPU's v_fma_f64 and the host's correctly rounded fma().
 NF_HD void two_prod(double a, double b, double& p, double& e) {
     p = a * b;
-    double ah, al, bh, bl;
-    split(a, ah, al);
-    split(b, bh, bl);
-    e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
+    e = __builtin_fma(a, b, -p);
 }
 
 // log(x) as an unevaluated sum hi + lo (|lo| <= ulp(hi)) for positive finite x.
 // Tang's table method: x = 2^k m, m in [3/4, 3/2); F = j/128 nearest to m (j = 96..192);
-// f = m - F exactly; r = f/F carried as r + r_lo (r by one correctly rounded division, r_lo
-// from the exact remainder f - r F, which needs no split of F: F has at most 8 significant
-// bits); log(1 + r) = r + r^2 (-1/2 + r/3 - ... - r^6/8) by Estrin's scheme, |r| <= 2^-7.5;
-// log x = k ln2 + log F + log(1 + r), summed as a double-double. The table supplies only
-// log F, which is not needed until the end, so its load is off the dependency chain.
+// f = m - F exactly; r = f/F carried as r + r_lo (r = f * (1/F), r_lo from the exact
+// remainder f - r F); log(1 + r) = r + r^2 (-1/2 + r/3 - ... - r^6/8) by Estrin's scheme,
+// |r| <= 2^-7.5; log x = k ln2 + log F + log(1 + r), summed as a double-double. No
+// division.
 // The tables are passed in so that a GPU kernel can read them from LDS (the same values,
 // copied there once per workgroup); the overloads below use the constant arrays directly.
 struct Tables {
@@ -114,14 +114,13 @@ NF_HD void log_dd(double x, double& hi, double& lo, const double (*LT)[3]) {
     const int j = (int)jd - 96;
     const double F = jd * 0x1p-7;
     const double f = m - F;  // exact (Sterbenz)
-    // r = f / F correctly rounded, and its remainder exactly: r F = p + e (Veltkamp split of
-    // r; F has at most 8 significant bits and needs none), so f - r F = (f - p) - e.
-    const double r = f / F;
-    const double p = r * F;
-    double rh, rl;
-    split(r, rh, rl);
-    const double e = (rh * F - p) + rl * F;
-    const double r_lo = ((f - p) - e) / F;
+    // r = f (1/F) and the remainder f - r F to full precision: r F = p + e exactly (e by an
+    // exact fma), f - p exactly (Sterbenz), so r_lo = ((f - p) - e) / F carries the rest.
+    const double invF = LT[j][0];
+    const double r = f * invF;
+    double p, e;
+    two_prod(r, F, p, e);
+    const double r_lo = ((f - p) - e) * invF;
     // log(1 + r) - r, Estrin
     const double r2 = r * r;
     const double r4 = r2 * r2;
