@@ -74,22 +74,25 @@ __device__ __forceinline__ double j2_solve(double norm, double stiffness, double
     double hi = norm / stiffness;
     double dGamma = hi;
     for (int iteration = 0; iteration < 25; iteration++) {
+        // The body is written with selects rather than branches (the values are the
+        // reference's): the strain term's pow and the rate term's log are independent, and
+        // without branches between them the compiler interleaves the two chains.
         const double dEps = NFD_SQRT_2_3 * dGamma;
         const double eps = plasticStrain + dEps;
         const double rate = dt > 0.0 ? dEps / dt : 0.0;
-        const double strain = jc_strainTerm(j, eps, T);
-        const double rateFactor = jc_rateTerm(j, rate, T);
+        const double pw = nfm::pow(eps, j.n, T);
+        const double strain = eps <= 0.0 ? j.a : j.a + j.b * pw;  // strainTerm
+        const bool clamped = rate <= j.eps0;
+        const double lg = nfm::log(j.eps0 == 1.0 ? rate : rate / j.eps0, T);
+        const double rateFactor = clamped ? 1.0 : 1.0 + j.c * lg;  // rateTerm
         const double g = norm - stiffness * dGamma - NFD_SQRT_2_3 * strain * rateFactor * thermal;
-        if (g > 0.0) lo = dGamma;
-        else hi = dGamma;
+        lo = g > 0.0 ? dGamma : lo;
+        hi = g > 0.0 ? hi : dGamma;
         if (jabs(g) <= tolerance) return dGamma;
-        double slope = -stiffness - NFD_TWO_THIRDS * (j.n * (strain - j.a) / eps) * rateFactor * thermal;
-        if (!(rate <= j.eps0)) {
-            slope -= NFD_SQRT_2_3 * strain * j.c * thermal / dGamma;
-        }
-        double next = dGamma - g / slope;
-        if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
-        dGamma = next;
+        const double slope0 = -stiffness - NFD_TWO_THIRDS * (j.n * (strain - j.a) / eps) * rateFactor * thermal;
+        const double slope = clamped ? slope0 : slope0 - NFD_SQRT_2_3 * strain * j.c * thermal / dGamma;
+        const double next = dGamma - g / slope;
+        dGamma = (next > lo && next < hi) ? next : 0.5 * (lo + hi);
     }
     return dGamma;
 }

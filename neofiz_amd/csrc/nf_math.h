@@ -96,19 +96,17 @@ struct Tables {
 };
 
 NF_HD void log_dd(double x, double& hi, double& lo, const double (*LT)[3]) {
-    uint64_t b = bits_of(x);
-    int k = 0;
-    if (b < 0x0010000000000000ULL) {  // subnormal
-        x = x * 0x1p54;
-        b = bits_of(x);
-        k = -54;
-    }
-    k += (int)(b >> 52) - 1023;
-    double m = from_bits((b & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);
-    if (m >= 1.5) {
-        m = m * 0.5;
-        k += 1;
-    }
+    // Branch-free: every range decision is a select, so the compiler can interleave this
+    // chain with independent work (the Newton loop evaluates a pow and a log side by side).
+    const uint64_t b0 = bits_of(x);
+    const bool sub = b0 < 0x0010000000000000ULL;  // subnormal (or zero)
+    const double xs = sub ? x * 0x1p54 : x;
+    const uint64_t b = bits_of(xs);
+    int k = (int)(b >> 52) - 1023 - (sub ? 54 : 0);
+    const double m1 = from_bits((b & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);
+    const bool high = m1 >= 1.5;
+    const double m = high ? m1 * 0.5 : m1;
+    k += high ? 1 : 0;
     const double shifter = 0x1.8p52;
     const double jd = (m * 128.0 + shifter) - shifter;  // nearest integer, 96..192
     const int j = (int)jd - 96;
@@ -138,22 +136,26 @@ NF_HD void log_dd(double x, double& hi, double& lo, const double (*LT)[3]) {
 }
 
 NF_HD double log(double x, const Tables& T) {
-    if (!(x > 0.0)) return x == 0.0 ? -inf() : qnan();
-    if (x == inf()) return x;
     double hi, lo;
-    log_dd(x, hi, lo, T.log);
-    return hi + lo;
+    // the main path on a harmless stand-in, then the special values by selection
+    const bool ok = x > 0.0 && x < inf();
+    log_dd(ok ? x : 1.0, hi, lo, T.log);
+    const double special = x == 0.0 ? -inf() : (x == inf() ? x : qnan());
+    return ok ? hi + lo : special;
 }
 
 NF_HD double log(double x) { return log(x, Tables{NFM_LOG_TBL, NFM_EXP_TBL}); }
 
 // exp(a + b) with |b| <= ulp(a). Table method: a + b = (64 k + j) ln2/64 + r, |r| <= ln2/128;
 // exp = 2^k 2^(j/64) (1 + r + r^2 (1/2 + r/6 + ... + r^5/5040)), the polynomial by Estrin's
-// scheme and the table value carried as hi + lo.
-NF_HD double exp_dd(double a, double b, const double (*ET)[2]) {
-    if (is_nan(a)) return a;
-    if (a > 709.782712893384) return inf();
-    if (a < -745.2) return 0.0;
+// scheme and the table value carried as hi + lo. Branch-free: the scaling by 2^k is two
+// exact multiplications by powers of two (one rounding at most, into the subnormal range),
+// and the out-of-range results are selected at the end.
+NF_HD double exp_dd(double a0, double b, const double (*ET)[2]) {
+    const bool nan = is_nan(a0);
+    const bool over = a0 > 709.782712893384;
+    const bool under = a0 < -745.2;
+    const double a = (nan || over || under) ? 0.0 : a0;
     const double shifter = 0x1.8p52;
     const double nd = (a * NFM_INV_L64 + shifter) - shifter;
     const int n = (int)nd;
@@ -171,37 +173,33 @@ NF_HD double exp_dd(double a, double b, const double (*ET)[2]) {
     const double p = r + E;
     const double th = ET[j][0], tl = ET[j][1];
     const double y = th + (tl + (th * r + (th * E + tl * p)));
-    // Scale by 2^k exactly where the result is normal.
-    if (k > 1023) {
-        return (y * from_bits((uint64_t)(1023 + 1023) << 52)) *
-               from_bits((uint64_t)(k - 1023 + 1023) << 52);
-    }
-    if (k < -1021) {
-        return (y * from_bits((uint64_t)(k + 54 + 1023) << 52)) * 0x1p-54;
-    }
-    return y * from_bits((uint64_t)(k + 1023) << 52);
+    const int k1 = k / 2;  // |k1|, |k - k1| <= 538: both factors are normal doubles
+    const int k2 = k - k1;
+    const double v = (y * from_bits((uint64_t)(k1 + 1023) << 52)) * from_bits((uint64_t)(k2 + 1023) << 52);
+    return nan ? a0 : (over ? inf() : (under ? 0.0 : v));
 }
 
 NF_HD double exp(double x) { return exp_dd(x, 0.0, NFM_EXP_TBL); }
 
 // x^y for the cases the solver needs: x > 0 finite, y finite. Special values follow IEEE pow
-// for x = 0, x = 1, y = 0, y = 1 (exactly x), NaN; a negative base yields NaN.
+// for x = 0, x = 1, y = 0, y = 1 (exactly x), NaN; a negative base yields NaN. Branch-free.
 NF_HD double pow(double x, double y, const Tables& T) {
-    if (y == 0.0) return 1.0;
-    if (y == 1.0) return x;
-    if (is_nan(x) || is_nan(y)) return qnan();
-    if (x == 1.0) return 1.0;
-    if (x == 0.0) return y > 0.0 ? 0.0 : inf();
-    if (x < 0.0) return qnan();
-    if (x == inf()) return y > 0.0 ? inf() : 0.0;
+    const bool regular = x > 0.0 && x < inf() && !is_nan(y);
     double l_hi, l_lo;
-    log_dd(x, l_hi, l_lo, T.log);
+    log_dd(regular ? x : 1.0, l_hi, l_lo, T.log);
     double p_hi, p_lo;
     two_prod(y, l_hi, p_hi, p_lo);
     p_lo = p_lo + y * l_lo;
     double t_hi, t_lo;
     fast_two_sum(p_hi, p_lo, t_hi, t_lo);
-    return exp_dd(t_hi, t_lo, T.exp);
+    const double v = exp_dd(t_hi, t_lo, T.exp);
+    double special;
+    if (is_nan(x) || is_nan(y)) special = qnan();
+    else if (x == 0.0) special = y > 0.0 ? 0.0 : inf();
+    else if (x < 0.0) special = qnan();
+    else special = y > 0.0 ? inf() : 0.0;  // x = +inf
+    const double out = regular ? v : special;
+    return y == 0.0 ? 1.0 : ((y == 1.0 || x == 1.0) ? (x == 1.0 ? 1.0 : x) : out);
 }
 
 NF_HD double pow(double x, double y) { return pow(x, y, Tables{NFM_LOG_TBL, NFM_EXP_TBL}); }
