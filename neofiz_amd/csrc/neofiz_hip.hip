@@ -27,6 +27,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -103,6 +104,7 @@ struct Dev {
     double* hgR; double* hgZ; double* hgK;
     double2* slot;
     const int* slotStart;
+    const int4* slot4;  // per node: its first four slots in gather order (-1 unused)
     const int* slotOf;
     const int2* pedges;
     // damage (null when off)
@@ -215,10 +217,23 @@ __device__ __forceinline__ void nodal_node(const Dev& d, int i, double2 fi, doub
 
 // Gather of one node, solver/ExplicitSolver.java:1147-1165: the node's force slots in the
 // reference's order (elements ascending, then pressure edges), plus gravity.
+// A node's first four slots come from one 16-byte load, so their four loads issue together
+// instead of each waiting on an index load; a node with more sources (rare on a quad mesh)
+// continues through the CSR. The sum runs in the same order either way.
 __device__ __forceinline__ double2 gather_node(const Dev& d, int n) {
+    const int4 k4 = d.slot4[n];
+    const double2 z = make_double2(0.0, 0.0);
+    const double2 s0 = k4.x >= 0 ? d.slot[k4.x] : z;
+    const double2 s1 = k4.y >= 0 ? d.slot[k4.y] : z;
+    const double2 s2 = k4.z >= 0 ? d.slot[k4.z] : z;
+    const double2 s3 = k4.w >= 0 ? d.slot[k4.w] : z;
     double sr = 0.0, sz = 0.0;
-    const int end = d.slotStart[n + 1];
-    for (int k = d.slotStart[n]; k < end; k++) {
+    if (k4.x >= 0) { sr += s0.x; sz += s0.y; }
+    if (k4.y >= 0) { sr += s1.x; sz += s1.y; }
+    if (k4.z >= 0) { sr += s2.x; sz += s2.y; }
+    if (k4.w >= 0) { sr += s3.x; sz += s3.y; }
+    const int start = d.slotStart[n] + 4, end = d.slotStart[n + 1];
+    for (int k = start; k < end; k++) {
         const double2 s = d.slot[d.slotOf[k]];
         sr += s.x;
         sz += s.y;
@@ -1350,6 +1365,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         return rc;
     };
     int rc;
+    if (const char* g = std::getenv("NEOFIZ_GRAPHS")) s->useGraphs = g[0] != '0';
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(NF_E_DEVICE, "hipStreamCreate failed: no usable GPU"));
     const int N = node_count, E = element_count;
@@ -1429,6 +1445,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
     int4* conn4 = nullptr;
     Mat* mats = nullptr;
     int* slotStart = nullptr;
+    int4* slot4 = nullptr;
     int* slotOf = nullptr;
     if ((rc = s->alloc(&s->ctl, 1)) || (rc = s->alloc(&x0, N)) || (rc = s->alloc(&z0, N)) || (rc = s->alloc(&d.u, N)) ||
         (rc = s->alloc(&d.v, N)) || (rc = s->alloc(&d.f, N)) || (rc = s->alloc(&mass, N)) ||
@@ -1441,6 +1458,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         (rc = s->alloc(&d.hgR, E)) || (rc = s->alloc(&d.hgZ, E)) || (rc = s->alloc(&d.hgK, E)) ||
         (rc = s->alloc(&d.slot, 4 * ((size_t)E + pressure_edge_count))) ||
         (rc = s->alloc(&slotStart, N + 1)) || (rc = s->alloc(&slotOf, s->slotOf.size())) ||
+        (rc = s->alloc(&slot4, N)) ||
         (rc = s->alloc(&d.wallPartial, 3 * (size_t)blocks_for(N))) ||
         (rc = s->alloc(&d.wallCountPartial, 2 * (size_t)blocks_for(N))) ||
         (rc = s->alloc(&d.erodePartial, (size_t)blocks_for(E + pressure_edge_count))) ||
@@ -1454,6 +1472,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
     d.mats = mats;
     d.slotStart = slotStart;
     d.slotOf = slotOf;
+    d.slot4 = slot4;
     if (pressure_edge_count) {
         int2* pe = nullptr;
         if ((rc = s->alloc(&pe, pressure_edge_count))) return bail(rc);
@@ -1468,6 +1487,12 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         if (hipMemcpy(mi, material_of, sizeof(int) * E, hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(NF_E_DEVICE, "upload failed"));
     }
+    std::vector<int4> s4(N);
+    for (int n = 0; n < N; n++) {
+        int k[4] = {-1, -1, -1, -1};
+        for (int c = 0; c < 4 && s->slotStart[n] + c < s->slotStart[n + 1]; c++) k[c] = s->slotOf[s->slotStart[n] + c];
+        s4[n] = make_int4(k[0], k[1], k[2], k[3]);
+    }
     std::vector<double2> x0h(N);
     for (int i = 0; i < N; i++) x0h[i] = make_double2(r[i], z[i]);
     std::vector<double> dens(E);
@@ -1477,6 +1502,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         hipMemcpy(conn4, conn, sizeof(int4) * E, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(mats, table.data(), sizeof(Mat) * material_count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(slotStart, s->slotStart.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(slot4, s4.data(), sizeof(int4) * N, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(slotOf, s->slotOf.data(), sizeof(int) * s->slotOf.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->densityOfElement, dens.data(), sizeof(double) * E, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(NF_E_DEVICE, "upload failed"));
