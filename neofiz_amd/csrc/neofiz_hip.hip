@@ -1199,7 +1199,7 @@ struct nf_solver {
     Step* statusHost = nullptr;  // pinned
     hipEvent_t statusEvent[2] = {nullptr, nullptr};
     bool graphDirty = true;
-    bool useGraphs = true;
+    bool useGraphs = false;  // NEOFIZ_GRAPHS=1 captures blocks of steps in hipGraphs instead
 
     void dropGraphs() {
         for (int k = 0; k < 2; k++)
@@ -1365,7 +1365,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         return rc;
     };
     int rc;
-    if (const char* g = std::getenv("NEOFIZ_GRAPHS")) s->useGraphs = g[0] != '0';
+    if (const char* g = std::getenv("NEOFIZ_GRAPHS")) s->useGraphs = g[0] == '1';
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(NF_E_DEVICE, "hipStreamCreate failed: no usable GPU"));
     const int N = node_count, E = element_count;
