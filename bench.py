@@ -14,9 +14,9 @@ Inputs are resident in HBM before the timed region starts.
 
 Extra objects on the line:
   roofline      the element kernel's algorithmic bytes per launch over its mean launch
-                duration (HIP events on the library's stream, a separate profiled pass of
-                the same steps), against the 8 TB/s HBM peak; traffic from the committed
-                rocprofv3 PMC summary when present.
+                duration over the timed steps (start/stop events recorded by each dispatch
+                on the library's stream), against the 8 TB/s HBM peak; traffic (HBM bytes
+                per launch) from the committed rocprofv3 PMC summary.
   cpu_baseline  the oracle (the C++ restatement, threaded like the reference's Parallel
                 pool) on the host's cores over a bounded sample of the same workload.
 """
@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "element_kernel_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "element_kernel_traffic.txt")
 
 
 def element_kernel_bytes(E, N, P):
@@ -89,7 +89,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--nr", type=int, default=128)
     ap.add_argument("--nz", type=int, default=1280)
-    ap.add_argument("--profile-steps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
@@ -111,22 +110,21 @@ def main():
     E, N = mesh.elementCount, mesh.nodeCount
     solver.run(args.warmup)  # synchronous: returns with the device idle
     print(f"# warmup {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    solver.run(args.steps)   # synchronous
-    elapsed = time.perf_counter() - t0
-    st = solver.state()
-    assert st.steps == args.warmup + args.steps, "step count mismatch"
-
-    # Profiled pass over the next steps: per-kernel mean durations from HIP events.
+    # Kernel durations over exactly the timed steps: each launch carries start/stop events
+    # recorded by the dispatch itself (hipExtLaunchKernelGGL), so no packet is added.
     import ctypes as C
     b = solver._b
     ms = (C.c_double * 3)()
     counts = (C.c_int64 * 3)()
     b.profile(solver._h, 1)
     b.profile_read(solver._h, ms, counts)
-    solver.run(args.profile_steps)
+    t0 = time.perf_counter()
+    solver.run(args.steps)   # synchronous
+    elapsed = time.perf_counter() - t0
     b.profile_read(solver._h, ms, counts)
     b.profile(solver._h, 0)
+    st = solver.state()
+    assert st.steps == args.warmup + args.steps, "step count mismatch"
     kernel_ms = [ms[k] / max(1, counts[k]) for k in range(3)]
     solver.shutdown()
 
@@ -134,10 +132,14 @@ def main():
     achieved = elem_bytes / (kernel_ms[1] * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
+        t = {}
         with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
+            for line in f:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    t[k.strip()] = float(v)
         if t.get("nr") == args.nr and t.get("nz") == args.nz:
-            traffic = t.get("bytes_per_launch")
+            traffic = int(t["bytes_per_launch"])
 
     value = E * args.steps / elapsed
     line = {
