@@ -85,6 +85,7 @@ _PROTOTYPES = {
     "get_state": (C.c_int, [_P, C.POINTER(StateC)]),
     "get_audits": (C.c_int, [_P, C.POINTER(AuditsC)]),
     "contact_pairs": (C.c_int, [_P, C.c_int32, c_int32_p, c_int32_p]),
+    "contact_surface": (C.c_int, [_P, C.c_int32, c_int32_p, c_int32_p]),
     "math_eval": (C.c_int, [C.c_int32, c_double_p, c_double_p, c_double_p, C.c_int32]),
     "profile": (C.c_int, [_P, C.c_int32]),
     "point_path": (C.c_int, [C.POINTER(MaterialC), C.c_int32, C.c_int32, C.c_int32, c_double_p,

@@ -176,6 +176,13 @@ class QuadMesh:
             raise IllegalArgumentException("pressure edges must be a multiple of 2")
         self.nodeCount = self.r.size
         self.elementCount = self.conn.size // 4
+        if self.elementCount and self.conn.min() < 0 or self.conn.size and self.conn.max() >= self.nodeCount:
+            raise IllegalArgumentException("connectivity refers to a node that does not exist")
+        bad = np.nonzero(self.signedAreas() <= 0.0)[0]
+        if bad.size:  # checkOrientation :228-236
+            raise IllegalStateException(
+                f"element {int(bad[0])} has non-positive area; node ordering must be "
+                "counter-clockwise in (r, z)")
 
     @staticmethod
     def _lattice(r_of, z_of, nr, nz):
@@ -233,6 +240,60 @@ class QuadMesh:
     def farFaceNodes(nr, nz):
         return [nz * (nr + 1) + i for i in range(nr + 1)]
 
+    def signedAreas(self):
+        """signedArea of every element, with the reference's summation order."""
+        c = self.conn.reshape(-1, 4)
+        a = np.zeros(c.shape[0])
+        for k in range(4):
+            i, j = c[:, k], c[:, (k + 1) % 4]
+            a = a + (self.r[i] * self.z[j] - self.r[j] * self.z[i])
+        return 0.5 * a
+
+    def minimumEdgeLength(self):  # :61-75
+        c = self.conn.reshape(-1, 4)
+        best = 1.7976931348623157e308
+        for k in range(4):
+            i, j = c[:, k], c[:, (k + 1) % 4]
+            dr, dz = self.r[j] - self.r[i], self.z[j] - self.z[i]
+            ln = np.sqrt(dr * dr + dz * dz)
+            if ln.size:
+                best = min(best, float(ln.min()))
+        return best
+
+    def _centroid(self, e, coord):  # :106-118
+        b = 4 * e
+        a = moment = 0.0
+        for k in range(4):
+            i, j = self.conn[b + k], self.conn[b + (k + 1) % 4]
+            cross = self.r[i] * self.z[j] - self.r[j] * self.z[i]
+            a += cross
+            moment += (coord[i] + coord[j]) * cross
+        return float(moment / (3.0 * a))
+
+    def centroidRadius(self, e):  # :96-98
+        return self._centroid(e, self.r)
+
+    def centroidZ(self, e):  # :101-103
+        return self._centroid(e, self.z)
+
+    def surface(self, skip=None):
+        """The free surface by connectivity, :188-226: an edge used by exactly one live
+        element is free; free edges are listed element by element, local edge by local
+        edge, and `nodes` holds every node on them in ascending order."""
+        c = self.conn.reshape(-1, 4)
+        a = c.reshape(-1)
+        b = c[:, [1, 2, 3, 0]].reshape(-1)
+        live = np.ones(self.elementCount, dtype=bool) if skip is None else ~np.asarray(skip, dtype=bool)
+        live4 = np.repeat(live, 4)
+        key = (np.minimum(a, b).astype(np.int64) << 32) | np.maximum(a, b).astype(np.int64)
+        uniq, inv, counts = np.unique(key[live4], return_inverse=True, return_counts=True)
+        free_live = counts[inv] == 1
+        idx = np.nonzero(live4)[0][free_live]
+        edges = np.stack([a[idx], b[idx]], axis=1).reshape(-1).astype(np.int32)
+        owner = (idx // 4).astype(np.int32)
+        nodes = np.unique(edges).astype(np.int32)
+        return Surface(edges, owner, nodes)
+
     def signedArea(self, e):
         c = self.conn[4 * e:4 * e + 4]
         a = 0.0
@@ -240,6 +301,18 @@ class QuadMesh:
             i, j = c[k], c[(k + 1) % 4]
             a += self.r[i] * self.z[j] - self.r[j] * self.z[i]
         return 0.5 * a
+
+
+@dataclass(frozen=True)
+class Surface:
+    """QuadMesh.Surface (mesh/QuadMesh.java:152-157): edges (a, b) pairs, owning element per
+    edge, and the surface nodes ascending."""
+    edges: np.ndarray
+    owner: np.ndarray
+    nodes: np.ndarray
+
+    def edgeCount(self):
+        return int(self.owner.size)
 
 
 # ---------------------------------------------------------------- boundary objects
@@ -264,6 +337,39 @@ class Contact:
         self.friction = 0.0
         self.damping = 0.0
         self.scale = 0.1
+        self._solver = None
+        self._segments = mesh.surface().edgeCount() if mesh is not None else None
+
+    # Audits (:371-404). The state lives in the solver this contact was handed to.
+    def _state(self):
+        if self._solver is None:
+            return None
+        return self._solver.state()
+
+    def pairCount(self):
+        st = self._state()
+        return 0 if st is None else st.contact_pairs
+
+    def energy(self):
+        st = self._state()
+        return 0.0 if st is None else st.contact_energy
+
+    def dissipation(self):
+        st = self._state()
+        return 0.0 if st is None else st.contact_dissipation
+
+    def maxPenetration(self):
+        st = self._state()
+        return 0.0 if st is None else st.contact_max_penetration
+
+    def escapedNodes(self):
+        st = self._state()
+        return 0 if st is None else int(st.contact_escaped_nodes)
+
+    def segmentCount(self):
+        if self._solver is None:
+            return self._segments
+        return self._solver.contactSurface().shape[0]
 
     def withFriction(self, mu):
         if mu < 0.0:
@@ -380,7 +486,12 @@ class ExplicitSolver:
         else:
             self._call("set_contact", 1, float(contact.friction), float(contact.damping),
                        float(contact.scale))
+            contact._solver = self
         self._contact = contact
+
+    def contact(self):
+        """The contact model, or None if none was set (:492-494)."""
+        return self._contact
 
     def setGravity(self, ar, az):
         self._call("set_gravity", float(ar), float(az))
@@ -557,6 +668,15 @@ class ExplicitSolver:
 
     def wallContactCount(self):
         return self.state().wall_contact_count
+
+    def contactSurface(self):
+        """The contact's current surface segments (a, b), in surface order (element by
+        element, mesh/QuadMesh.java:188-226, over the elements contact has not forgotten)."""
+        n = C.c_int32()
+        self._call("contact_surface", 0, None, C.byref(n))
+        out = np.empty(max(1, 2 * n.value), np.int32)
+        self._call("contact_surface", n.value, _iptr(out), C.byref(n))
+        return out[:2 * n.value].reshape(-1, 2)
 
     def contactPairs(self):
         """(node, segment end a, segment end b) for every pair of the last step, node order."""

@@ -1917,6 +1917,25 @@ int nfo_contact_pairs(nfo_solver* s, int32_t capacity, int32_t* out, int32_t* co
     return NF_OK;
 }
 
+// The contact surface's segments (a, b) in surface order.
+int nfo_contact_surface(nfo_solver* s, int32_t capacity, int32_t* out, int32_t* count) {
+    if (!s->hasContact) {
+        *count = 0;
+        return NF_OK;
+    }
+    // The surface the next step will use: rebuilt from the contact's dead set, as adopt()
+    // would, without touching the contact's state.
+    std::vector<int> seg, owner, nodes;
+    s->mesh.surface(s->contact.dead, seg, owner, nodes);
+    const int S = (int)owner.size();
+    for (int k = 0; k < S && k < capacity; k++) {
+        out[2 * k] = seg[2 * k];
+        out[2 * k + 1] = seg[2 * k + 1];
+    }
+    *count = S;
+    return NF_OK;
+}
+
 }  // extern "C"
 
 // The shared transcendental functions evaluated on the host, for the math self-checks.
