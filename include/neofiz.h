@@ -184,10 +184,20 @@ typedef struct nf_audits {
 } nf_audits;
 int nf_get_audits(nf_solver* s, nf_audits* out);
 
+/* Contact queries (solver/Contact.java:371-404 and the pair set the search chose, :446-462).
+ * nf_contact_pairs: for every surface node, in ascending node order, that resolved a pair
+ * in the last assembly, {node, segment end a, segment end b}; nf_contact_surface: the
+ * surface segments {a, b} in surface order (element by element, mesh/QuadMesh.java:188-226,
+ * over the elements the contact has not forgotten). count receives the total; at most
+ * capacity entries are written. Both report nothing when contact is off. */
+int nf_contact_pairs(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count);
+int nf_contact_surface(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count);
+
 /* Per-kernel timing of the step loop (no counterpart in the reference, which times whole
  * runs with System.nanoTime, validate/TaylorImpactCase.java:254-260). When enabled, each
  * launch is bracketed by HIP events on the handle's stream; nf_profile_read returns summed
- * milliseconds and launch counts for {nodal, element, gather} and resets them. */
+ * milliseconds and launch counts for {node pass, element pass, contact launches} and resets
+ * them. */
 int nf_profile(nf_solver* s, int32_t enable);
 int nf_profile_read(nf_solver* s, double* ms, int64_t* counts);
 

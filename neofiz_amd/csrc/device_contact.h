@@ -1,0 +1,559 @@
+// Penalty contact between deformable surfaces on the GPU: solver/Contact.java:132-763.
+//
+// Included by neofiz_hip.hip after the Dev and Ctl definitions.
+//
+// Each assembly that gathers forces runs five launches after the gather and before the
+// nodal update (Contact.begin / search / accumulate, :424-462):
+//
+//   k_c_rebuild   only when an element was forgotten since the last rebuild (adopt,
+//                 :290-322): live flags of every potential segment and surface node, compact
+//                 lists of both, lastMaster cleared.
+//   k_c_geometry  segment lengths and outward normals (:465-487), the grid's bounding box
+//                 and longest segment (exact min/max on order-preserving bit keys), and the
+//                 clearing of the hash heads and per-segment pair lists.
+//   k_c_grid      node normals in the reference's order (:488-499) and every segment end
+//                 filed under its grid cell (:515-551).
+//   k_c_search    per surface node, the deepest penetrated segment (:570-658) and the
+//                 accumulate bookkeeping (:446-462); the pair's forces are evaluated
+//                 (resolve, :660-762) and the slave is linked to its master segment.
+//   k_c_apply     per node, the pair forces that touch it added to the gathered force in
+//                 the reference's order: by slave node, then normal before tangent.
+//
+// Identity instead of order. The reference walks a surface list built in element order and
+// a bucket grid built in segment order, and several results depend on that order: the first
+// of two equally deep candidates wins, and each node's force takes its contact terms in
+// slave order. Here every segment is named by its potential index p = 4 e + k, which sorts
+// exactly as the reference's surface index does, and every surface node by its node index,
+// which sorts as the reference's surface-node index does. The grid is a hash of the
+// reference's own cells (same origin, cell size and column count), whose buckets may hold
+// entries of other cells; only entries of the nine reference cells are considered, and the
+// first-visit position of a candidate is reconstructed as (cell rank in the 3 x 3 walk,
+// entry index 2p + end). The winner is then the deepest candidate with the smallest such
+// position, which is the candidate the reference's sequential scan keeps. So every pair,
+// every force and every escape count is the reference's, bit for bit, while the lists the
+// kernels walk are built with atomics in any order.
+//
+// The per-step audits (pairs, stored energy, maximum penetration) and the running friction
+// and damper dissipation are reduced per launch in a fixed tree (node-index blocks), not in
+// the reference's serial pair order, so the two energies agree with the reference to
+// rounding rather than bitwise; the counts and the maximum are exact.
+#pragma once
+
+constexpr double C_MAX_DEPTH = 0.5;         // Contact.java:153
+constexpr double C_STICK_RELAXATION = 0.5;  // :171
+constexpr double C_FACING = 0.25;           // :195
+constexpr int CPAIR = 10;                   // doubles per resolved pair record
+constexpr int C_MAX_TERMS = 32;             // pair terms one node may receive in one step
+constexpr int CONTACT_GRID = 1024;          // blocks of the grid-stride contact launches
+
+// A pair record: what resolve() computed, read back by the nodes it pushes on.
+enum { PR_NORMAL = 0, PR_TANGENT, PR_WA, PR_WB, PR_NR, PR_NZ, PR_ENERGY, PR_DAMPER, PR_FRICTION, PR_DEPTH };
+
+__device__ __forceinline__ unsigned long long order_key(double x) {
+    const unsigned long long b = (unsigned long long)nfm::bits_of(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+__device__ __forceinline__ double from_order_key(unsigned long long k) {
+    return nfm::from_bits((long long)((k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFULL) : ~k));
+}
+
+// Java's (int) cast of a double: truncation toward zero, saturating, NaN to 0.
+__device__ __forceinline__ int jint(double x) {
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return 2147483647;
+    if (x <= -2147483648.0) return (-2147483647 - 1);
+    return (int)x;
+}
+__device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : v >= n ? n - 1 : v; }
+
+__device__ __forceinline__ void seg_ends(const Dev& d, int p, int& a, int& b) {
+    const int4 c = d.conn[p >> 2];
+    const int k = p & 3;
+    a = k == 0 ? c.x : k == 1 ? c.y : k == 2 ? c.z : c.w;
+    b = k == 0 ? c.y : k == 1 ? c.z : k == 2 ? c.w : c.x;
+}
+
+// Current position, cr = r + ur, cz = z + uz (:467-470).
+__device__ __forceinline__ double2 cpos(const Dev& d, int n) {
+    const double2 x = d.x0[n], u = d.u[n];
+    return make_double2(x.x + u.x, x.y + u.y);
+}
+
+// p is on the surface of the elements the contact still knows: its element is live and no
+// other live element uses the same edge (QuadMesh.surface, mesh/QuadMesh.java:188-226).
+__device__ __forceinline__ bool seg_live(const Dev& d, int p) {
+    if (d.cDead[p >> 2]) return false;
+    for (int k = d.cShareStart[p]; k < d.cShareStart[p + 1]; k++)
+        if (!d.cDead[d.cShareOf[k] >> 2]) return false;
+    return true;
+}
+
+struct CGrid {
+    double r0, z0, cell;
+    int cols, rows;
+    unsigned mask;  // hash table size - 1
+};
+
+__device__ __forceinline__ unsigned c_hash_size(int nAct) {
+    unsigned h = 64;
+    while (h < 4u * (unsigned)nAct) h <<= 1;
+    return h;
+}
+
+// Contact.grid :515-533, from the reduced bounding box and longest segment.
+__device__ __forceinline__ CGrid c_grid(const Dev& d) {
+    const Ctl* c = d.ctl;
+    CGrid g;
+    const double rlo = from_order_key(c->cBox[0]), zlo = from_order_key(c->cBox[1]);
+    const double rhi = -from_order_key(c->cBox[2]), zhi = -from_order_key(c->cBox[3]);
+    const double longest = nfm::from_bits((long long)c->cLongest);
+    g.cell = longest > 0.0 ? 1.5 * longest : 1.0;
+    g.r0 = rlo;
+    g.z0 = zlo;
+    const int ci = (int)((unsigned)jint((rhi - rlo) / g.cell) + 1u);
+    const int cj = (int)((unsigned)jint((zhi - zlo) / g.cell) + 1u);
+    g.cols = ci > 1 ? ci : 1;
+    g.rows = cj > 1 ? cj : 1;
+    g.mask = c_hash_size(c->cNAct) - 1u;
+    return g;
+}
+
+__device__ __forceinline__ unsigned c_bucket(int i, int j, unsigned mask) {
+    unsigned h = (unsigned)i * 0x9E3779B1u ^ (unsigned)j * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return h & mask;
+}
+
+__device__ __forceinline__ bool c_skip(const Dev& d, int q, int checkGo) {
+    return checkGo && !d.ctl->s[q].go;
+}
+
+// ------------------------------------------------------------------ launches
+
+// adopt() :290-322 when the contact has forgotten elements since the last rebuild. The
+// counters were zeroed by whoever raised the stale flag. Every assembly: the grid's
+// reductions are reset here, ahead of k_c_geometry.
+__global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) {
+    if (c_skip(d, q, checkGo)) return;
+    Ctl* c = d.ctl;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned long long hi = order_key(NFD_MAX_VALUE);
+        c->cBox[0] = hi;
+        c->cBox[1] = hi;
+        c->cBox[2] = hi;  // keys of -rhi and -zhi: both start at -(-MAX_VALUE)
+        c->cBox[3] = hi;
+        c->cLongest = 0ULL;
+    }
+    if (!c->surfaceStale) return;
+    const int stride = gridDim.x * BLOCK;
+    for (int p = blockIdx.x * BLOCK + threadIdx.x; p < 4 * d.E; p += stride) {
+        const bool live = seg_live(d, p);
+        d.cActive[p] = live ? 1 : 0;
+        if (live) {
+            const int at = atomicAdd(&c->cNAct, 1);
+            if (at < 4 * d.E) d.cAct[at] = p;
+        }
+    }
+    for (int n = blockIdx.x * BLOCK + threadIdx.x; n < d.N; n += stride) {
+        bool on = false;
+        for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1] && !on; k++) on = seg_live(d, d.cNodeSegOf[k]);
+        d.cSurfFlag[n] = on ? 1 : 0;
+        d.cPairOk[n] = 0;
+        d.cLast[n] = -1;
+        if (on) {
+            const int at = atomicAdd(&c->cNSurf, 1);
+            if (at < d.N) d.cSurf[at] = n;
+        }
+    }
+}
+
+// geometry() :465-487 per live segment; the grid's box and longest segment; clears.
+__global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo) {
+    if (c_skip(d, q, checkGo)) return;
+    Ctl* c = d.ctl;
+    const int nAct = c->cNAct;
+    const int stride = gridDim.x * BLOCK;
+    const int t0 = blockIdx.x * BLOCK + threadIdx.x;
+    if (t0 == 0) c->surfaceStale = 0;
+    unsigned long long box[4] = {~0ULL, ~0ULL, ~0ULL, ~0ULL};
+    unsigned long long longest = 0ULL;
+    for (int t = t0; t < nAct; t += stride) {
+        const int p = d.cAct[t];
+        int a, b;
+        seg_ends(d, p, a, b);
+        const double2 xa = cpos(d, a), xb = cpos(d, b);
+        const double dr = xb.x - xa.x, dz = xb.y - xa.y;
+        const double len = __builtin_sqrt(dr * dr + dz * dz);
+        d.cLen[p] = len;
+        d.cNrm[p] = len > 0.0 ? make_double2(dz / len, -dr / len) : make_double2(0.0, 0.0);
+        d.cPairHead[p] = -1;
+        // Both ends of every live segment are exactly the surface nodes. NaN never wins a
+        // comparison in the reference's loops, so it is left out here.
+        const double xs[4] = {xa.x, xa.y, xb.x, xb.y};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double v = xs[k];
+            if (v != v) continue;
+            const unsigned long long lo = order_key(v), hi = order_key(-v);
+            const int axis = k & 1;
+            box[axis] = lo < box[axis] ? lo : box[axis];
+            box[2 + axis] = hi < box[2 + axis] ? hi : box[2 + axis];
+        }
+        if (len == len) {
+            const unsigned long long lb = (unsigned long long)nfm::bits_of(len);
+            longest = lb > longest ? lb : longest;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        unsigned long long v = box[k];
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(v, off, 64);
+            v = o < v ? o : v;
+        }
+        if ((threadIdx.x & 63) == 0 && v != ~0ULL) atomicMin(&c->cBox[k], v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(longest, off, 64);
+        longest = o > longest ? o : longest;
+    }
+    if ((threadIdx.x & 63) == 0 && longest) atomicMax(&c->cLongest, longest);
+    const unsigned H = c_hash_size(nAct);
+    for (unsigned h = t0; h < H; h += stride) d.cHead[h] = -1;
+}
+
+// Node normals :488-499 (the node's live segments in surface order) and the grid :534-551.
+__global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
+    if (c_skip(d, q, checkGo)) return;
+    const Ctl* c = d.ctl;
+    const int nAct = c->cNAct, nSurf = c->cNSurf;
+    const int stride = gridDim.x * BLOCK;
+    const int t0 = blockIdx.x * BLOCK + threadIdx.x;
+    for (int t = t0; t < nSurf; t += stride) {
+        const int n = d.cSurf[t];
+        double sr = 0.0, sz = 0.0;
+        for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1]; k++) {
+            const int p = d.cNodeSegOf[k];
+            if (!d.cActive[p]) continue;
+            const double2 nv = d.cNrm[p];
+            sr += nv.x;
+            sz += nv.y;
+        }
+        const double len = __builtin_sqrt(sr * sr + sz * sz);
+        d.cPn[n] = len > 0.0 ? make_double2(sr / len, sz / len) : make_double2(0.0, 0.0);
+    }
+    if (nAct == 0) return;
+    const CGrid g = c_grid(d);
+    for (int t = t0; t < 2 * nAct; t += stride) {
+        const int p = d.cAct[t >> 1];
+        int a, b;
+        seg_ends(d, p, a, b);
+        const double2 x = cpos(d, (t & 1) ? b : a);
+        const int i = clampi(jint((x.x - g.r0) / g.cell), g.cols);
+        const int j = clampi(jint((x.y - g.z0) / g.cell), g.rows);
+        const int entry = 2 * p + (t & 1);
+        d.cEntCell[entry] = make_int2(i, j);
+        d.cNext[entry] = atomicExch(&d.cHead[c_bucket(i, j, g.mask)], entry);
+    }
+}
+
+// excluded() :617-636
+__device__ __forceinline__ bool c_excluded(const Dev& d, int n, double2 pn, int p, int a, int b) {
+    const double2 ns = d.cNrm[p];
+    if (pn.x * ns.x + pn.y * ns.y > -C_FACING) return true;
+    if (a == n || b == n) return true;
+    for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1]; k++) {
+        const int own = d.cNodeSegOf[k];
+        if (!d.cActive[own]) continue;
+        int pa, pb;
+        seg_ends(d, own, pa, pb);
+        if (pa == a || pa == b || pb == a || pb == b) return true;
+    }
+    return false;
+}
+
+// depth() :639-649
+__device__ __forceinline__ double c_depth(const Dev& d, double2 xn, int p, int a, int b) {
+    const double len = d.cLen[p];
+    if (len <= 0.0) return 0.0;
+    const double2 xa = cpos(d, a), xb = cpos(d, b);
+    const double tr = (xb.x - xa.x) / len, tz = (xb.y - xa.y) / len;
+    const double dr = xn.x - xa.x, dz = xn.y - xa.y;
+    const double along = dr * tr + dz * tz;
+    if (along < 0.0 || along > len) return 0.0;
+    const double2 ns = d.cNrm[p];
+    const double gap = dr * ns.x + dz * ns.y;
+    return gap < 0.0 ? -gap : 0.0;
+}
+
+// nearestPenetrated :570-606 and accumulate :446-462 for one surface node, with resolve
+// :660-762 evaluated into the node's pair record.
+__global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
+    if (c_skip(d, q, checkGo)) return;
+    Ctl* c = d.ctl;
+    const int nAct = c->cNAct, nSurf = c->cNSurf;
+    if (nAct == 0) return;
+    const CGrid g = c_grid(d);
+    const double dt = c->s[q].dt;
+    const int stride = gridDim.x * BLOCK;
+    long long escaped = 0;
+    for (int t = blockIdx.x * BLOCK + threadIdx.x; t < nSurf; t += stride) {
+        const int n = d.cSurf[t];
+        const double2 xn = cpos(d, n);
+        const double2 pn = d.cPn[n];
+        const int last = d.cLast[n];
+        const int ci = clampi(jint((xn.x - g.r0) / g.cell), g.cols);
+        const int cj = clampi(jint((xn.y - g.z0) / g.cell), g.rows);
+        int best = -1;
+        double deepest = 0.0;
+        unsigned long long bestKey = ~0ULL;
+        bool fell = false;
+        for (int dj = -1; dj <= 1; dj++) {
+            const int j = cj + dj;
+            if (j < 0 || j >= g.rows) continue;
+            for (int di = -1; di <= 1; di++) {
+                const int i = ci + di;
+                if (i < 0 || i >= g.cols) continue;
+                const unsigned long long rank = (unsigned long long)((dj + 1) * 3 + (di + 1)) << 40;
+                for (int x = d.cHead[c_bucket(i, j, g.mask)]; x >= 0; x = d.cNext[x]) {
+                    const int2 cell = d.cEntCell[x];
+                    if (cell.x != i || cell.y != j) continue;
+                    const int p = x >> 1;
+                    int a, b;
+                    seg_ends(d, p, a, b);
+                    if (c_excluded(d, n, pn, p, a, b)) continue;
+                    const double dd = c_depth(d, xn, p, a, b);
+                    if (dd <= 0.0) continue;
+                    if (dd > C_MAX_DEPTH * d.cLen[p]) {
+                        if (p == last) fell = true;
+                        continue;
+                    }
+                    const unsigned long long key = rank | (unsigned long long)x;
+                    if (dd > deepest || (dd == deepest && key < bestKey)) {
+                        deepest = dd;
+                        best = p;
+                        bestKey = key;
+                    }
+                }
+            }
+        }
+        // accumulate :449-459
+        d.cPairOk[n] = 0;
+        if (best < 0) {
+            if (last >= 0 && fell) escaped++;
+            d.cLast[n] = -1;
+            continue;
+        }
+        d.cLast[n] = best;
+
+        // resolve :660-762
+        int a, b;
+        seg_ends(d, best, a, b);
+        const double dep = c_depth(d, xn, best, a, b);
+        const double len = d.cLen[best];
+        const double2 xa = cpos(d, a), xb = cpos(d, b);
+        const double tr0 = (xb.x - xa.x) / len, tz0 = (xb.y - xa.y) / len;
+        const double tpar = ((xn.x - xa.x) * tr0 + (xn.y - xa.y) * tz0) / len;  // parameter :652-658
+        const double wa = 1.0 - tpar, wb = tpar;
+        const double ma = d.mass[a], mb = d.mass[b], mn = d.mass[n];
+        const double invMaster = 0.25 / ma + 0.25 / mb;
+        if (!(invMaster > 0.0)) continue;
+        const double mMaster = 1.0 / invMaster;
+        const double mStar = mn * mMaster / (mn + mMaster);
+        const double k = d.cScale * mStar / (d.cRef * d.cRef);
+        const double2 ns = d.cNrm[best];
+        const double nR = ns.x, nZ = ns.y;
+        const double mTrue = 1.0 / (wa * wa / ma + wb * wb / mb);
+        const double mTangent = mn * mTrue / (mn + mTrue);
+        const double2 vn = d.v[n], va = d.v[a], vb = d.v[b];
+        const double relR = vn.x - wa * va.x - wb * vb.x;
+        const double relZ = vn.y - wa * va.y - wb * vb.y;
+        const double closing = relR * nR + relZ * nZ;
+        const double damper = 2.0 * d.cDamping * mStar * __builtin_sqrt(d.cScale) / d.cRef;
+        const double normal = nfd::jmax(0.0, k * dep - damper * closing);
+        double* rec = d.cPair + (size_t)n * CPAIR;
+        rec[PR_NORMAL] = normal;
+        rec[PR_WA] = wa;
+        rec[PR_WB] = wb;
+        rec[PR_NR] = nR;
+        rec[PR_NZ] = nZ;
+        rec[PR_ENERGY] = 0.5 * k * dep * dep;
+        rec[PR_DEPTH] = dep;
+        const double damperForce = normal - k * dep;
+        double damperLoss = 0.0;
+        if (damperForce != 0.0) {
+            const double closingAfter = closing + damperForce * dt / mTangent;
+            damperLoss = damperForce * 0.5 * (closing + closingAfter) * dt;
+        }
+        rec[PR_DAMPER] = damperLoss;
+        double tangent = 0.0, frictionLoss = 0.0;
+        if (!(d.cFriction <= 0.0)) {
+            const double tR = -nZ, tZ = nR;
+            const double slide = relR * tR + relZ * tZ;
+            tangent = -C_STICK_RELAXATION * mTangent * slide / dt;
+            const double cone = d.cFriction * normal;
+            if (tangent > cone) tangent = cone;
+            else if (tangent < -cone) tangent = -cone;
+            const double after = slide + tangent * dt / mTangent;
+            frictionLoss = -tangent * 0.5 * (slide + after) * dt;
+        }
+        rec[PR_TANGENT] = tangent;
+        rec[PR_FRICTION] = frictionLoss;
+        d.cPairOk[n] = 1;
+        d.cPairNext[n] = atomicExch(&d.cPairHead[best], n);
+    }
+    for (int off = 32; off > 0; off >>= 1) escaped += __shfl_xor(escaped, off, 64);
+    if ((threadIdx.x & 63) == 0 && escaped) atomicAdd((unsigned long long*)&c->cEscaped, (unsigned long long)escaped);
+}
+
+// The pair terms that touch one node, added to its gathered force in the reference's
+// order: by slave node ascending (accumulate walks the surface nodes in order), and within
+// a pair the normal terms before the tangential ones (resolve :687-759).
+__global__ void __launch_bounds__(BLOCK) k_c_apply(Dev d, int q, int checkGo) {
+    if (c_skip(d, q, checkGo)) return;
+    Ctl* c = d.ctl;
+    const bool any = c->cNAct > 0;
+    const bool friction = !(d.cFriction <= 0.0);  // :725, NaN included
+    double acc[2] = {0.0, 0.0};
+    double maxPen = 0.0;
+    int pairs = 0;
+    const int stride = gridDim.x * BLOCK;
+    for (int n = blockIdx.x * BLOCK + threadIdx.x; any && n < d.N; n += stride) {
+        if (!d.cSurfFlag[n]) continue;
+        int who[C_MAX_TERMS];
+        int role[C_MAX_TERMS];  // 0 slave, 1 end a, 2 end b
+        int m = 0;
+        if (d.cPairOk[n]) {
+            who[m] = n;
+            role[m] = 0;
+            m++;
+            const double* rec = d.cPair + (size_t)n * CPAIR;
+            pairs++;
+            acc[0] += rec[PR_ENERGY];
+            acc[1] = (acc[1] - rec[PR_DAMPER]) + rec[PR_FRICTION];
+            if (rec[PR_DEPTH] > maxPen) maxPen = rec[PR_DEPTH];
+        }
+        for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1]; k++) {
+            const int p = d.cNodeSegOf[k];
+            if (!d.cActive[p]) continue;
+            int a, b;
+            seg_ends(d, p, a, b);
+            for (int s = d.cPairHead[p]; s >= 0; s = d.cPairNext[s]) {
+                if (m == C_MAX_TERMS) {
+                    c->cOverflow = 1;
+                    break;
+                }
+                who[m] = s;
+                role[m] = n == a ? 1 : 2;
+                m++;
+            }
+        }
+        if (m == 0) continue;
+        // Insertion sort by slave node; a node meets each pair at most once.
+        for (int i = 1; i < m; i++) {
+            const int w = who[i], r = role[i];
+            int j = i - 1;
+            while (j >= 0 && who[j] > w) {
+                who[j + 1] = who[j];
+                role[j + 1] = role[j];
+                j--;
+            }
+            who[j + 1] = w;
+            role[j + 1] = r;
+        }
+        double2 f = d.f[n];
+        for (int i = 0; i < m; i++) {
+            const double* rec = d.cPair + (size_t)who[i] * CPAIR;
+            const double normal = rec[PR_NORMAL], nR = rec[PR_NR], nZ = rec[PR_NZ];
+            const double tangent = rec[PR_TANGENT];
+            const double tR = -nZ, tZ = nR;
+            if (role[i] == 0) {
+                f.x += normal * nR;
+                f.y += normal * nZ;
+                if (friction) {
+                    f.x += tangent * tR;
+                    f.y += tangent * tZ;
+                }
+            } else {
+                const double w = role[i] == 1 ? rec[PR_WA] : rec[PR_WB];
+                f.x -= w * normal * nR;
+                f.y -= w * normal * nZ;
+                if (friction) {
+                    f.x -= w * tangent * tR;
+                    f.y -= w * tangent * tZ;
+                }
+            }
+        }
+        d.f[n] = f;
+    }
+    // Fixed-order per-block audit partials; block 0 of the nodal launch adds them up.
+    __shared__ double lds[4 * 2];
+    block_sum<2>(acc, lds);
+    for (int off = 32; off > 0; off >>= 1) {
+        pairs += __shfl_xor(pairs, off, 64);
+        const double o = __shfl_xor(maxPen, off, 64);
+        maxPen = o > maxPen ? o : maxPen;
+    }
+    __shared__ double ldsM[BLOCK / 64];
+    __shared__ int ldsP[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) {
+        ldsM[threadIdx.x >> 6] = maxPen;
+        ldsP[threadIdx.x >> 6] = pairs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0.0;
+        int pc = 0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            mx = ldsM[w] > mx ? ldsM[w] : mx;
+            pc += ldsP[w];
+        }
+        d.cPartial[3 * blockIdx.x] = acc[0];
+        d.cPartial[3 * blockIdx.x + 1] = acc[1];
+        d.cPartial[3 * blockIdx.x + 2] = mx;
+        d.cPairsPartial[blockIdx.x] = pc;
+    }
+}
+
+// The contact audits of the assembly just finished (Contact.begin resets pairs, energy and
+// maxPenetration; dissipation and escapes run on), from k_c_apply's partials in block order.
+__device__ void reduce_contact_partials(const Dev& d) {
+    __shared__ double lds[4 * 2];
+    __shared__ double ldsM[BLOCK / 64];
+    __shared__ int ldsP[BLOCK / 64];
+    double tot[2] = {0.0, 0.0};
+    double mx = 0.0;
+    int pc = 0;
+    for (int b = threadIdx.x; b < d.cApplyBlocks; b += BLOCK) {
+        tot[0] += d.cPartial[3 * b];
+        tot[1] += d.cPartial[3 * b + 1];
+        mx = d.cPartial[3 * b + 2] > mx ? d.cPartial[3 * b + 2] : mx;
+        pc += d.cPairsPartial[b];
+    }
+    block_sum<2>(tot, lds);
+    for (int off = 32; off > 0; off >>= 1) {
+        pc += __shfl_xor(pc, off, 64);
+        const double o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        ldsM[threadIdx.x >> 6] = mx;
+        ldsP[threadIdx.x >> 6] = pc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Ctl* c = d.ctl;
+        double m = 0.0;
+        int p = 0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            m = ldsM[w] > m ? ldsM[w] : m;
+            p += ldsP[w];
+        }
+        c->cPairs = p;
+        c->cEnergy = tot[0];
+        c->cMaxPen = m;
+        c->cDissipation += tot[1];
+    }
+}

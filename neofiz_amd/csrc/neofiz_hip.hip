@@ -25,6 +25,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -78,6 +79,15 @@ struct Ctl {
     double erodedEnergy;
     int surfaceStale;
     int pad;
+    // Penalty contact (solver/Contact.java). Counts of the compact lists of live surface
+    // segments and surface nodes; the grid's bounding box as order-preserving keys
+    // {rlo, zlo, -rhi, -zhi} (all minima) and the longest segment's bits; the audits.
+    int cNAct, cNSurf;
+    unsigned long long cBox[4];
+    unsigned long long cLongest;
+    int cPairs, cOverflow;
+    double cEnergy, cDissipation, cMaxPen;
+    long long cEscaped;
 };
 
 constexpr unsigned long long MAX_BITS = 0x7FEFFFFFFFFFFFFFULL;  // Double.MAX_VALUE
@@ -122,6 +132,34 @@ struct Dev {
     double plane, friction;  // friction may be +inf
     double targetPressure, rampDuration;
     int nodalBlocks, elementBlocks;
+    // Penalty contact (null when off). Segments are indexed by their potential index
+    // p = 4 e + k (edge k of element e, from corner k to corner k + 1), which orders them
+    // exactly as the reference's surface list does; surface nodes by node index.
+    int contact;
+    uint8_t* cDead;          // [E] elements the contact has forgotten
+    uint8_t* cActive;        // [4E] p is a live surface segment
+    uint8_t* cSurfFlag;      // [N] node is on the surface
+    uint8_t* cPairOk;        // [N] node resolved a pair this assembly
+    int* cAct;               // compact list of live segments (any order)
+    int* cSurf;              // compact list of surface nodes (any order)
+    const int* cShareStart;  // [4E+1] CSR: the other potential segments with the same edge
+    const int* cShareOf;
+    const int* cNodeSegStart;  // [N+1] CSR: potential segments touching a node, ascending
+    const int* cNodeSegOf;
+    double* cLen;            // [4E]
+    double2* cNrm;           // [4E] outward unit normal
+    double2* cPn;            // [N] node normal
+    int* cLast;              // [N] lastMaster (potential index) or -1
+    int* cHead;              // hash-grid heads
+    int* cNext;              // [8E] per entry 2p + end
+    int2* cEntCell;          // [8E] the entry's grid cell
+    int* cPairHead;          // [4E] resolved slaves whose master is p
+    int* cPairNext;          // [N]
+    double* cPair;           // [N * CPAIR] the resolved pair of each slave node
+    double* cPartial;        // [cApplyBlocks * 3] energy, dissipation, max penetration
+    int* cPairsPartial;      // [cApplyBlocks]
+    int cApplyBlocks;
+    double cFriction, cDamping, cScale, cRef;
 };
 
 __device__ __forceinline__ double pressureAt(const Dev& d, double t) {  // :1203-1207
@@ -242,7 +280,12 @@ __device__ __forceinline__ double2 gather_node(const Dev& d, int n) {
     return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
 }
 
+#include "device_contact.h"
+
 constexpr int PHASE_STEP = 0, PHASE_START = 1, PHASE_ASSEMBLE_END = 2;
+// With penalty contact the launch is split around the contact launches: PART_GATHER only
+// gathers, PART_REST does everything after the gather.
+constexpr int PART_ALL = 0, PART_GATHER = 1, PART_REST = 2;
 
 // The node-centred launch that closes step n and opens step n + 1 (the reference's
 // advanceMaterialAndAssemble tail, the adaptive step, and the next step's nodal update,
@@ -254,16 +297,23 @@ constexpr int PHASE_STEP = 0, PHASE_START = 1, PHASE_ASSEMBLE_END = 2;
 //   PHASE_ASSEMBLE_END  gather only, after advance().
 // Reads control buffer q; block 0 writes buffer q ^ 1.
 template <bool WALL>
-__global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst) {
+__global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part) {
     Ctl* c = d.ctl;
     const Step& in = c->s[q];
     Step& out = c->s[q ^ 1];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (phase == PHASE_STEP && !in.go) {  // nothing ran: carry the state over
-        if (lead) out = in;
+        if (lead && part != PART_GATHER) out = in;
         return;
     }
-    const bool gather = phase == PHASE_STEP || gatherFirst;
+    const bool gathered = phase == PHASE_STEP || gatherFirst;
+    const bool gather = gathered && part != PART_REST;
+    if (part == PART_GATHER) {
+        const int i = blockIdx.x * BLOCK + threadIdx.x;
+        if (gather && i < d.N) d.f[i] = gather_node(d, i);
+        return;
+    }
+    if (part == PART_REST && gathered && blockIdx.x == 0) reduce_contact_partials(d);
     double dt = in.dt, dtPrev = in.dtPrev;
     bool next = true;
     if (phase == PHASE_STEP) {
@@ -476,6 +526,7 @@ __device__ bool erode_element(const Dev& d, int e, int4 n, double& energy, int& 
     const bool inv = currentArea(d, n) <= 0.0;
     if (!failed && !inv) return false;
     d.gone[e] = 1;
+    if (d.contact) d.cDead[e] = 1;  // contact.forget(e) :676
     eroded++;
     if (inv && !failed) inverted++;
     const Mat m = load_mat(d, e);
@@ -808,6 +859,10 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
         if ((threadIdx.x & 63) == 0 && (a | b | sflag)) {
             atomicAdd(&c->erodedCount, a);
             atomicAdd(&c->invertedCount, b);
+            if (sflag && d.contact) {
+                c->cNAct = 0;  // the rebuild that follows appends from zero
+                c->cNSurf = 0;
+            }
             if (sflag) c->surfaceStale = 1;
         }
         if (threadIdx.x == 0) d.erodePartial[blockIdx.x] = ev[0];
@@ -1171,16 +1226,43 @@ struct nf_solver {
     }
 
     template <bool WALL>
-    int launchGNT(int phase, int parity, int gatherFirst, hipEvent_t a, hipEvent_t b) {
-        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst);
-        else hipLaunchKernelGGL(k_gn<WALL>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst);
+    int launchGNT(int phase, int parity, int gatherFirst, int part, hipEvent_t a, hipEvent_t b) {
+        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part);
+        else hipLaunchKernelGGL(k_gn<WALL>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
 
+    int launchGNPart(int phase, int parity, int gatherFirst, int part, hipEvent_t a = nullptr, hipEvent_t b = nullptr) {
+        return hasWall ? launchGNT<true>(phase, parity, gatherFirst, part, a, b)
+                       : launchGNT<false>(phase, parity, gatherFirst, part, a, b);
+    }
+
+    // The contact launches between the gather and the rest of the node pass
+    // (advanceMaterialAndAssemble :1100-1108). checkGo: inside a step, which runs only if
+    // the device's end test let it.
+    int launchContact(int parity, int checkGo, hipEvent_t a = nullptr, hipEvent_t b = nullptr) {
+        const int g4 = std::min(blocks_for(4 * E), CONTACT_GRID);
+        const int gn = std::min(blocks_for(N), CONTACT_GRID);
+        if (a) HIP_TRY(hipEventRecord(a, stream));
+        hipLaunchKernelGGL(k_c_rebuild, dim3(std::max(g4, gn)), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_geometry, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_grid, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_search, dim3(gn), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_apply, dim3(d.cApplyBlocks), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        if (b) HIP_TRY(hipEventRecord(b, stream));
+        HIP_TRY(hipGetLastError());
+        return NF_OK;
+    }
+
+    // The node-centred launch of an assembly that gathered (phase, gatherFirst), with the
+    // contact launches spliced in after the gather when contact is on.
     int launchGN(int phase, int parity, int gatherFirst, hipEvent_t a = nullptr, hipEvent_t b = nullptr) {
-        return hasWall ? launchGNT<true>(phase, parity, gatherFirst, a, b)
-                       : launchGNT<false>(phase, parity, gatherFirst, a, b);
+        const bool gathers = phase == PHASE_STEP || gatherFirst;
+        if (!hasContact || !gathers) return launchGNPart(phase, parity, gatherFirst, PART_ALL, a, b);
+        TRY(launchGNPart(phase, parity, gatherFirst, PART_GATHER));
+        TRY(timed(2, [&](hipEvent_t x, hipEvent_t y) { return launchContact(parity, phase == PHASE_STEP, x, y); }));
+        return launchGNPart(phase, parity, gatherFirst, PART_REST, a, b);
     }
 
     // One step on the stream: the element pass of the open step, then the launch that
@@ -1242,7 +1324,6 @@ struct nf_solver {
     // zero-interval assembly is followed by the first nodal update (PHASE_START with
     // gatherFirst); advance() ends with a bare gather.
     int assembleAdvance() {
-        if (hasContact) return fail(NF_E_STATE, "penalty contact is not available in this build");
         TRY(launchElement(MODE_ADVANCE, q));
         TRY(launchGN(PHASE_ASSEMBLE_END, q, 1));
         q ^= 1;
@@ -1256,7 +1337,6 @@ struct nf_solver {
         const long long start = h.steps;
         if (taken) *taken = 0;
         if (!(h.time < endTime) || maxSteps == 0) return NF_OK;
-        if (hasContact) return fail(NF_E_STATE, "penalty contact is not available in this build");
         h.endTime = endTime;
         h.stepLimit = maxSteps > 0 ? start + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
         TRY(writeStep(h));
@@ -1298,9 +1378,13 @@ struct nf_solver {
                 slot ^= 1;
             }
         }
-        TRY(readStep(h));
+        Ctl full;
+        TRY(readCtl(full));
+        h = full.s[q];
         TRY(harvest());
         if (taken) *taken = h.steps - start;
+        if (full.cOverflow)
+            return fail(NF_E_STATE, "contact: a node received more than 32 pair terms in one step");
         return NF_OK;
     }
 };
@@ -1618,12 +1702,122 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
     g_error.clear();
     if (!enabled) {
         s->hasContact = false;
+        s->d.contact = 0;
+        s->graphDirty = true;
         return NF_OK;
     }
     if (friction < 0.0) return fail(NF_E_ARGUMENT, "friction must not be negative");
     if (damping < 0.0 || damping > 1.0) return fail(NF_E_ARGUMENT, "contact damping is a fraction of critical, so it lies in [0, 1]");
     if (!(scale > 0.0) || scale >= 2.0) return fail(NF_E_ARGUMENT, "the stiffness scale must lie in (0, 2)");
-    return fail(NF_E_STATE, "penalty contact is not available in this build");
+    Dev& d = s->d;
+    const int N = s->N, E = s->E;
+    if (!d.cDead) {
+        // Static topology, built once: for every potential segment p = 4 e + k the other
+        // potential segments with the same edge, and for every node the potential segments
+        // touching it in ascending order.
+        std::vector<std::pair<unsigned long long, int>> keyed(4 * (size_t)E);
+        for (int p = 0; p < 4 * E; p++) {
+            const int a = s->conn[p], b = s->conn[(p & ~3) + ((p + 1) & 3)];
+            const unsigned lo = (unsigned)std::min(a, b), hi = (unsigned)std::max(a, b);
+            keyed[p] = {((unsigned long long)lo << 32) | hi, p};
+        }
+        std::sort(keyed.begin(), keyed.end());
+        std::vector<int> shareStart(4 * (size_t)E + 1, 0), shareOf;
+        std::vector<std::vector<int>> shares(4 * (size_t)E);
+        for (size_t i = 0; i < keyed.size();) {
+            size_t j = i;
+            while (j < keyed.size() && keyed[j].first == keyed[i].first) j++;
+            for (size_t x = i; x < j; x++)
+                for (size_t y = i; y < j; y++)
+                    if (x != y) shares[keyed[x].second].push_back(keyed[y].second);
+            i = j;
+        }
+        for (int p = 0; p < 4 * E; p++) {
+            shareStart[p + 1] = shareStart[p] + (int)shares[p].size();
+            for (int o : shares[p]) shareOf.push_back(o);
+        }
+        std::vector<int> nodeStart(N + 1, 0), nodeOf(8 * (size_t)E);
+        for (int p = 0; p < 4 * E; p++) {
+            nodeStart[s->conn[p] + 1]++;
+            nodeStart[s->conn[(p & ~3) + ((p + 1) & 3)] + 1]++;
+        }
+        for (int n = 0; n < N; n++) nodeStart[n + 1] += nodeStart[n];
+        {
+            std::vector<int> cur(nodeStart.begin(), nodeStart.end() - 1);
+            for (int p = 0; p < 4 * E; p++) {
+                nodeOf[cur[s->conn[p]]++] = p;
+                nodeOf[cur[s->conn[(p & ~3) + ((p + 1) & 3)]]++] = p;
+            }
+        }
+        int* shareStartD = nullptr;
+        int* shareOfD = nullptr;
+        int* nodeStartD = nullptr;
+        int* nodeOfD = nullptr;
+        unsigned hmax = 64;
+        while (hmax < 16u * (unsigned)E) hmax <<= 1;
+        d.cApplyBlocks = std::min(blocks_for(N), CONTACT_GRID);
+        TRY(s->alloc(&d.cDead, E));
+        TRY(s->alloc(&d.cActive, 4 * (size_t)E));
+        TRY(s->alloc(&d.cSurfFlag, N));
+        TRY(s->alloc(&d.cPairOk, N));
+        TRY(s->alloc(&d.cAct, 4 * (size_t)E));
+        TRY(s->alloc(&d.cSurf, N));
+        TRY(s->alloc(&shareStartD, 4 * (size_t)E + 1));
+        TRY(s->alloc(&shareOfD, std::max<size_t>(1, shareOf.size())));
+        TRY(s->alloc(&nodeStartD, N + 1));
+        TRY(s->alloc(&nodeOfD, 8 * (size_t)E));
+        TRY(s->alloc(&d.cLen, 4 * (size_t)E));
+        TRY(s->alloc(&d.cNrm, 4 * (size_t)E));
+        TRY(s->alloc(&d.cPn, N));
+        TRY(s->alloc(&d.cLast, N));
+        TRY(s->alloc(&d.cHead, hmax));
+        TRY(s->alloc(&d.cNext, 8 * (size_t)E));
+        TRY(s->alloc(&d.cEntCell, 8 * (size_t)E));
+        TRY(s->alloc(&d.cPairHead, 4 * (size_t)E));
+        TRY(s->alloc(&d.cPairNext, N));
+        TRY(s->alloc(&d.cPair, (size_t)CPAIR * N));
+        TRY(s->alloc(&d.cPartial, 3 * (size_t)d.cApplyBlocks));
+        TRY(s->alloc(&d.cPairsPartial, d.cApplyBlocks));
+        HIP_TRY(hipMemcpy(shareStartD, shareStart.data(), sizeof(int) * shareStart.size(), hipMemcpyHostToDevice));
+        if (!shareOf.empty())
+            HIP_TRY(hipMemcpy(shareOfD, shareOf.data(), sizeof(int) * shareOf.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(nodeStartD, nodeStart.data(), sizeof(int) * nodeStart.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(nodeOfD, nodeOf.data(), sizeof(int) * nodeOf.size(), hipMemcpyHostToDevice));
+        d.cShareStart = shareStartD;
+        d.cShareOf = shareOfD;
+        d.cNodeSegStart = nodeStartD;
+        d.cNodeSegOf = nodeOfD;
+    }
+    // new Contact(mesh) :273-280: nothing forgotten, no history, audits from zero.
+    HIP_TRY(hipMemsetAsync(d.cDead, 0, E, s->stream));
+    Ctl h;
+    TRY(s->readCtl(h));
+    h.surfaceStale = 1;
+    h.cNAct = 0;
+    h.cNSurf = 0;
+    h.cPairs = 0;
+    h.cOverflow = 0;
+    h.cEnergy = 0.0;
+    h.cDissipation = 0.0;
+    h.cMaxPen = 0.0;
+    h.cEscaped = 0;
+    TRY(s->writeCtl(h));
+    d.contact = 1;
+    d.cFriction = friction;
+    d.cDamping = damping;
+    d.cScale = scale;
+    d.cRef = s->dt0;
+    hipLaunchKernelGGL(k_c_rebuild, dim3(std::min(blocks_for(4 * E), CONTACT_GRID)), dim3(BLOCK), 0,
+                       s->stream, d, s->q, 0);
+    HIP_TRY(hipGetLastError());
+    // The lists are built: clear the flag so the first assembly does not append them again.
+    TRY(s->sync());
+    TRY(s->readCtl(h));
+    h.surfaceStale = 0;
+    TRY(s->writeCtl(h));
+    s->hasContact = true;
+    s->graphDirty = true;
+    return NF_OK;
 }
 int nf_set_gravity(nf_solver* s, double ar, double az) {
     s->d.gravityR = ar;
@@ -1728,6 +1922,7 @@ int nf_set_mass_scaling(nf_solver* s, double target) {  // :601-619
     h.dt = target;
     h.dtPrev = target;
     s->dt0 = target;
+    s->d.cRef = target;
     s->graphDirty = true;
     return s->writeStep(h);
 }
@@ -1897,6 +2092,61 @@ int nf_get_state(nf_solver* s, nf_state* o) {
         o->wall_energy_loss = h.wallEnergyLoss;
         o->wall_friction_dissipation = h.wallFriction;
     }
+    if (s->hasContact) {
+        o->contact_pairs = h.cPairs;
+        o->contact_energy = h.cEnergy;
+        o->contact_dissipation = h.cDissipation;
+        o->contact_max_penetration = h.cMaxPen;
+        o->contact_escaped_nodes = h.cEscaped;
+    }
+    return NF_OK;
+}
+
+// The contact's pairs of the last step, in surface-node order, as (node, a, b).
+int nf_contact_pairs(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count) {
+    g_error.clear();
+    if (!count) return fail(NF_E_ARGUMENT, "count must not be null");
+    *count = 0;
+    if (!s->hasContact) return NF_OK;
+    std::vector<int> last(s->N);
+    std::vector<uint8_t> on(s->N);
+    HIP_TRY(hipMemcpyAsync(last.data(), s->d.cLast, sizeof(int) * s->N, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(on.data(), s->d.cSurfFlag, s->N, hipMemcpyDeviceToHost, s->stream));
+    TRY(s->sync());
+    int k = 0;
+    for (int n = 0; n < s->N; n++) {
+        if (!on[n] || last[n] < 0) continue;
+        const int p = last[n];
+        if (k < capacity && out) {
+            out[3 * k] = n;
+            out[3 * k + 1] = s->conn[p];
+            out[3 * k + 2] = s->conn[(p & ~3) + ((p + 1) & 3)];
+        }
+        k++;
+    }
+    *count = k;
+    return NF_OK;
+}
+
+// The contact's current surface segments (a, b) in surface order.
+int nf_contact_surface(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count) {
+    g_error.clear();
+    if (!count) return fail(NF_E_ARGUMENT, "count must not be null");
+    *count = 0;
+    if (!s->hasContact) return NF_OK;
+    std::vector<uint8_t> live(4 * (size_t)s->E);
+    HIP_TRY(hipMemcpyAsync(live.data(), s->d.cActive, live.size(), hipMemcpyDeviceToHost, s->stream));
+    TRY(s->sync());
+    int k = 0;
+    for (int p = 0; p < 4 * s->E; p++) {
+        if (!live[p]) continue;
+        if (k < capacity && out) {
+            out[2 * k] = s->conn[p];
+            out[2 * k + 1] = s->conn[(p & ~3) + ((p + 1) & 3)];
+        }
+        k++;
+    }
+    *count = k;
     return NF_OK;
 }
 
