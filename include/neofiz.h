@@ -193,6 +193,26 @@ int nf_get_audits(nf_solver* s, nf_audits* out);
 int nf_contact_pairs(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count);
 int nf_contact_surface(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count);
 
+/* Batched ensemble (no single counterpart in the reference, which runs an ensemble as one
+ * solver per thread: M0Cylinder.java:870-907, sweep/Sweep.java:77-106). A batch is made from
+ * solvers already set up through the calls above (mesh, material, wall, velocities ...) and
+ * steps them all together on the GPU, each with its own time step and end time; every
+ * instance's state stays bit-identical to what a solo run would give. nf_batch_scatter
+ * copies each instance's state back into its solver, whose getters then read it. A batch
+ * refuses solvers with contact, damage, erosion, pressure edges or several materials
+ * (NF_E_STATE), and mixed integration rules (NF_E_ARGUMENT); the wall audits of the
+ * solvers are not advanced by a batch. The solvers must outlive the batch.
+ * nf_batch_set_tile sets the threads per tile (64, 128 or 256; instance -1 = all) that an
+ * instance's elements and nodes are cut into; nf_batch_run_until runs every instance while
+ * its time is below end_time, at most max_steps steps each when max_steps > 0. */
+typedef struct nf_batch nf_batch;
+int nf_batch_create(nf_batch** out, int32_t count, nf_solver* const* solvers);
+int nf_batch_destroy(nf_batch* b);
+int nf_batch_set_tile(nf_batch* b, int32_t instance, int32_t threads);
+int nf_batch_get_tile(nf_batch* b, int32_t instance, int32_t* threads);
+int nf_batch_run_until(nf_batch* b, double end_time, int64_t max_steps);
+int nf_batch_scatter(nf_batch* b);
+
 /* Per-kernel timing of the step loop (no counterpart in the reference, which times whole
  * runs with System.nanoTime, validate/TaylorImpactCase.java:254-260). When enabled, each
  * launch is bracketed by HIP events on the handle's stream; nf_profile_read returns summed

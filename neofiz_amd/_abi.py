@@ -86,6 +86,12 @@ _PROTOTYPES = {
     "get_audits": (C.c_int, [_P, C.POINTER(AuditsC)]),
     "contact_pairs": (C.c_int, [_P, C.c_int32, c_int32_p, c_int32_p]),
     "contact_surface": (C.c_int, [_P, C.c_int32, c_int32_p, c_int32_p]),
+    "batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
+    "batch_destroy": (C.c_int, [_P]),
+    "batch_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "batch_get_tile": (C.c_int, [_P, C.c_int32, c_int32_p]),
+    "batch_run_until": (C.c_int, [_P, C.c_double, C.c_int64]),
+    "batch_scatter": (C.c_int, [_P]),
     "math_eval": (C.c_int, [C.c_int32, c_double_p, c_double_p, c_double_p, C.c_int32]),
     "profile": (C.c_int, [_P, C.c_int32]),
     "point_path": (C.c_int, [C.POINTER(MaterialC), C.c_int32, C.c_int32, C.c_int32, c_double_p,

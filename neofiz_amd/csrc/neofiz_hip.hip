@@ -1389,6 +1389,8 @@ struct nf_solver {
     }
 };
 
+#include "batch.h"
+
 // ------------------------------------------------------------------ C ABI
 
 extern "C" {
@@ -2412,6 +2414,163 @@ int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int3
     (void)hipFree(dout);
     if (err != hipSuccess) return fail(NF_E_DEVICE, hipGetErrorString(err));
     return NF_OK;
+}
+
+
+// ------------------------------------------------------------------ batched ensemble
+
+int nf_batch_create(nf_batch** out, int32_t count, nf_solver* const* solvers) {
+    g_error.clear();
+    if (!out) return fail(NF_E_ARGUMENT, "out must not be null");
+    *out = nullptr;
+    if (count < 1 || !solvers) return fail(NF_E_ARGUMENT, "a batch needs at least one solver");
+    int integration = -1;
+    for (int i = 0; i < count; i++) {
+        nf_solver* s = solvers[i];
+        if (!s) return fail(NF_E_ARGUMENT, "null solver in batch");
+        for (int j = 0; j < i; j++)
+            if (solvers[j] == s) return fail(NF_E_ARGUMENT, "a solver appears twice in the batch");
+        if (s->hasContact || s->hasDamage || s->hasErosion || s->edges || !s->materialOf.empty())
+            return fail(NF_E_STATE, "a batch carries neither contact, damage, erosion, pressure edges nor "
+                                    "several materials in one instance");
+        if (integration < 0) integration = s->integration;
+        if (s->integration != integration) return fail(NF_E_ARGUMENT, "every instance of a batch uses the same integration");
+        TRY(s->sync());
+    }
+    nf_batch* b = new nf_batch();
+    auto bail = [&](int rc) {
+        delete b;
+        return rc;
+    };
+    int rc;
+    if (const char* g = std::getenv("NEOFIZ_BATCH_GRAPHS")) b->useGraphs = g[0] != '0';
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(NF_E_DEVICE, "hipStreamCreate failed: no usable GPU"));
+    b->full = integration == NF_FULL;
+    const int ppe = b->full ? 4 : 1;
+    std::vector<int4> conn;
+    std::vector<int> slotStart(1, 0), slotOf;
+    std::vector<int4> slot4;
+    for (int i = 0; i < count; i++) {
+        nf_solver* s = solvers[i];
+        b->solvers.push_back(s);
+        BInst I;
+        std::memset(&I, 0, sizeof I);
+        I.mat = s->d.mat0;
+        I.wall = s->hasWall ? 1 : 0;
+        I.wallFrictional = s->d.wallFrictional;
+        I.friction = s->d.friction;
+        I.plane = s->hasWall ? s->d.plane : -HUGE_VAL;  // no wall: no node ever arrives
+        I.cfl = s->d.cfl;
+        I.fastestWave = s->d.fastestWave;
+        I.massScale = s->d.massScale;
+        I.gravityR = s->d.gravityR;
+        I.gravityZ = s->d.gravityZ;
+        I.damping = s->d.damping;
+        I.qhg = s->d.qhg;
+        I.thickness = s->d.thickness;
+        I.axi = s->d.axi;
+        I.finite = s->d.finite;
+        I.nodeBase = b->N;
+        I.nodes = s->N;
+        I.elemBase = b->E;
+        I.elems = s->E;
+        b->inst.push_back(I);
+        b->instTs.push_back(BLOCK);
+        b->live.push_back(1);
+        if (s->hasKin) b->kin = true;
+        if (s->hasWall) b->wall = true;
+        for (int e = 0; e < s->E; e++)
+            conn.push_back(make_int4(s->conn[4 * e] + b->N, s->conn[4 * e + 1] + b->N, s->conn[4 * e + 2] + b->N,
+                                     s->conn[4 * e + 3] + b->N));
+        const int slotBase = (int)slotOf.size();
+        for (int n = 0; n < s->N; n++) {
+            slotStart.push_back(slotBase + s->slotStart[n + 1]);
+            int k[4] = {-1, -1, -1, -1};
+            for (int c = 0; c < 4 && s->slotStart[n] + c < s->slotStart[n + 1]; c++)
+                k[c] = s->slotOf[s->slotStart[n] + c] + 4 * b->E;
+            slot4.push_back(make_int4(k[0], k[1], k[2], k[3]));
+        }
+        for (int x : s->slotOf) slotOf.push_back(x + 4 * b->E);
+        b->N += s->N;
+        b->E += s->E;
+    }
+    b->P = b->E * ppe;
+    Dev& d = b->d;
+    d.N = b->N;
+    d.E = b->E;
+    d.P = b->P;
+    d.ppe = ppe;
+    d.erosionDamage = HUGE_VAL;
+    double2* x0 = nullptr;
+    double* z0 = nullptr;
+    double* mass = nullptr;
+    int4* conn4 = nullptr;
+    int* ss = nullptr;
+    int* so = nullptr;
+    int4* s4 = nullptr;
+    const size_t tilesMax = (size_t)b->E / 64 + b->N / 64 + 4 * count + 8;
+    if ((rc = b->alloc(&x0, b->N)) || (rc = b->alloc(&z0, b->N)) || (rc = b->alloc(&d.u, b->N)) ||
+        (rc = b->alloc(&d.v, b->N)) || (rc = b->alloc(&d.f, b->N)) || (rc = b->alloc(&mass, b->N)) ||
+        (rc = b->alloc(&d.flags, b->N)) || (rc = b->alloc(&conn4, b->E)) || (rc = b->alloc(&d.sig0, b->P)) ||
+        (rc = b->alloc(&d.sig1, b->P)) || (rc = b->alloc(&d.sig2, b->P)) || (rc = b->alloc(&d.sig3, b->P)) ||
+        (rc = b->alloc(&d.back0, b->P)) || (rc = b->alloc(&d.back1, b->P)) || (rc = b->alloc(&d.back2, b->P)) ||
+        (rc = b->alloc(&d.back3, b->P)) || (rc = b->alloc(&d.epsP, b->P)) || (rc = b->alloc(&d.work, b->P)) ||
+        (rc = b->alloc(&d.hgR, b->E)) || (rc = b->alloc(&d.hgZ, b->E)) || (rc = b->alloc(&d.hgK, b->E)) ||
+        (rc = b->alloc(&d.slot, 4 * (size_t)b->E)) || (rc = b->alloc(&ss, b->N + 1)) ||
+        (rc = b->alloc(&so, slotOf.size())) || (rc = b->alloc(&s4, b->N)) ||
+        (rc = b->alloc(&b->instDev, count)) || (rc = b->alloc(&b->steps, 2 * (size_t)count)) ||
+        (rc = b->alloc(&b->eBlocks, tilesMax)) || (rc = b->alloc(&b->eTiles, tilesMax)) ||
+        (rc = b->alloc(&b->nBlocks, tilesMax)) || (rc = b->alloc(&b->nTiles, tilesMax)))
+        return bail(rc);
+    d.x0 = x0;
+    d.z0 = z0;
+    d.mass = mass;
+    d.conn = conn4;
+    d.slotStart = ss;
+    d.slotOf = so;
+    d.slot4 = s4;
+    if (hipMemcpy(conn4, conn.data(), sizeof(int4) * conn.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ss, slotStart.data(), sizeof(int) * slotStart.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(so, slotOf.data(), sizeof(int) * slotOf.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s4, slot4.data(), sizeof(int4) * slot4.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->instDev, b->inst.data(), sizeof(BInst) * count, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(NF_E_DEVICE, "upload failed"));
+    if ((rc = batch_transfer(b, true))) return bail(rc);
+    *out = b;
+    return NF_OK;
+}
+
+int nf_batch_destroy(nf_batch* b) {
+    delete b;
+    return NF_OK;
+}
+
+int nf_batch_set_tile(nf_batch* b, int32_t instance, int32_t threads) {
+    g_error.clear();
+    if (instance < -1 || instance >= (int)b->inst.size()) return fail(NF_E_ARGUMENT, "instance out of range");
+    if (threads != 64 && threads != 128 && threads != 256) return fail(NF_E_ARGUMENT, "a tile is 64, 128 or 256 threads");
+    for (int i = 0; i < (int)b->inst.size(); i++)
+        if (instance < 0 || i == instance) b->instTs[i] = threads;
+    b->graphDirty = true;
+    return NF_OK;
+}
+
+int nf_batch_get_tile(nf_batch* b, int32_t instance, int32_t* threads) {
+    g_error.clear();
+    if (instance < 0 || instance >= (int)b->inst.size() || !threads) return fail(NF_E_ARGUMENT, "instance out of range");
+    *threads = b->instTs[instance];
+    return NF_OK;
+}
+
+int nf_batch_run_until(nf_batch* b, double end_time, int64_t max_steps) {
+    g_error.clear();
+    return b->runUntil(end_time, max_steps > 0 ? max_steps : -1);
+}
+
+int nf_batch_scatter(nf_batch* b) {
+    g_error.clear();
+    return batch_transfer(b, false);
 }
 
 }  // extern "C"

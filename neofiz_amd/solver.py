@@ -687,3 +687,49 @@ class ExplicitSolver:
         out = np.empty(max(1, 3 * n.value), np.int32)
         self._call("contact_pairs", n.value, _iptr(out), C.byref(n))
         return out[:3 * n.value].reshape(-1, 3)
+
+
+class Batch:
+    """Solvers stepped together on the GPU (nf_batch_* in include/neofiz.h).
+
+    Every instance keeps its own time step and end time, and its state stays bit-identical
+    to a solo run. After runUntil, scatter() copies the state back into the solvers, whose
+    accessors then read it. The solvers must share one library (one binding)."""
+
+    def __init__(self, solvers: Sequence[ExplicitSolver]):
+        if not solvers:
+            raise IllegalArgumentException("a batch needs at least one solver")
+        self._b = solvers[0]._b
+        if any(s._b is not self._b for s in solvers):
+            raise IllegalArgumentException("every solver of a batch must use the same library")
+        self.solvers = list(solvers)
+        handles = (_abi._P * len(solvers))(*[s._h for s in solvers])
+        h = _abi._P()
+        check(self._b, self._b.batch_create(C.byref(h), len(solvers), handles))
+        self._h = h
+
+    def setTile(self, instance, threads):
+        """Threads per tile (64, 128 or 256) for one instance, or every instance (-1)."""
+        check(self._b, self._b.batch_set_tile(self._h, instance, threads))
+
+    def tile(self, instance):
+        t = C.c_int32()
+        check(self._b, self._b.batch_get_tile(self._h, instance, C.byref(t)))
+        return t.value
+
+    def runUntil(self, endTime, maxSteps=0):
+        check(self._b, self._b.batch_run_until(self._h, endTime, maxSteps))
+
+    def scatter(self):
+        check(self._b, self._b.batch_scatter(self._h))
+
+    def close(self):
+        if self._h:
+            self._b.batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1698,6 +1698,41 @@ int nfo_run_until(nfo_solver* s, double end_time, int64_t max_steps, int64_t* ta
     if (taken) *taken = n;
     return NF_OK;
 }
+// The batch surface: on the CPU a batch is its solvers, each run to the end in turn.
+struct nfo_batch {
+    std::vector<nfo_solver*> solvers;
+    std::vector<int> tile;
+};
+int nfo_batch_create(nfo_batch** out, int32_t count, nfo_solver* const* solvers) {
+    if (!out || count < 1 || !solvers) return fail(NF_E_ARGUMENT, "a batch needs at least one solver");
+    auto* b = new nfo_batch();
+    b->solvers.assign(solvers, solvers + count);
+    b->tile.assign(count, 256);
+    *out = b;
+    return NF_OK;
+}
+int nfo_batch_destroy(nfo_batch* b) {
+    delete b;
+    return NF_OK;
+}
+int nfo_batch_set_tile(nfo_batch* b, int32_t instance, int32_t threads) {
+    if (instance < -1 || instance >= (int)b->tile.size()) return fail(NF_E_ARGUMENT, "instance out of range");
+    if (threads != 64 && threads != 128 && threads != 256) return fail(NF_E_ARGUMENT, "a tile is 64, 128 or 256 threads");
+    for (int i = 0; i < (int)b->tile.size(); i++)
+        if (instance < 0 || i == instance) b->tile[i] = threads;
+    return NF_OK;
+}
+int nfo_batch_get_tile(nfo_batch* b, int32_t instance, int32_t* threads) {
+    if (instance < 0 || instance >= (int)b->tile.size() || !threads) return fail(NF_E_ARGUMENT, "instance out of range");
+    *threads = b->tile[instance];
+    return NF_OK;
+}
+int nfo_batch_run_until(nfo_batch* b, double end_time, int64_t max_steps) {
+    for (nfo_solver* s : b->solvers) nfo_run_until(s, end_time, max_steps, nullptr);
+    return NF_OK;
+}
+int nfo_batch_scatter(nfo_batch*) { return NF_OK; }
+
 int nfo_advance(nfo_solver* s) {
     s->advance();
     return NF_OK;

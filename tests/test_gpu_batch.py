@@ -1,0 +1,60 @@
+"""The batched ensemble against the oracle: every instance of a batch, stepped together on
+the GPU with its own time step, ends bit-identical to the same rod run alone on the CPU."""
+import numpy as np
+import pytest
+
+from conftest import first_difference
+from neofiz_amd import ensemble
+from neofiz_amd.solver import Batch
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("ur", "uz", "vr", "vz", "fr", "fz", "on_wall")
+
+
+def small_ensemble(count, seed):
+    """Instances drawn as in the ensemble generator, meshes limited to 8x80 and 16x160."""
+    out = []
+    for inst in ensemble.draw(count * 4, seed):
+        if inst.nr <= 16:
+            out.append(inst)
+        if len(out) == count:
+            return out
+    raise AssertionError("not enough small meshes drawn")
+
+
+@pytest.mark.parametrize("tiles", [(256,), (64, 128, 256)])
+def test_batch_matches_solo_runs_bitwise(tiles):
+    from oracle.pyoracle import OracleSolver
+    insts = small_ensemble(6, 11)
+    built = [ensemble.build(i) for i in insts]
+    batch = Batch([s for _, s in built])
+    for k in range(len(insts)):
+        batch.setTile(k, tiles[k % len(tiles)])
+    end = 12e-6
+    print(f"batch of {len(insts)} to {end}", flush=True)
+    batch.runUntil(end, 300)          # a capped first call, then the rest
+    batch.runUntil(end)
+    batch.scatter()
+    for k, inst in enumerate(insts):
+        mesh, ref = ensemble.build(inst, OracleSolver)
+        ref.runUntil(end, 300)
+        ref.runUntil(end)
+        gpu = built[k][1]
+        assert gpu.steps() == ref.steps(), f"instance {k}: {gpu.steps()} vs {ref.steps()} steps"
+        assert gpu.time() == ref.time() and gpu.timestep() == ref.timestep()
+        for f in FIELDS:
+            assert first_difference(gpu.nodeField(f), ref.nodeField(f)) is None, f"instance {k}: {f}"
+        for f in ("stress", "plastic_strain", "plastic_work"):
+            assert first_difference(gpu.pointField(f), ref.pointField(f)) is None, f"instance {k}: {f}"
+        ref.shutdown()
+    batch.close()
+
+
+def test_batch_refuses_contact_and_damage():
+    from neofiz_amd.solver import Contact, IllegalStateException
+    inst = ensemble.draw(1, 3)[0]
+    mesh, s = ensemble.build(inst)
+    s.setDamage(np.full(mesh.elementCount, 1e4), np.full(mesh.elementCount, 0.5))
+    with pytest.raises(IllegalStateException):
+        Batch([s])
