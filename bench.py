@@ -83,6 +83,50 @@ def stream_copy_gbps():
     return gbps.value
 
 
+def ensemble_leg(count, cap, per_size):
+    """The batched ensemble against one-by-one on the same GPU.
+
+    Batched: all `count` instances as one batch, each to 80 us or `cap` steps. One-by-one:
+    `per_size` instances of each mesh size through the single-instance path with the same
+    end test; the one-by-one time of the whole ensemble is then estimated per mesh size as
+    (measured seconds per step) x (the steps every instance of that size took, known exactly
+    from the batched run, whose states are bit-identical to solo runs)."""
+    from neofiz_amd import ensemble, taylor
+    insts = ensemble.draw(count)
+    print(f"# ensemble: {count} instances, batched, cap {cap} steps", file=sys.stderr, flush=True)
+    meshes, solvers, batched_s, tiles = ensemble.run_batched(insts, taylor.ROD_DURATION, cap)
+    steps = [s.steps() for s in solvers]
+    reached = sum(1 for s in solvers if s.time() >= taylor.ROD_DURATION)
+    work = sum(m.elementCount * k for m, k in zip(meshes, steps))
+    sample_checks = []
+    by_size = {}
+    for inst, k in zip(insts, steps):
+        by_size.setdefault(inst.elements, []).append(k)
+    per_step = {}
+    print(f"# ensemble: one-by-one, {per_size} per mesh size", file=sys.stderr, flush=True)
+    for e in sorted(by_size):
+        picks = [(i, n) for n, i in enumerate(insts) if i.elements == e][:per_size]
+        solo = ensemble.run_one_by_one([i for i, _ in picks], taylor.ROD_DURATION, cap)
+        per_step[e] = sum(sec for _, sec, _ in solo) / sum(k for k, _, _ in solo)
+        for (inst, n), (k, _, res) in zip(picks, solo):
+            batched_res = ensemble.results(meshes[n], solvers[n])
+            sample_checks.append(k == steps[n] and res == batched_res)
+    one_by_one_s = sum(per_step[e] * sum(ks) for e, ks in by_size.items())
+    for s in solvers:
+        s.shutdown()
+    return {
+        "instances": count, "step_cap": cap, "reached_end_time": reached,
+        "element_steps": work, "value": work / batched_s, "seconds": batched_s,
+        "unit": "element-updates/s",
+        "one_by_one_value": work / one_by_one_s, "one_by_one_seconds_estimated": one_by_one_s,
+        "one_by_one_us_per_step": {str(e): v * 1e6 for e, v in per_step.items()},
+        "speedup": one_by_one_s / batched_s,
+        "tiles": {str(k): v for k, v in (tiles or {}).items()},
+        "solo_equals_batched": all(sample_checks),
+        "note": f"one-by-one measured on {per_size} instances per mesh size, scaled by the "
+                "ensemble's exact step counts"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +138,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=256, help="instances in the batched ensemble")
+    ap.add_argument("--ensemble-cap", type=int, default=20000, help="step cap per instance")
+    ap.add_argument("--ensemble-solo", type=int, default=4, help="one-by-one sample per mesh size")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -175,6 +223,8 @@ def main():
         except Exception as exc:  # torch missing or no device: report, do not hide
             line["stream_copy_GBps"] = None
             line["stream_copy_error"] = str(exc)
+    if not args.no_ensemble:
+        line["batched"] = ensemble_leg(args.ensemble, args.ensemble_cap, args.ensemble_solo)
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, 16)

@@ -80,7 +80,8 @@ def tune(batch, instances, end_time, trial_steps=24):
     size (threads per tile, so how many of its tiles share a workgroup) is chosen in turn
     from TILES by timing `trial_steps` real steps of the whole batch with every other bucket
     held at its best so far. The trial steps are part of the run (they advance every
-    instance exactly as any other steps would). Returns {elements: tile}."""
+    instance exactly as any other steps would). Returns ({elements: tile}, timings, steps
+    taken by every instance still running)."""
     buckets = sorted({i.elements for i in instances})
     choice = {b: 256 for b in buckets}
 
@@ -89,6 +90,7 @@ def tune(batch, instances, end_time, trial_steps=24):
             batch.setTile(k, choice[inst.elements])
 
     timings = {}
+    taken = 0
     for b in buckets:
         best, best_t = choice[b], None
         for ts in TILES:
@@ -98,44 +100,48 @@ def tune(batch, instances, end_time, trial_steps=24):
             t0 = time.perf_counter()
             batch.runUntil(end_time, trial_steps)
             t = time.perf_counter() - t0
+            taken += 2 + trial_steps
             timings[(b, ts)] = t
             if best_t is None or t < best_t:
                 best, best_t = ts, t
         choice[b] = best
     apply()
-    return choice, timings
+    return choice, timings, taken
 
 
-def run_batched(instances, end_time=taylor.ROD_DURATION, autotune=True):
-    """Builds every instance and runs them as one batch. Returns (solvers, meshes, seconds,
-    element-steps, tile choice). The timed region includes the tuner's steps."""
+def run_batched(instances, end_time=taylor.ROD_DURATION, max_steps=0, autotune=True):
+    """Builds every instance and runs them as one batch (each while its time is below
+    end_time, for at most max_steps steps when max_steps > 0). Returns (meshes, solvers,
+    seconds, tile choice); the solvers hold the final states. The timed region includes the
+    tuner's steps, which are part of the run."""
     built = [build(i) for i in instances]
     meshes = [m for m, _ in built]
     solvers = [s for _, s in built]
     batch = Batch(solvers)
     t0 = time.perf_counter()
     choice = None
+    taken = 0
     if autotune:
-        choice, _ = tune(batch, instances, end_time)
-    batch.runUntil(end_time)
+        choice, _, taken = tune(batch, instances, end_time)
+    if max_steps > 0:
+        max_steps = max(max_steps - taken, 0)
+    if max_steps > 0 or taken == 0:
+        batch.runUntil(end_time, max_steps)
     seconds = time.perf_counter() - t0
     batch.scatter()
     batch.close()
-    work = sum(m.elementCount * s.steps() for m, s in zip(meshes, solvers))
-    return solvers, meshes, seconds, work, choice
+    return meshes, solvers, seconds, choice
 
 
-def run_one_by_one(instances, end_time=taylor.ROD_DURATION):
-    """The same instances through the single-instance GPU path, one after another."""
-    seconds = 0.0
-    work = 0
+def run_one_by_one(instances, end_time=taylor.ROD_DURATION, max_steps=0):
+    """The same instances through the single-instance GPU path, one after another. Returns
+    per instance (steps, seconds, (L/L0, mushroom diameter))."""
     out = []
     for inst in instances:
         mesh, solver = build(inst)
         t0 = time.perf_counter()
-        solver.runUntil(end_time)
-        seconds += time.perf_counter() - t0
-        work += mesh.elementCount * solver.steps()
-        out.append(results(mesh, solver))
+        solver.runUntil(end_time, max_steps)
+        seconds = time.perf_counter() - t0
+        out.append((solver.steps(), seconds, results(mesh, solver)))
         solver.shutdown()
-    return out, seconds, work
+    return out
