@@ -193,6 +193,19 @@ int nf_get_audits(nf_solver* s, nf_audits* out);
 int nf_contact_pairs(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count);
 int nf_contact_surface(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count);
 
+/* Parameter sensitivities (no counterpart in the reference): the exact derivative of the
+ * discrete step loop with respect to {anvil friction, Johnson-Cook A, B, n, C, m}, carried
+ * forward alongside the state in those six directions. nf_enable_sensitivity switches it on
+ * before the first step, for reduced integration of one Johnson-Cook material without
+ * kinematic hardening, damage, erosion, contact or pressure edges (else NF_E_STATE); the
+ * trajectory stays bit-identical to a run without it. nf_get_node_sensitivity writes
+ * d(field)/d(parameter) per node for UR, UZ, VR, VZ, FR, FZ; nf_get_step_sensitivity writes
+ * the derivatives of the current and the previous time step (2 x 6 doubles). advance() is
+ * refused while sensitivities are on; the wall audits are not accumulated. */
+int nf_enable_sensitivity(nf_solver* s);
+int nf_get_node_sensitivity(nf_solver* s, int32_t field, int32_t parameter, double* out);
+int nf_get_step_sensitivity(nf_solver* s, double* out);
+
 /* Batched ensemble (no single counterpart in the reference, which runs an ensemble as one
  * solver per thread: M0Cylinder.java:870-907, sweep/Sweep.java:77-106). A batch is made from
  * solvers already set up through the calls above (mesh, material, wall, velocities ...) and

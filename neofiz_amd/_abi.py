@@ -86,6 +86,9 @@ _PROTOTYPES = {
     "get_audits": (C.c_int, [_P, C.POINTER(AuditsC)]),
     "contact_pairs": (C.c_int, [_P, C.c_int32, c_int32_p, c_int32_p]),
     "contact_surface": (C.c_int, [_P, C.c_int32, c_int32_p, c_int32_p]),
+    "enable_sensitivity": (C.c_int, [_P]),
+    "get_node_sensitivity": (C.c_int, [_P, C.c_int32, C.c_int32, c_double_p]),
+    "get_step_sensitivity": (C.c_int, [_P, c_double_p]),
     "batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
     "batch_destroy": (C.c_int, [_P]),
     "batch_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32]),

@@ -1082,6 +1082,8 @@ int check_material(const nf_material* m) {
 
 // ------------------------------------------------------------------ the handle
 
+#include "tangent.h"
+
 struct nf_solver {
     hipStream_t stream = nullptr;
     Dev d{};
@@ -1100,6 +1102,8 @@ struct nf_solver {
     bool hasWall = false, hasDamage = false, hasErosion = false, hasContact = false;
     bool anyJC = false;
     bool hasKin = false;  // some material has kinematic hardening
+    bool tangent = false;  // parameter sensitivities are carried (nf_enable_sensitivity)
+    TDev td{};
     double wallPlane = 0.0, wallFriction = 0.0;
     // Optional per-kernel timing with HIP events on the handle's stream (nf_profile).
     bool profiling = false;
@@ -1267,7 +1271,25 @@ struct nf_solver {
 
     // One step on the stream: the element pass of the open step, then the launch that
     // gathers it and opens the next. The device decides whether either does anything.
+    int launchTangent(int mode, int phase, int gatherFirst) {
+        if (mode >= 0) {
+            hipLaunchKernelGGL(k_t_element, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, td, q, mode);
+            if (mode == MODE_STEP) hipLaunchKernelGGL(k_t_argmin, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, td, q);
+        }
+        if (phase >= 0) {
+            if (hasWall) hipLaunchKernelGGL(k_t_gn<true>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, td, q, phase, gatherFirst);
+            else hipLaunchKernelGGL(k_t_gn<false>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, td, q, phase, gatherFirst);
+        }
+        HIP_TRY(hipGetLastError());
+        return NF_OK;
+    }
+
     int enqueueStep() {
+        if (tangent) {
+            TRY(launchTangent(MODE_STEP, PHASE_STEP, 0));
+            q ^= 1;
+            return NF_OK;
+        }
         TRY(timed(1, [&](hipEvent_t a, hipEvent_t b) { return launchElement(MODE_STEP, q, a, b); }));
         TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_STEP, q, 0, a, b); }));
         q ^= 1;
@@ -1324,6 +1346,7 @@ struct nf_solver {
     // zero-interval assembly is followed by the first nodal update (PHASE_START with
     // gatherFirst); advance() ends with a bare gather.
     int assembleAdvance() {
+        if (tangent) return fail(NF_E_STATE, "advance() is not differentiated; sensitivities follow step() only");
         TRY(launchElement(MODE_ADVANCE, q));
         TRY(launchGN(PHASE_ASSEMBLE_END, q, 1));
         q ^= 1;
@@ -1341,7 +1364,9 @@ struct nf_solver {
         h.stepLimit = maxSteps > 0 ? start + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
         TRY(writeStep(h));
         // Open the first step of this call.
-        if (h.steps == 0) {
+        if (tangent) {
+            TRY(launchTangent(h.steps == 0 ? MODE_INITIAL : -1, PHASE_START, h.steps == 0 ? 1 : 0));
+        } else if (h.steps == 0) {
             TRY(launchElement(MODE_INITIAL, q));  // :945
             TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 1, a, b); }));
         } else {
@@ -2416,6 +2441,66 @@ int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int3
     return NF_OK;
 }
 
+
+// ------------------------------------------------------------------ sensitivities
+
+int nf_enable_sensitivity(nf_solver* s) {
+    g_error.clear();
+    if (s->tangent) return NF_OK;
+    Step h;
+    TRY(s->readStep(h));
+    if (h.steps != 0) return fail(NF_E_STATE, "sensitivities start from the initial state; enable them before the first step");
+    if (s->integration != NF_REDUCED || s->hasDamage || s->hasErosion || s->hasContact || s->edges ||
+        !s->materialOf.empty() || !s->anyJC || s->hasKin)
+        return fail(NF_E_STATE, "sensitivities cover reduced integration of one Johnson-Cook material without "
+                                "kinematic hardening, damage, erosion, contact or pressure edges");
+    TDev& t = s->td;
+    const size_t N = s->N, E = s->E, P = s->P;
+    TRY(s->alloc(&t.tu, NP * N));
+    TRY(s->alloc(&t.tv, NP * N));
+    TRY(s->alloc(&t.tf, NP * N));
+    TRY(s->alloc(&t.tslot, NP * 4 * E));
+    TRY(s->alloc(&t.tsig, NP * 4 * P));
+    TRY(s->alloc(&t.tep, NP * P));
+    TRY(s->alloc(&t.twork, NP * P));
+    TRY(s->alloc(&t.thg, NP * 2 * E));
+    TRY(s->alloc(&t.tedge, NP * E));
+    TRY(s->alloc(&t.edgeVal, E));
+    TRY(s->alloc(&t.tstep, 4 * NP));
+    TRY(s->alloc(&t.argmin, 3));
+    const int none[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0};
+    HIP_TRY(hipMemcpy(t.argmin, none, sizeof none, hipMemcpyHostToDevice));
+    s->tangent = true;
+    s->graphDirty = true;
+    return NF_OK;
+}
+
+int nf_get_node_sensitivity(nf_solver* s, int32_t field, int32_t param, double* out) {
+    g_error.clear();
+    if (!s->tangent) return fail(NF_E_STATE, "sensitivities are not enabled");
+    if (param < 0 || param >= NP) return fail(NF_E_ARGUMENT, "parameter out of range");
+    const double2* src = nullptr;
+    switch (field) {
+        case NF_NODE_UR: case NF_NODE_UZ: src = s->td.tu; break;
+        case NF_NODE_VR: case NF_NODE_VZ: src = s->td.tv; break;
+        case NF_NODE_FR: case NF_NODE_FZ: src = s->td.tf; break;
+        default: return fail(NF_E_ARGUMENT, "sensitivities exist for UR, UZ, VR, VZ, FR and FZ");
+    }
+    std::vector<double2> h(s->N);
+    HIP_TRY(hipMemcpyAsync(h.data(), src + (size_t)param * s->N, sizeof(double2) * s->N, hipMemcpyDeviceToHost, s->stream));
+    TRY(s->sync());
+    const bool r = field == NF_NODE_UR || field == NF_NODE_VR || field == NF_NODE_FR;
+    for (int i = 0; i < s->N; i++) out[i] = r ? h[i].x : h[i].y;
+    return NF_OK;
+}
+
+int nf_get_step_sensitivity(nf_solver* s, double* out) {
+    g_error.clear();
+    if (!s->tangent) return fail(NF_E_STATE, "sensitivities are not enabled");
+    TRY(s->sync());
+    HIP_TRY(hipMemcpy(out, s->td.tstep + (size_t)s->q * 2 * NP, sizeof(double) * 2 * NP, hipMemcpyDeviceToHost));
+    return NF_OK;
+}
 
 // ------------------------------------------------------------------ batched ensemble
 

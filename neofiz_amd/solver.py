@@ -669,6 +669,28 @@ class ExplicitSolver:
     def wallContactCount(self):
         return self.state().wall_contact_count
 
+    # Parameter sensitivities (nf_enable_sensitivity / nf_get_node_sensitivity): the
+    # derivative of the discrete step loop with respect to SENSITIVITY_PARAMETERS.
+    SENSITIVITY_PARAMETERS = ("friction", "A", "B", "n", "C", "m")
+
+    def enableSensitivity(self):
+        self._call("enable_sensitivity")
+
+    def nodeSensitivity(self, field, parameter):
+        """d(node field)/d(parameter) for UR, UZ, VR, VZ, FR or FZ; parameter is an index
+        into SENSITIVITY_PARAMETERS or its name."""
+        if isinstance(parameter, str):
+            parameter = self.SENSITIVITY_PARAMETERS.index(parameter)
+        out = np.empty(self.mesh.nodeCount)
+        self._call("get_node_sensitivity", _abi.NODE_FIELDS[field], parameter, _dptr(out))
+        return out
+
+    def stepSensitivity(self):
+        """(d timestep / d parameter, d previous timestep / d parameter), six each."""
+        out = np.empty(2 * len(self.SENSITIVITY_PARAMETERS))
+        self._call("get_step_sensitivity", _dptr(out))
+        return out[:6], out[6:]
+
     def contactSurface(self):
         """The contact's current surface segments (a, b), in surface order (element by
         element, mesh/QuadMesh.java:188-226, over the elements contact has not forgotten)."""
