@@ -127,6 +127,25 @@ def ensemble_leg(count, cap, per_size):
                 "ensemble's exact step counts"}
 
 
+def sensitivity_leg(nr, nz):
+    """Cost of the step loop's derivative: the Johnson-Cook Taylor rod on an nr x nz mesh to
+    80 us, once plain and once carrying its six parameter sensitivities (the same steps,
+    bit-identical trajectory), both timed on the GPU."""
+    from neofiz_amd import sensitivity as S, taylor
+    jc = taylor.ROD_JC.johnsonCook
+    pt = S.Point(nr, nz, taylor.ROD_SPEED, taylor.ROD_JC_FRICTION, jc.a, jc.b, jc.n, jc.c, jc.m)
+    print(f"# sensitivities: {nr}x{nz} rod to 80 us, forward then with derivatives", file=sys.stderr, flush=True)
+    obj, fwd_steps, fwd_s = S.forward(pt)
+    obj2, grad, steps, sens_s = S.gradients(pt)
+    return {"workload": f"taylor-rod-jc-{nr}x{nz}, 80 us", "steps": steps,
+            "forward_seconds": fwd_s, "with_derivative_seconds": sens_s,
+            "cost_ratio": sens_s / fwd_s, "same_trajectory": fwd_steps == steps and obj == obj2,
+            "objectives": {"L/L0": obj[0], "mushroom_diameter": obj[1]},
+            "gradient": {name: {"L/L0": grad[0, k], "mushroom_diameter": grad[1, k]}
+                         for k, name in enumerate(S.PARAMETERS)},
+            "mode": "forward (tangent) derivative in six directions"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +158,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-sensitivity", action="store_true")
     ap.add_argument("--ensemble", type=int, default=256, help="instances in the batched ensemble")
     ap.add_argument("--ensemble-cap", type=int, default=20000, help="step cap per instance")
     ap.add_argument("--ensemble-solo", type=int, default=4, help="one-by-one sample per mesh size")
@@ -223,6 +243,8 @@ def main():
         except Exception as exc:  # torch missing or no device: report, do not hide
             line["stream_copy_GBps"] = None
             line["stream_copy_error"] = str(exc)
+    if not args.no_sensitivity:
+        line["sensitivity"] = sensitivity_leg(32, 320)
     if not args.no_ensemble:
         line["batched"] = ensemble_leg(args.ensemble, args.ensemble_cap, args.ensemble_solo)
     if not args.no_cpu:
