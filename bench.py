@@ -146,6 +146,39 @@ def sensitivity_leg(nr, nz):
             "mode": "forward (tangent) derivative in six directions"}
 
 
+def perforation_leg(elements, warmup, steps):
+    """The multi-body perforation scene at about `elements` elements: `steps` steps timed
+    after `warmup` from t = 0 (the slug starts 1 mm from the wall and strikes it during the
+    warmup), with the per-launch split of element pass, node pass and contact."""
+    import ctypes as C
+    from neofiz_amd import perforation
+    cell = perforation.cell_for(elements)
+    print(f"# perforation: building {elements} elements", file=sys.stderr, flush=True)
+    mesh, s, _ = perforation.build(cell)
+    s.run(warmup)
+    t0 = time.perf_counter()
+    s.run(steps)
+    elapsed = time.perf_counter() - t0
+    st = s.state()
+    b = s._b
+    ms = (C.c_double * 3)()
+    counts = (C.c_int64 * 3)()
+    b.profile(s._h, 1)
+    b.profile_read(s._h, ms, counts)
+    s.run(200)
+    b.profile_read(s._h, ms, counts)
+    b.profile(s._h, 0)
+    s.shutdown()
+    per = [ms[k] / max(1, counts[k]) for k in range(3)]
+    return {"workload": f"perforation-{mesh.elementCount}", "elements": mesh.elementCount,
+            "nodes": mesh.nodeCount, "cell_m": cell, "steps_from_t0": [warmup, warmup + steps],
+            "value": mesh.elementCount * steps / elapsed, "unit": "element-updates/s",
+            "ms_per_step": elapsed / steps * 1e3,
+            "kernel_ms": {"node_pass": per[0], "element": per[1], "contact_launches": per[2]},
+            "contact_pairs": st.contact_pairs, "eroded_elements": st.eroded_elements,
+            "data": "synthetic scene (seeded Weibull failure strains)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +192,9 @@ def main():
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-sensitivity", action="store_true")
+    ap.add_argument("--no-perforation", action="store_true")
+    ap.add_argument("--perforation-steps", type=int, default=2000)
+    ap.add_argument("--perforation-warmup", type=int, default=1500)
     ap.add_argument("--ensemble", type=int, default=256, help="instances in the batched ensemble")
     ap.add_argument("--ensemble-cap", type=int, default=20000, help="step cap per instance")
     ap.add_argument("--ensemble-solo", type=int, default=4, help="one-by-one sample per mesh size")
@@ -243,6 +279,8 @@ def main():
         except Exception as exc:  # torch missing or no device: report, do not hide
             line["stream_copy_GBps"] = None
             line["stream_copy_error"] = str(exc)
+    if not args.no_perforation:
+        line["perforation"] = perforation_leg(200000, args.perforation_warmup, args.perforation_steps)
     if not args.no_sensitivity:
         line["sensitivity"] = sensitivity_leg(32, 320)
     if not args.no_ensemble:

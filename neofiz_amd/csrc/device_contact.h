@@ -2,12 +2,13 @@
 //
 // Included by neofiz_hip.hip after the Dev and Ctl definitions.
 //
-// Each assembly that gathers forces runs five launches after the gather and before the
+// Each assembly that gathers forces runs six launches after the gather and before the
 // nodal update (Contact.begin / search / accumulate, :424-462):
 //
-//   k_c_rebuild   only when an element was forgotten since the last rebuild (adopt,
-//                 :290-322): live flags of every potential segment and surface node, compact
-//                 lists of both, lastMaster cleared.
+//   k_c_rebuild, k_c_rebuild_nodes
+//                 only when an element was forgotten since the last rebuild (adopt,
+//                 :290-322): live flags of every potential segment, then of every surface
+//                 node, compact lists of both, lastMaster cleared.
 //   k_c_geometry  segment lengths and outward normals (:465-487), the grid's bounding box
 //                 and longest segment (exact min/max on order-preserving bit keys), and the
 //                 clearing of the hash heads and per-segment pair lists.
@@ -156,9 +157,17 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
             if (at < 4 * d.E) d.cAct[at] = p;
         }
     }
+}
+
+// The second half of the rebuild: surface nodes from the live flags just written.
+__global__ void __launch_bounds__(BLOCK) k_c_rebuild_nodes(Dev d, int q, int checkGo) {
+    if (c_skip(d, q, checkGo)) return;
+    Ctl* c = d.ctl;
+    if (!c->surfaceStale) return;
+    const int stride = gridDim.x * BLOCK;
     for (int n = blockIdx.x * BLOCK + threadIdx.x; n < d.N; n += stride) {
         bool on = false;
-        for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1] && !on; k++) on = seg_live(d, d.cNodeSegOf[k]);
+        for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1] && !on; k++) on = d.cActive[d.cNodeSegOf[k]] != 0;
         d.cSurfFlag[n] = on ? 1 : 0;
         d.cPairOk[n] = 0;
         d.cLast[n] = -1;
@@ -323,9 +332,12 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
                     const int p = x >> 1;
                     int a, b;
                     seg_ends(d, p, a, b);
-                    if (c_excluded(d, n, pn, p, a, b)) continue;
+                    // The reference tests the exclusions first, then the depth; both are
+                    // pure, and a candidate is kept only if it passes both, so testing the
+                    // cheaper depth first selects the same candidates.
                     const double dd = c_depth(d, xn, p, a, b);
                     if (dd <= 0.0) continue;
+                    if (c_excluded(d, n, pn, p, a, b)) continue;
                     if (dd > C_MAX_DEPTH * d.cLen[p]) {
                         if (p == last) fell = true;
                         continue;

@@ -314,6 +314,15 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
         return;
     }
     if (part == PART_REST && gathered && blockIdx.x == 0) reduce_contact_partials(d);
+    if (phase == PHASE_STEP && d.gone && blockIdx.x == 0) {
+        // eroded energy of step n: the element launch's per-block partials, summed by this
+        // workgroup in a fixed order (strided per thread, then the block tree)
+        __shared__ double ldsE[4 * 1];
+        double tot[1] = {0.0};
+        for (int b = threadIdx.x; b < d.elementBlocks; b += BLOCK) tot[0] += d.erodePartial[b];
+        block_sum<1>(tot, ldsE);
+        if (threadIdx.x == 0) c->erodedEnergy += tot[0];
+    }
     double dt = in.dt, dtPrev = in.dtPrev;
     bool next = true;
     if (phase == PHASE_STEP) {
@@ -338,12 +347,6 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
         o.nanEdge = 0;
         o.minEdgeBits = MAX_BITS;
         out = o;
-        if (phase == PHASE_STEP && d.gone) {
-            // eroded energy of step n, summed over the element launch's blocks in order
-            double sum = 0.0;
-            for (int b = 0; b < d.elementBlocks; b++) sum += d.erodePartial[b];
-            c->erodedEnergy += sum;
-        }
     }
 
     const int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -1249,7 +1252,8 @@ struct nf_solver {
         const int g4 = std::min(blocks_for(4 * E), CONTACT_GRID);
         const int gn = std::min(blocks_for(N), CONTACT_GRID);
         if (a) HIP_TRY(hipEventRecord(a, stream));
-        hipLaunchKernelGGL(k_c_rebuild, dim3(std::max(g4, gn)), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_rebuild, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(gn), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_geometry, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_grid, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_search, dim3(gn), dim3(BLOCK), 0, stream, d, parity, checkGo);
@@ -1835,6 +1839,8 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
     d.cScale = scale;
     d.cRef = s->dt0;
     hipLaunchKernelGGL(k_c_rebuild, dim3(std::min(blocks_for(4 * E), CONTACT_GRID)), dim3(BLOCK), 0,
+                       s->stream, d, s->q, 0);
+    hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(std::min(blocks_for(N), CONTACT_GRID)), dim3(BLOCK), 0,
                        s->stream, d, s->q, 0);
     HIP_TRY(hipGetLastError());
     // The lists are built: clear the flag so the first assembly does not append them again.

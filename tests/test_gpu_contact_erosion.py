@@ -221,3 +221,16 @@ def test_multibody_perforation_bitwise():
     g, r, seen = lockstep(mini_wall, 150, 20, damage=True, label="perforation")
     assert seen > 0
     assert r.eroded_elements > 0
+
+
+def test_stacked_wall_perforation_bitwise():
+    """The perforation scene (slug into twelve stacked blocks on a floor) at ~3,000 elements:
+    resting contact between the blocks from the first step, the slug's impact, erosion and
+    the surface rebuilds it forces, all bit-identical to the oracle."""
+    from neofiz_amd import perforation
+
+    def build(cls):
+        return perforation.build(perforation.cell_for(3000), cls)[1]
+    g, r, seen = lockstep(build, 30, 50, damage=True, label="stacked wall")
+    assert seen > 0
+    assert r.eroded_elements > 100
