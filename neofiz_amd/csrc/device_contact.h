@@ -127,6 +127,21 @@ __device__ __forceinline__ unsigned c_bucket(int i, int j, unsigned mask) {
     return h & mask;
 }
 
+// Capture-grid hash table size: 16 heads per live segment, within what was allocated.
+__device__ __forceinline__ unsigned c_capture_size(const Dev& d, int nAct) {
+    unsigned h = 64;
+    while (h < 16u * (unsigned)nAct && h < d.cFHashMax) h <<= 1;
+    return h;
+}
+
+// The capture-grid cell holding coordinate x (cells anchored at the grid origin).
+__device__ __forceinline__ int c_fine(double x, double origin, double h) {
+    const double f = __builtin_floor((x - origin) / h);
+    return f < -1.0e9 ? -1000000000 : (f > 1.0e9 ? 1000000000 : (int)f);
+}
+
+constexpr int C_MAX_CAPTURE_CELLS = 1 << 16;  // per segment; beyond it the step falls back
+
 __device__ __forceinline__ bool c_skip(const Dev& d, int q, int checkGo) {
     return checkGo && !d.ctl->s[q].go;
 }
@@ -146,6 +161,8 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
         c->cBox[2] = hi;  // keys of -rhi and -zhi: both start at -(-MAX_VALUE)
         c->cBox[3] = hi;
         c->cLongest = 0ULL;
+        c->cFCount = 0;
+        c->cFOverflow = 0;
     }
     if (!c->surfaceStale) return;
     const int stride = gridDim.x * BLOCK;
@@ -231,12 +248,14 @@ __global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo)
     if ((threadIdx.x & 63) == 0 && longest) atomicMax(&c->cLongest, longest);
     const unsigned H = c_hash_size(nAct);
     for (unsigned h = t0; h < H; h += stride) d.cHead[h] = -1;
+    const unsigned HF = c_capture_size(d, nAct);
+    for (unsigned h = t0; h < HF; h += stride) d.cFHead[h] = -1;
 }
 
 // Node normals :488-499 (the node's live segments in surface order) and the grid :534-551.
 __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
     if (c_skip(d, q, checkGo)) return;
-    const Ctl* c = d.ctl;
+    Ctl* c = d.ctl;
     const int nAct = c->cNAct, nSurf = c->cNSurf;
     const int stride = gridDim.x * BLOCK;
     const int t0 = blockIdx.x * BLOCK + threadIdx.x;
@@ -265,6 +284,40 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
         const int entry = 2 * p + (t & 1);
         d.cEntCell[entry] = make_int2(i, j);
         d.cNext[entry] = atomicExch(&d.cHead[c_bucket(i, j, g.mask)], entry);
+    }
+    // The capture grid. A segment can be a node's master only if the node lies within
+    // its capture rectangle: projection along it in [0, len], depth below it in (0, len / 2]
+    // (depth() :639-649 with the MAX_DEPTH cut). That rectangle lies inside the segment's
+    // bounding box grown by len / 2 (a hair more, for rounding), and the segment is listed
+    // in every cell that box overlaps, so the node's own cell lists every candidate.
+    const unsigned fmask = c_capture_size(d, nAct) - 1u;
+    for (int t = t0; t < nAct; t += stride) {
+        const int p = d.cAct[t];
+        const double len = d.cLen[p];
+        if (!(len > 0.0) || len > NFD_MAX_VALUE) continue;  // such a segment never wins
+        int a, b;
+        seg_ends(d, p, a, b);
+        const double2 xa = cpos(d, a), xb = cpos(d, b);
+        const double m = 0.5 * len * (1.0 + 1.0e-6) + 1.0e-12;
+        const int i0 = c_fine(nfd::jmin(xa.x, xb.x) - m, g.r0, d.cFine);
+        const int i1 = c_fine(nfd::jmax(xa.x, xb.x) + m, g.r0, d.cFine);
+        const int j0 = c_fine(nfd::jmin(xa.y, xb.y) - m, g.z0, d.cFine);
+        const int j1 = c_fine(nfd::jmax(xa.y, xb.y) + m, g.z0, d.cFine);
+        if ((long long)(i1 - i0 + 1) * (j1 - j0 + 1) > C_MAX_CAPTURE_CELLS) {
+            atomicOr(&c->cFOverflow, 1);
+            continue;
+        }
+        for (int j = j0; j <= j1; j++)
+            for (int i = i0; i <= i1; i++) {
+                const int at = atomicAdd(&c->cFCount, 1);
+                if (at >= d.cFCap) {
+                    atomicOr(&c->cFOverflow, 1);
+                    continue;
+                }
+                d.cFSeg[at] = p;
+                d.cFCell[at] = make_int2(i, j);
+                d.cFNext[at] = atomicExch(&d.cFHead[c_bucket(i, j, fmask)], at);
+            }
     }
 }
 
@@ -305,6 +358,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
     const int nAct = c->cNAct, nSurf = c->cNSurf;
     if (nAct == 0) return;
     const CGrid g = c_grid(d);
+    const unsigned fmask = c_capture_size(d, nAct) - 1u;
     const double dt = c->s[q].dt;
     const int stride = gridDim.x * BLOCK;
     long long escaped = 0;
@@ -319,34 +373,77 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
         double deepest = 0.0;
         unsigned long long bestKey = ~0ULL;
         bool fell = false;
-        for (int dj = -1; dj <= 1; dj++) {
-            const int j = cj + dj;
-            if (j < 0 || j >= g.rows) continue;
-            for (int di = -1; di <= 1; di++) {
-                const int i = ci + di;
-                if (i < 0 || i >= g.cols) continue;
-                const unsigned long long rank = (unsigned long long)((dj + 1) * 3 + (di + 1)) << 40;
-                for (int x = d.cHead[c_bucket(i, j, g.mask)]; x >= 0; x = d.cNext[x]) {
-                    const int2 cell = d.cEntCell[x];
-                    if (cell.x != i || cell.y != j) continue;
-                    const int p = x >> 1;
-                    int a, b;
-                    seg_ends(d, p, a, b);
-                    // The reference tests the exclusions first, then the depth; both are
-                    // pure, and a candidate is kept only if it passes both, so testing the
-                    // cheaper depth first selects the same candidates.
-                    const double dd = c_depth(d, xn, p, a, b);
-                    if (dd <= 0.0) continue;
-                    if (c_excluded(d, n, pn, p, a, b)) continue;
-                    if (dd > C_MAX_DEPTH * d.cLen[p]) {
-                        if (p == last) fell = true;
-                        continue;
-                    }
-                    const unsigned long long key = rank | (unsigned long long)x;
-                    if (dd > deepest || (dd == deepest && key < bestKey)) {
-                        deepest = dd;
-                        best = p;
-                        bestKey = key;
+        // One candidate as the reference's loop body treats it (:585-602). The reference
+        // tests the exclusions first, then the depth; both are pure, and a candidate is kept
+        // only if it passes both, so testing the cheaper depth first selects the same ones.
+        auto consider = [&](int p, int a, int b, unsigned long long key) {
+            const double dd = c_depth(d, xn, p, a, b);
+            if (dd <= 0.0) return;
+            if (c_excluded(d, n, pn, p, a, b)) return;
+            if (dd > C_MAX_DEPTH * d.cLen[p]) {
+                if (p == last) fell = true;
+                return;
+            }
+            if (dd > deepest || (dd == deepest && key < bestKey)) {
+                deepest = dd;
+                best = p;
+                bestKey = key;
+            }
+        };
+        // Where segment p's first entry falls in the reference's walk over the nine cells
+        // around (ci, cj): (cell rank, entry index 2p + end), or ~0 if neither end is there.
+        auto walkKey = [&](int p, int a, int b) {
+            unsigned long long key = ~0ULL;
+            const int ends[2] = {a, b};
+            for (int e = 0; e < 2; e++) {
+                const double2 x = cpos(d, ends[e]);
+                const int di = clampi(jint((x.x - g.r0) / g.cell), g.cols) - ci;
+                const int dj = clampi(jint((x.y - g.z0) / g.cell), g.rows) - cj;
+                if (di < -1 || di > 1 || dj < -1 || dj > 1) continue;
+                const unsigned long long k =
+                    ((unsigned long long)((dj + 1) * 3 + (di + 1)) << 40) | (unsigned long long)(2 * p + e);
+                key = k < key ? k : key;
+            }
+            return key;
+        };
+        if (!c->cFOverflow) {
+            // The node's capture cell lists every segment that can be its master; each is
+            // considered only if the reference's walk reaches it, at its walk position.
+            const int fi = c_fine(xn.x, g.r0, d.cFine), fj = c_fine(xn.y, g.z0, d.cFine);
+            for (int x = d.cFHead[c_bucket(fi, fj, fmask)]; x >= 0; x = d.cFNext[x]) {
+                const int2 cell = d.cFCell[x];
+                if (cell.x != fi || cell.y != fj) continue;
+                const int p = d.cFSeg[x];
+                int a, b;
+                seg_ends(d, p, a, b);
+                const unsigned long long key = walkKey(p, a, b);
+                if (key != ~0ULL) consider(p, a, b, key);
+            }
+            // The previous master may have gone too deep to be captured: the drop test.
+            if (last >= 0 && !fell) {
+                int a, b;
+                seg_ends(d, last, a, b);
+                if (walkKey(last, a, b) != ~0ULL) {
+                    const double dd = c_depth(d, xn, last, a, b);
+                    if (dd > 0.0 && dd > C_MAX_DEPTH * d.cLen[last] && !c_excluded(d, n, pn, last, a, b)) fell = true;
+                }
+            }
+        } else {
+            // The reference's own walk over the nine cells.
+            for (int dj = -1; dj <= 1; dj++) {
+                const int j = cj + dj;
+                if (j < 0 || j >= g.rows) continue;
+                for (int di = -1; di <= 1; di++) {
+                    const int i = ci + di;
+                    if (i < 0 || i >= g.cols) continue;
+                    const unsigned long long rank = (unsigned long long)((dj + 1) * 3 + (di + 1)) << 40;
+                    for (int x = d.cHead[c_bucket(i, j, g.mask)]; x >= 0; x = d.cNext[x]) {
+                        const int2 cell = d.cEntCell[x];
+                        if (cell.x != i || cell.y != j) continue;
+                        const int p = x >> 1;
+                        int a, b;
+                        seg_ends(d, p, a, b);
+                        consider(p, a, b, rank | (unsigned long long)x);
                     }
                 }
             }

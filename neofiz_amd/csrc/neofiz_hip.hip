@@ -88,6 +88,7 @@ struct Ctl {
     int cPairs, cOverflow;
     double cEnergy, cDissipation, cMaxPen;
     long long cEscaped;
+    int cFCount, cFOverflow;  // entries of the capture grid this assembly; it overflowed
 };
 
 constexpr unsigned long long MAX_BITS = 0x7FEFFFFFFFFFFFFFULL;  // Double.MAX_VALUE
@@ -160,6 +161,15 @@ struct Dev {
     int* cPairsPartial;      // [cApplyBlocks]
     int cApplyBlocks;
     double cFriction, cDamping, cScale, cRef;
+    // The capture grid: square cells of side cFine, each listing every segment whose capture
+    // rectangle (the region where it can be a node's master) overlaps the cell.
+    int* cFHead;             // [cFHashMax] hash heads
+    int* cFNext;             // [cFCap]
+    int* cFSeg;              // [cFCap] segment p of the entry
+    int2* cFCell;            // [cFCap] its cell
+    int cFCap;
+    unsigned cFHashMax;
+    double cFine;
 };
 
 __device__ __forceinline__ double pressureAt(const Dev& d, double t) {  // :1203-1207
@@ -1807,6 +1817,31 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
         TRY(s->alloc(&d.cPairHead, 4 * (size_t)E));
         TRY(s->alloc(&d.cPairNext, N));
         TRY(s->alloc(&d.cPair, (size_t)CPAIR * N));
+        // Capture grid: entries for four cells per segment on average, at least 2^20.
+        d.cFCap = (int)std::min<size_t>(0x7FFFFFFF, std::max<size_t>(1u << 20, 16 * (size_t)E));
+        d.cFHashMax = 64;
+        while (d.cFHashMax < 16u * (unsigned)E && d.cFHashMax < (1u << 28)) d.cFHashMax <<= 1;
+        TRY(s->alloc(&d.cFHead, d.cFHashMax));
+        TRY(s->alloc(&d.cFNext, (size_t)d.cFCap));
+        TRY(s->alloc(&d.cFSeg, (size_t)d.cFCap));
+        TRY(s->alloc(&d.cFCell, (size_t)d.cFCap));
+        // Capture cells twice the median surface segment of the reference mesh.
+        {
+            std::vector<double> lens;
+            for (int p = 0; p < 4 * E; p++) {
+                if (shareStart[p + 1] != shareStart[p]) continue;  // interior edge
+                const int a = s->conn[p], b = s->conn[(p & ~3) + ((p + 1) & 3)];
+                const double dr = s->r[b] - s->r[a], dz = s->z[b] - s->z[a];
+                const double l = std::sqrt(dr * dr + dz * dz);
+                if (l > 0.0) lens.push_back(l);
+            }
+            if (lens.empty()) {
+                d.cFine = 1.0;
+            } else {
+                std::nth_element(lens.begin(), lens.begin() + lens.size() / 2, lens.end());
+                d.cFine = 2.0 * lens[lens.size() / 2];
+            }
+        }
         TRY(s->alloc(&d.cPartial, 3 * (size_t)d.cApplyBlocks));
         TRY(s->alloc(&d.cPairsPartial, d.cApplyBlocks));
         HIP_TRY(hipMemcpy(shareStartD, shareStart.data(), sizeof(int) * shareStart.size(), hipMemcpyHostToDevice));
