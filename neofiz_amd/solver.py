@@ -685,6 +685,23 @@ class ExplicitSolver:
         self._call("get_node_sensitivity", _abi.NODE_FIELDS[field], parameter, _dptr(out))
         return out
 
+    def centroidStress(self, e):
+        """Mean stress over an element's Gauss points {rr, zz, tt, rz} (:1886-1894)."""
+        ppe = self.state().points_per_element
+        sig = self.pointField("stress").reshape(-1, 4)
+        out = np.zeros(4)
+        for g in range(ppe):
+            out += sig[e * ppe + g]
+        return out / ppe
+
+    def elementPlasticStrain(self, e):  # :1897-1901
+        ppe = self.state().points_per_element
+        ep = self.pointField("plastic_strain")
+        total = 0.0
+        for g in range(ppe):
+            total += ep[e * ppe + g]
+        return total / ppe
+
     def stepSensitivity(self):
         """(d timestep / d parameter, d previous timestep / d parameter), six each."""
         out = np.empty(2 * len(self.SENSITIVITY_PARAMETERS))
