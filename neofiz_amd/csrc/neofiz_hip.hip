@@ -1150,10 +1150,13 @@ struct nf_solver {
     }
 
     // Runs the launch `fn(start, stop)`, with a fresh event pair when profiling.
+    // Events are handed out from a counter, so a timed launch sequence may contain another.
+    int evUsed = 0;
     template <typename F>
     int timed(int kernel, F fn) {
         if (!profiling) return fn(nullptr, nullptr);
-        const int base = 2 * (int)evPending.size();
+        const int base = evUsed;
+        evUsed += 2;
         hipEvent_t a, b;
         TRY(event(base, &a));
         TRY(event(base + 1, &b));
@@ -1172,6 +1175,7 @@ struct nf_solver {
             kernelCount[p.first] += 1;
         }
         evPending.clear();
+        evUsed = 0;
         return NF_OK;
     }
 
