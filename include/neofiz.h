@@ -226,6 +226,88 @@ int nf_batch_get_tile(nf_batch* b, int32_t instance, int32_t* threads);
 int nf_batch_run_until(nf_batch* b, double end_time, int64_t max_steps);
 int nf_batch_scatter(nf_batch* b);
 
+/* ---- Defect-field seeding (core/DefectField.java, core/Weibull.java, core/Normal.java) ----
+ *
+ * new DefectField(new Weibull(scale, modulus), referenceVolume, correlationLength, seed)
+ * (core/DefectField.java:123-142, core/Weibull.java:53-69): a spatially correlated standard
+ * normal field (stateless SplitMix64 white noise on a grid of correlationLength/3, smoothed by
+ * a separable Gaussian) mapped through the normal cdf onto the Weibull that an element's
+ * effective volume earns it. Every value is a function of (seed, position) alone, so two
+ * meshes of one body see the same field bit for bit. The handle owns a HIP stream and the
+ * scratch it sizes on demand (the white-noise grid over the mesh's bounding box). */
+typedef struct nf_defect_params {
+    double scale;              /* Weibull scale at the reference volume */
+    double modulus;            /* Weibull modulus m */
+    double reference_volume;   /* V0, m^3 */
+    double correlation_length; /* l, m */
+    int64_t seed;
+} nf_defect_params;
+
+typedef struct nf_defect nf_defect;
+
+/* The constructor's checks, in Java's order: Weibull scale and modulus positive
+ * (core/Weibull.java:62-69), then referenceVolume and correlationLength positive
+ * (core/DefectField.java:124-131); NaN fails each. A non-finite correlationLength, which the
+ * reference accepts and degenerates on, is refused. */
+int nf_defect_create(nf_defect** out, const nf_defect_params* params);
+int nf_defect_destroy(nf_defect* f);
+
+/* Effective-volume rule of a sampled element. RING is DefectField.effectiveVolume
+ * (:204-208), the rule of sampleOnto (:239-248): l^3 max(1, A/l^2) max(1, 2 pi rc/l).
+ * SLAB has no counterpart in the reference; it is the plane-strain analogue, the same
+ * in-plane count times the number of correlation lengths through the depth:
+ * l^3 max(1, A/l^2) max(1, depth/l), passed through failureStrainForVolume (:226-228). */
+enum nf_defect_volume { NF_DEFECT_RING = 0, NF_DEFECT_SLAB = 1 };
+
+/* Flag: hash every white-noise tap instead of reading the memoised grid (the reference's
+ * path when the grid would exceed its cache, :365-366). The values are identical. */
+#define NF_DEFECT_UNCACHED 1
+
+/* sampleOnto(QuadMesh) :239-248, for seed_count fields that differ only in their seed
+ * (seeds NULL: the handle's own seed, seed_count 1). Element centroids are area centroids
+ * and areas signed areas of the reference configuration (mesh/QuadMesh.java:78-118). out
+ * receives seed_count x element_count failure strains, seed-major. Host arrays in and out. */
+int nf_defect_sample_onto(nf_defect* f, int32_t seed_count, const int64_t* seeds,
+                          int32_t node_count, const double* r, const double* z,
+                          int32_t element_count, const int32_t* conn, int32_t volume_rule,
+                          double depth, int32_t flags, double* out);
+
+/* The same with the mesh and the output already in device memory (r, z, conn, out device
+ * pointers; seeds a host array). Runs on the handle's stream and returns when it is done. */
+int nf_defect_sample_onto_device(nf_defect* f, int32_t seed_count, const int64_t* seeds,
+                                 int32_t node_count, const double* r, const double* z,
+                                 int32_t element_count, const int32_t* conn,
+                                 int32_t volume_rule, double depth, int32_t flags, double* out);
+
+/* Point evaluations at n points (host arrays):
+ *   GAUSSIAN          gaussianAt(r, z) :172-174                      (w unused)
+ *   STRAIN_AREA       failureStrainAt(r, z, crossSectionArea = w) :217-219
+ *   STRAIN_VOLUME     failureStrainForVolume(r, z, volume = w) :226-228
+ *   EFFECTIVE_VOLUME  effectiveVolume(centroidRadius = r, crossSectionArea = w) :204-208
+ *   TRANSFORM         transform(g = r, volume = w) :261-271 (private in the reference)
+ *   WHITE             white(i = r, j = z) :325-332 (private in the reference; r and z must
+ *                     be integers in int32 range)
+ * A volume that is not positive, or one whose weakest-link scale underflows, fails with
+ * NF_E_ARGUMENT as Weibull.forVolume / minimumOf / the Weibull record do. */
+enum nf_defect_function {
+    NF_DEFECT_GAUSSIAN = 0,
+    NF_DEFECT_STRAIN_AREA = 1,
+    NF_DEFECT_STRAIN_VOLUME = 2,
+    NF_DEFECT_EFFECTIVE_VOLUME = 3,
+    NF_DEFECT_TRANSFORM = 4,
+    NF_DEFECT_WHITE = 5
+};
+int nf_defect_points(nf_defect* f, int32_t function, int32_t n, const double* r,
+                     const double* z, const double* w, double* out);
+
+/* Device milliseconds of the last sampling call's kernels, measured with HIP events on the
+ * handle's stream: {bounding box, white-noise grid, smoothing and transform}. */
+int nf_defect_timing(nf_defect* f, double* ms);
+
+/* Normal.erfc (0), erf (1), cdf (2), survival (3) (core/Normal.java:39-89), Cody's rational
+ * approximations, evaluated on the device at n points. */
+int nf_normal_eval(int32_t function, int32_t n, const double* x, double* out);
+
 /* Per-kernel timing of the step loop (no counterpart in the reference, which times whole
  * runs with System.nanoTime, validate/TaylorImpactCase.java:254-260). When enabled, each
  * launch is bracketed by HIP events on the handle's stream; nf_profile_read returns summed
@@ -251,9 +333,10 @@ int nf_point_path(const nf_material* material, int32_t formulation, int32_t path
 int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_copy);
 
 /* Device self-check: evaluates the GPU's log (0), exp (1), pow (2), atan2 (3), sin (4),
- * cos (5), hypot (6), sqrt (7) or division (8) on n inputs. These are the functions the
- * kernels use in place of java.lang.Math's (core/JohnsonCook.java:139, :182, :196;
- * solver/Damage.java:315-322), evaluated where the kernels run. */
+ * cos (5), hypot (6), sqrt (7), division (8) or log1p (9) on n inputs. These are the
+ * functions the kernels use in place of java.lang.Math's (core/JohnsonCook.java:139, :182,
+ * :196; solver/Damage.java:315-322; core/Weibull.java:118, :130; core/DefectField.java:331),
+ * evaluated where the kernels run. */
 int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n);
 
 #ifdef __cplusplus

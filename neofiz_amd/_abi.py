@@ -45,6 +45,11 @@ class AuditsC(C.Structure):
         "damaged_fraction", "failed_fraction")]
 
 
+class DefectParamsC(C.Structure):
+    _fields_ = [("scale", C.c_double), ("modulus", C.c_double), ("reference_volume", C.c_double),
+                ("correlation_length", C.c_double), ("seed", C.c_int64)]
+
+
 NODE_FIELDS = {"ur": 0, "uz": 1, "vr": 2, "vz": 3, "fr": 4, "fz": 5, "mass": 6, "r": 7,
                "z": 8, "on_wall": 9, "fixed_r": 10, "fixed_z": 11}
 POINT_FIELDS = {"stress": 0, "back": 1, "plastic_strain": 2, "plastic_work": 3,
@@ -101,6 +106,18 @@ _PROTOTYPES = {
                              c_double_p, c_double_p]),
     "copy_bandwidth": (C.c_int, [C.c_int64, C.c_int32, c_double_p, c_double_p]),
     "profile_read": (C.c_int, [_P, c_double_p, C.POINTER(C.c_int64)]),
+    "defect_create": (C.c_int, [C.POINTER(_P), C.POINTER(DefectParamsC)]),
+    "defect_destroy": (C.c_int, [_P]),
+    "defect_sample_onto": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.c_int32, c_double_p,
+                                     c_double_p, C.c_int32, c_int32_p, C.c_int32, C.c_double,
+                                     C.c_int32, c_double_p]),
+    "defect_sample_onto_device": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.c_int32, _P,
+                                            _P, C.c_int32, _P, C.c_int32, C.c_double, C.c_int32,
+                                            _P]),
+    "defect_points": (C.c_int, [_P, C.c_int32, C.c_int32, c_double_p, c_double_p, c_double_p,
+                                c_double_p]),
+    "defect_timing": (C.c_int, [_P, c_double_p]),
+    "normal_eval": (C.c_int, [C.c_int32, C.c_int32, c_double_p, c_double_p]),
 }
 
 

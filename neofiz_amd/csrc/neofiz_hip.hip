@@ -1004,6 +1004,7 @@ __global__ void k_math_eval(int fn, const double* x, const double* y, double* ou
         case 6: r = nfm::hypot(x[i], y[i]); break;
         case 7: r = __builtin_sqrt(x[i]); break;
         case 8: r = x[i] / y[i]; break;
+        case 9: r = nfm::log1p(x[i]); break;
         default: r = 0.0;
     }
     out[i] = r;
@@ -1433,6 +1434,7 @@ struct nf_solver {
 };
 
 #include "batch.h"
+#include "defect_field.h"
 
 // ------------------------------------------------------------------ C ABI
 
@@ -2465,11 +2467,11 @@ int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_
 // Evaluates the device implementations of the shared transcendental functions (and the
 // device's division and square root) on n inputs: a self-check that the GPU computes exactly
 // what the host computes. fn: 0 log, 1 exp, 2 pow, 3 atan2(x, y), 4 sin, 5 cos, 6 hypot,
-// 7 sqrt, 8 x / y.
+// 7 sqrt, 8 x / y, 9 log1p.
 int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n) {
     g_error.clear();
     if (n <= 0) return NF_OK;
-    if (fn < 0 || fn > 8) return fail(NF_E_ARGUMENT, "unknown function");
+    if (fn < 0 || fn > 9) return fail(NF_E_ARGUMENT, "unknown function");
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;
     TRY(dev_alloc(&dx, n));
     TRY(dev_alloc(&dy, n));

@@ -17,6 +17,8 @@
 //    degree 8 on |r| <= 2^-7.5), double-double recombination with Knuth's two-sum and an
 //    exact product by Veltkamp splitting. No division: the dependency chain is short, which
 //    is what the Johnson-Cook Newton loop waits on.
+//  - log1p: log of the rounded 1 + x as a double-double, plus the two-sum error of 1 + x
+//    divided by it.
 //  - exp: table of 2^(j/64) and Cody-Waite reduction by ln2/64 (split constant), Taylor
 //    series of expm1 to degree 7 on |r| <= ln2/128.
 //  - pow(x, y) for x > 0: exp(y log x) with log x carried as a double-double.
@@ -145,6 +147,24 @@ NF_HD double log(double x, const Tables& T) {
 }
 
 NF_HD double log(double x) { return log(x, Tables{NFM_LOG_TBL, NFM_EXP_TBL}); }
+
+// log(1 + x). For |x| >= 2^-20: u = 1 + x rounds, with an exact error c (Knuth's two-sum), so
+// log(1 + x) = log(u) + log(1 + c/u) = log(u) + c/u, the dropped terms and the rounding of
+// c/u below 2^-80 of the result; log u is the double-double above. For |x| < 2^-20 the series
+// x - x^2/2 + x^3/3 - x^4/4, whose truncation is below 2^-80 relative; x*x underflows
+// harmlessly, and a zero keeps its sign.
+NF_HD double log1p(double x) {
+    double u, c;
+    two_sum(1.0, x, u, c);
+    const bool ok = u > 0.0 && u < inf();
+    double hi, lo;
+    log_dd(ok ? u : 1.0, hi, lo, NFM_LOG_TBL);
+    const double v = hi + (lo + c / u);
+    const double series = x + (x * x) * (-0.5 + x * (1.0 / 3.0 + x * -0.25));
+    const double special = is_nan(x) ? x : (u == 0.0 ? -inf() : (u == inf() ? u : qnan()));
+    const double ax = x < 0.0 ? -x : x;
+    return ax < 0x1p-20 ? series : (ok ? v : special);
+}
 
 // exp(a + b) with |b| <= ulp(a). Table method: a + b = (64 k + j) ln2/64 + r, |r| <= ln2/128;
 // exp = 2^k 2^(j/64) (1 + r + r^2 (1/2 + r/6 + ... + r^5/5040)), the polynomial by Estrin's
@@ -298,12 +318,17 @@ NF_HD double cos_kernel(double r) {
     return w + (((1.0 - w) - hz) + (z * z) * p);
 }
 
-// Quadrant reduction for |x| up to a few thousand (the solver only needs |x| <= pi/2).
+// Quadrant reduction for |x| up to a few thousand. k (pi/2)_hi = p + e exactly (two_prod),
+// and x - p is exact (Sterbenz: p lies within a factor of two of x whenever k != 0), so
+// r = ((x - p) - e) - k (pi/2)_lo carries the reduced argument to about 2^-100 absolute. For
+// |k| <= 1 the product is exact (e = 0) and r is (x - k hi) - k lo.
 NF_HD void reduce_pi_2(double x, double& r, int& q) {
     const double shifter = 0x1.8p52;
     const double kd = (x * 0.6366197723675814 + shifter) - shifter;  // 2/pi
     q = (int)kd;
-    r = (x - kd * NFM_PI_2_HI) - kd * NFM_PI_2_LO;
+    double p, e;
+    two_prod(kd, NFM_PI_2_HI, p, e);
+    r = ((x - p) - e) - kd * NFM_PI_2_LO;
 }
 
 NF_HD double sin(double x) {

@@ -300,7 +300,16 @@ class QuadMesh:
         for k in range(4):
             i, j = c[k], c[(k + 1) % 4]
             a += self.r[i] * self.z[j] - self.r[j] * self.z[i]
-        return 0.5 * a
+        return float(0.5 * a)
+
+    def ringVolume(self, e):  # :132-134, Pappus: 2 pi rc A
+        return 2.0 * math.pi * self.centroidRadius(e) * self.signedArea(e)
+
+    def totalRingVolume(self):  # :137-141, summed in element order
+        v = 0.0
+        for e in range(self.elementCount):
+            v += self.ringVolume(e)
+        return v
 
 
 @dataclass(frozen=True)

@@ -1390,6 +1390,9 @@ struct nfo_solver {
 };
 
 // ------------------------------------------------------------------ C ABI (prefix nfo_)
+// The error channel of the oracle's other translation units (defect_field.cpp).
+int nfo_report(int code, const std::string& msg) { return fail(code, msg); }
+
 extern "C" {
 
 const char* nfo_last_error(void) { return g_error.c_str(); }
@@ -1987,6 +1990,7 @@ extern "C" int nfo_math_eval(int32_t fn, const double* x, const double* y, doubl
             case 6: r = nfm::hypot(x[i], y[i]); break;
             case 7: r = std::sqrt(x[i]); break;
             case 8: r = x[i] / y[i]; break;
+            case 9: r = nfm::log1p(x[i]); break;
             default: return fail(NF_E_ARGUMENT, "unknown function");
         }
         out[i] = r;
