@@ -26,15 +26,16 @@ LENGTH = 1e-3
 
 
 def jittered(nr, nz, seed):
-    """A graded cylinder wall whose interior nodes are moved by up to a fifth of a cell."""
+    """A cylinder wall whose interior nodes are moved by up to a fifth of a cell, then graded
+    radially by r = 0.05 + 0.05 s^1.5 (a monotone map, so every element keeps its orientation)."""
     m = QuadMesh.cylinderWall(0.05, 0.10, 0.02, nr, nz)
     rng = np.random.default_rng(seed)
-    r = 0.05 + 0.05 * ((m.r - 0.05) / 0.05) ** 1.5
+    s = (m.r - 0.05) / 0.05
     z = m.z.copy()
-    inner = (m.r > 0.05) & (m.r < 0.10) & (m.z > 0.0) & (m.z < 0.02)
-    r[inner] += rng.uniform(-0.2, 0.2, inner.sum()) * (0.05 / nr)
+    inner = (s > 0.0) & (s < 1.0) & (m.z > 0.0) & (m.z < 0.02)
+    s[inner] += rng.uniform(-0.2, 0.2, inner.sum()) / nr
     z[inner] += rng.uniform(-0.2, 0.2, inner.sum()) * (0.02 / nz)
-    return QuadMesh(r, z, m.conn)
+    return QuadMesh(0.05 + 0.05 * s ** 1.5, z, m.conn)
 
 
 MESHES = {
@@ -107,22 +108,56 @@ def test_normal_matches_oracle_and_published_values(oracle):
         assert abs(Normal.erfc(xv) - expected) <= abs(expected) * 1e-13
 
 
+class DeviceArrays:
+    """Device buffers from the HIP runtime the library itself links (torch ships a runtime of
+    its own, which cannot open the GPU once this one has)."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.held = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.C.c_void_p()
+        assert self.hip.hipMalloc(self.C.byref(p), max(a.nbytes, 16)) == 0
+        self.held.append(p)
+        assert self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0   # host to device
+        return p.value
+
+    def get(self, ptr, shape, dtype=np.float64):
+        out = np.empty(shape, dtype=dtype)
+        assert self.hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        for p in self.held:
+            self.hip.hipFree(p)
+
+
 def test_device_pointer_entry(fields):
-    import torch
     mesh = MESHES["jittered"]()
     dev, ref = fields(11)
     seeds = np.array([11, 12], dtype=np.int64)
-    r = torch.from_numpy(mesh.r).cuda()
-    z = torch.from_numpy(mesh.z).cuda()
-    conn = torch.from_numpy(mesh.conn.astype(np.int32)).cuda()
-    out = torch.empty((2, mesh.elementCount), dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    dev.sampleOntoDevice(mesh.nodeCount, r.data_ptr(), z.data_ptr(), mesh.elementCount, conn.data_ptr(),
-                         out.data_ptr(), seeds=seeds)
-    got = out.cpu().numpy()
-    assert same_bits(got, ref.sampleOnto(mesh, seeds=seeds))
-    ms = dev.timing()
-    assert len(ms) == 3 and all(v >= 0.0 for v in ms) and ms[2] > 0.0
+    mem = DeviceArrays()
+    try:
+        r, z, conn = mem.put(mesh.r), mem.put(mesh.z), mem.put(mesh.conn.astype(np.int32))
+        out = mem.put(np.zeros((2, mesh.elementCount)))
+        dev.sampleOntoDevice(mesh.nodeCount, r, z, mesh.elementCount, conn, out, seeds=seeds)
+        got = mem.get(out, (2, mesh.elementCount))
+        assert same_bits(got, ref.sampleOnto(mesh, seeds=seeds))
+        ms = dev.timing()
+        assert len(ms) == 3 and all(v >= 0.0 for v in ms) and ms[2] > 0.0
+        bad = mesh.conn.astype(np.int32).copy()
+        bad[7] = -1
+        with pytest.raises(IllegalArgumentException):   # checked on the device before any read
+            dev.sampleOntoDevice(mesh.nodeCount, r, z, mesh.elementCount, mem.put(bad), out)
+    finally:
+        mem.free()
 
 
 def test_errors_are_the_references(fields):

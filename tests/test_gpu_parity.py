@@ -53,6 +53,8 @@ def rel(a, b):
     (6, -2.0, 2.0, (-2.0, 2.0)),   # hypot
     (7, -30.0, 30.0, None),     # sqrt of exp(U)
     (8, -3.0, 3.0, (-3.0, 3.0)),   # division
+    (9, -1.0, 0.5, None),       # log1p
+    (5, 0.0, 6.3, None),        # cos over the whole turn (the white noise's angle)
 ])
 def test_device_math_matches_host(hip, oracle, fn, lo, hi, second):
     rng = np.random.default_rng(100 + fn)
