@@ -20,6 +20,12 @@ Extra objects on the line:
                 the committed rocprofv3 PMC summary.
   cpu_baseline  the oracle (the C++ restatement, threaded like the reference's Parallel
                 pool) on the host's cores over a bounded sample of the same workload.
+  defect_field  device-side defect-field seeding of the ~200k-element perforation assembly
+                (kernel time per call, inputs resident) and the size-effect population of
+                3000 seeds, with the serial oracle on the same mesh beside it.
+  perforation, sensitivity, batched
+                the multi-body contact scene, the derivative's cost, the 256-instance
+                ensemble against one-by-one runs.
 """
 import argparse
 import json
@@ -176,7 +182,72 @@ def perforation_leg(elements, warmup, steps):
             "ms_per_step": elapsed / steps * 1e3,
             "kernel_ms": {"node_pass": per[0], "element": per[1], "contact_launches": per[2]},
             "contact_pairs": st.contact_pairs, "eroded_elements": st.eroded_elements,
-            "data": "synthetic scene (seeded Weibull failure strains)"}
+            "data": "synthetic scene (failure strains from the seeded defect field)"}
+
+
+def defect_leg(reps, cpu):
+    """Device-side defect-field seeding (DefectField.sampleOnto) for the perforation scene's
+    ~200k-element assembly, plane-strain rule: device time of the three kernels per call,
+    inputs already in device memory (the copies precede the first event), median of `reps`
+    calls; then the weakest-link population of the reference's size-effect test (3000 seeds x
+    128 elements on each of four tubes) as four multi-seed calls. With `cpu`, the oracle's
+    serial sampleOnto on the same mesh beside it."""
+    import numpy as np
+    from neofiz_amd import perforation
+    from neofiz_amd.defects import DefectField, Weibull
+    from neofiz_amd.mesh import Outline
+    from neofiz_amd.solver import QuadMesh
+    cell = perforation.cell_for(200_000)
+    wall, slug = perforation.outlines()
+    mesh = Outline.assemble(cell, *wall, slug)
+    E = mesh.elementCount
+    base = Weibull(perforation.COPPER_STRAIN_SCALE, perforation.WEIBULL_MODULUS)
+    print(f"# defect field: {E} elements, {reps} calls", file=sys.stderr, flush=True)
+    field = DefectField(base, perforation.REFERENCE_VOLUME, perforation.CORRELATION_LENGTH, 5)
+    first = field.sampleOnto(mesh, depth=perforation.DEPTH)
+    calls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        field.sampleOnto(mesh, depth=perforation.DEPTH)
+        calls.append((sum(field.timing()), field.timing(), time.perf_counter() - t0))
+    calls.sort(key=lambda c: c[0])
+    total_ms, parts, wall_s = calls[len(calls) // 2]
+    # The smoothing's own arithmetic per element: 400 taps (multiply, add), 20 row weights
+    # (multiply, add), 40 weights ((-d) d / 2s^2 and the squared sums): 1000 FP64 operations
+    # beside 40 exponentials and the copula's 2 exponentials, a log and two powers.
+    flops = 1000.0 * E
+    size_seeds = 9000 + np.arange(3000)
+    size_ms = 0.0
+    size_elements = 0
+    for ri in (0.02, 0.04, 0.08, 0.16):
+        tube = QuadMesh.cylinderWall(ri, ri + 0.005, 0.02, 8, 16)
+        f2 = DefectField(Weibull(0.42, 20.0), 1e-9, 1e-3, 9000)
+        f2.sampleOnto(tube, seeds=size_seeds)
+        size_ms += sum(f2.timing())
+        size_elements += tube.elementCount * size_seeds.size
+        f2.close()
+    field.close()
+    out = {"workload": f"defect-field-perforation-{E}", "elements": E, "cell_m": cell,
+           "value": E / (total_ms * 1e-3), "unit": "elements/s",
+           "kernel_ms": {"bounding_box": parts[0], "noise_grid": parts[1], "smooth_transform": parts[2]},
+           "wall_ms_per_call_with_copies": wall_s * 1e3,
+           "smoothing_fp64": {"flops_per_element": 1000, "achieved_tflops": flops / (parts[2] * 1e-3) / 1e12,
+                              "peak_tflops": 78.6},
+           "median_failure_strain": float(np.median(first)),
+           "size_effect_population": {"samples": size_elements, "kernel_ms": size_ms,
+                                      "value": size_elements / (size_ms * 1e-3), "unit": "element-samples/s"}}
+    if cpu:
+        from oracle.pyoracle import OracleDefectField
+        ref = OracleDefectField(base, perforation.REFERENCE_VOLUME, perforation.CORRELATION_LENGTH, 5)
+        t0 = time.perf_counter()
+        want = ref.sampleOnto(mesh, depth=perforation.DEPTH)
+        cpu_s = time.perf_counter() - t0
+        ref.close()
+        out["cpu_baseline"] = {"value": E / cpu_s, "unit": "elements/s", "cores": 1, "kind": "port",
+                               "sample": f"the same {E}-element sampleOnto, oracle (serial C++ "
+                                         "restatement with the reference's memoised noise grid)"}
+        out["bitwise_equal_to_oracle"] = bool(np.array_equal(first.view(np.uint64), want.view(np.uint64)))
+    return out
 
 
 def main():
@@ -193,6 +264,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-sensitivity", action="store_true")
     ap.add_argument("--no-perforation", action="store_true")
+    ap.add_argument("--no-defect", action="store_true")
+    ap.add_argument("--defect-reps", type=int, default=20)
     ap.add_argument("--perforation-steps", type=int, default=2000)
     ap.add_argument("--perforation-warmup", type=int, default=1500)
     ap.add_argument("--ensemble", type=int, default=256, help="instances in the batched ensemble")
@@ -279,6 +352,8 @@ def main():
         except Exception as exc:  # torch missing or no device: report, do not hide
             line["stream_copy_GBps"] = None
             line["stream_copy_error"] = str(exc)
+    if not args.no_defect:
+        line["defect_field"] = defect_leg(args.defect_reps, not args.no_cpu)
     if not args.no_perforation:
         line["perforation"] = perforation_leg(200000, args.perforation_warmup, args.perforation_steps)
     if not args.no_sensitivity:

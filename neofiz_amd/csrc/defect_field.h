@@ -7,8 +7,8 @@
 // noise at grid index (i, j) is Box-Muller on two SplitMix64 hashes of (seed, i, j). A
 // failure strain is that value mapped through the normal cdf onto the Weibull quantile of the
 // element's effective volume. sampleOnto is three kernels on the handle's stream:
-//   k_df_bounds  bounding box of the nodes (Math.min / Math.max order-free: min and max with
-//                Java's NaN rule are associative);
+//   k_df_bounds  bounding box of the nodes, two passes (Math.min / Math.max order-free: min
+//                and max with Java's NaN rule are associative and commutative);
 //   k_df_noise   the white noise on the box's grid, for a chunk of seeds (the reference's
 //                memoisation, :351-384; the values are the hash's, so caching is invisible);
 //   k_df_sample  one thread per (element, seed): area centroid and signed area
@@ -32,7 +32,8 @@ constexpr double CLAMP_SIGMA = 9.0;  // :104
 // window has 2 radius + 2 = 20 taps. The constructor on the host checks this.
 constexpr int RADIUS = 9;
 constexpr int TAPS = 2 * RADIUS + 2;
-constexpr int BOUNDS_THREADS = 1024;
+constexpr int BOUNDS_THREADS = 256;
+constexpr int BOX_BLOCKS = 512;
 
 enum { OK = 0, BAD_VOLUME = 1, BAD_COUNT = 2, BAD_SCALE = 3, BAD_PROBABILITY = 4 };
 enum { SITES_MESH = 0, SITES_POINTS = 1 };
@@ -255,23 +256,16 @@ __device__ __forceinline__ double smooth(const Params& p, double r, double z, ui
 
 // ---- kernels ----
 
-// Math.min / Math.max over the coordinates, in one workgroup: [rMin, rMax, zMin, zMax].
-__global__ void __launch_bounds__(BOUNDS_THREADS) k_df_bounds(const double* __restrict__ r,
-                                                              const double* __restrict__ z, int n,
-                                                              double* __restrict__ box) {
-    __shared__ double s[4][BOUNDS_THREADS];
-    double rMin = 1.7976931348623157e308, rMax = -1.7976931348623157e308;
-    double zMin = 1.7976931348623157e308, zMax = -1.7976931348623157e308;
-    for (int k = threadIdx.x; k < n; k += BOUNDS_THREADS) {
-        rMin = dmin(rMin, r[k]);
-        rMax = dmax(rMax, r[k]);
-        zMin = dmin(zMin, z[k]);
-        zMax = dmax(zMax, z[k]);
-    }
-    s[0][threadIdx.x] = rMin;
-    s[1][threadIdx.x] = rMax;
-    s[2][threadIdx.x] = zMin;
-    s[3][threadIdx.x] = zMax;
+// Math.min / Math.max over the coordinates in two passes: BOX_BLOCKS workgroups each reduce
+// a grid-strided share to one partial [rMin, rMax, zMin, zMax], then one workgroup reduces the
+// partials. dmin and dmax are associative and commutative (NaN wins either way round; -0 is
+// below +0 either way round), so the result is the serial loop's.
+__device__ __forceinline__ void box_reduce(double (*s)[BOUNDS_THREADS], double v0, double v1, double v2,
+                                           double v3, double* __restrict__ dst) {
+    s[0][threadIdx.x] = v0;
+    s[1][threadIdx.x] = v1;
+    s[2][threadIdx.x] = v2;
+    s[3][threadIdx.x] = v3;
     __syncthreads();
     for (int h = BOUNDS_THREADS / 2; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) {
@@ -283,7 +277,36 @@ __global__ void __launch_bounds__(BOUNDS_THREADS) k_df_bounds(const double* __re
         }
         __syncthreads();
     }
-    if (threadIdx.x < 4) box[threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x < 4) dst[threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(BOUNDS_THREADS) k_df_bounds(const double* __restrict__ r,
+                                                              const double* __restrict__ z, int n,
+                                                              double* __restrict__ partial) {
+    __shared__ double s[4][BOUNDS_THREADS];
+    double rMin = 1.7976931348623157e308, rMax = -1.7976931348623157e308;
+    double zMin = 1.7976931348623157e308, zMax = -1.7976931348623157e308;
+    for (int k = blockIdx.x * BOUNDS_THREADS + threadIdx.x; k < n; k += gridDim.x * BOUNDS_THREADS) {
+        rMin = dmin(rMin, r[k]);
+        rMax = dmax(rMax, r[k]);
+        zMin = dmin(zMin, z[k]);
+        zMax = dmax(zMax, z[k]);
+    }
+    box_reduce(s, rMin, rMax, zMin, zMax, partial + 4 * blockIdx.x);
+}
+
+__global__ void __launch_bounds__(BOUNDS_THREADS) k_df_bounds_final(const double* __restrict__ partial,
+                                                                    int blocks, double* __restrict__ box) {
+    __shared__ double s[4][BOUNDS_THREADS];
+    double rMin = 1.7976931348623157e308, rMax = -1.7976931348623157e308;
+    double zMin = 1.7976931348623157e308, zMax = -1.7976931348623157e308;
+    for (int k = threadIdx.x; k < blocks; k += BOUNDS_THREADS) {
+        rMin = dmin(rMin, partial[4 * k]);
+        rMax = dmax(rMax, partial[4 * k + 1]);
+        zMin = dmin(zMin, partial[4 * k + 2]);
+        zMax = dmax(zMax, partial[4 * k + 3]);
+    }
+    box_reduce(s, rMin, rMax, zMin, zMax, box);
 }
 
 // White noise of `seeds` seeds on an ni x nj grid from (i0, j0): grid[s][b][a].
@@ -408,7 +431,8 @@ struct nf_defect {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double lastMs[3] = {0.0, 0.0, 0.0};
     // device scratch, grown on demand
-    double* box = nullptr;  // 4 doubles
+    double* box = nullptr;         // 4 doubles
+    double* boxPartial = nullptr;  // 4 per bounding-box workgroup
     int* status = nullptr;
     uint64_t* h0 = nullptr;
     size_t h0Cap = 0;
@@ -420,7 +444,7 @@ struct nf_defect {
 
     ~nf_defect() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* q : {(void*)box, (void*)status, (void*)h0, (void*)grid, (void*)r, (void*)z,
+        for (void* q : {(void*)box, (void*)boxPartial, (void*)status, (void*)h0, (void*)grid, (void*)r, (void*)z,
                         (void*)w, (void*)out, (void*)conn})
             if (q) (void)hipFree(q);
         for (hipEvent_t e : ev)
@@ -450,7 +474,12 @@ struct nf_defect {
             HIP_TRY(hipEventRecord(ev[1], stream));
             return NF_OK;
         }
-        hipLaunchKernelGGL(dfd::k_df_bounds, dim3(1), dim3(dfd::BOUNDS_THREADS), 0, stream, r_d, z_d, n, box);
+        const int blocks = std::max(1, std::min(dfd::BOX_BLOCKS, (n + 4 * dfd::BOUNDS_THREADS - 1) /
+                                                                     (4 * dfd::BOUNDS_THREADS)));
+        hipLaunchKernelGGL(dfd::k_df_bounds, dim3(blocks), dim3(dfd::BOUNDS_THREADS), 0, stream, r_d, z_d, n,
+                           boxPartial);
+        hipLaunchKernelGGL(dfd::k_df_bounds_final, dim3(1), dim3(dfd::BOUNDS_THREADS), 0, stream,
+                           (const double*)boxPartial, blocks, box);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ev[1], stream));
         double hb[4];
@@ -621,6 +650,7 @@ int nf_defect_create(nf_defect** out, const nf_defect_params* params) {
         HIP_TRY(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
         for (hipEvent_t& e : f->ev) HIP_TRY(hipEventCreate(&e));
         TRY(dev_alloc(&f->box, 4));
+        TRY(dev_alloc(&f->boxPartial, 4 * dfd::BOX_BLOCKS));
         TRY(dev_alloc(&f->status, 1));
         return NF_OK;
     };

@@ -8,9 +8,12 @@ blocks 20 mm (r) x 10 mm (z) stacked on the floor, a 10 mm x 20 mm steel slug at
 aimed at the third and fourth blocks, contact friction 0.2, floor friction 0.4, fracture
 energy 10 kJ/m^2, erosion at full damage, no mass scaling.
 
-The copper's failure strain is a Weibull draw of modulus 10 per element from a seeded numpy
-PCG64 generator; the reference draws a spatially correlated field instead
-(core/DefectField.java:239-332), which this scene does not restate.
+The copper's failure strain is the reference's spatially correlated defect field
+(core/DefectField.java:92-385), sampled on the device: Weibull modulus 10 and scale 1.2 at a
+reference volume of 1 mm^3, correlation length 1 mm, seed 5, with the plane-strain volume rule
+(each element holds l^3 max(1, A/l^2) max(1, depth/l), twenty correlation lengths through the
+20 mm depth), which puts the elements' characteristic failure strain near 0.89. The steel slug
+fails at 0.6.
 """
 import numpy as np
 
@@ -30,7 +33,9 @@ STEEL = Material.STEEL_4340.withJohnsonCook(JohnsonCook.STEEL_4340)
 COPPER = Material.COPPER_OFHC.withJohnsonCook(JohnsonCook.COPPER_OFHC)
 FRACTURE_ENERGY = 10.0e3
 WEIBULL_MODULUS = 10.0
-COPPER_STRAIN_SCALE = 0.9
+COPPER_STRAIN_SCALE = 1.2        # Weibull scale at the reference volume
+REFERENCE_VOLUME = 1.0e-9
+CORRELATION_LENGTH = 1.0e-3
 SLUG_FAILURE_STRAIN = 0.6
 
 
@@ -47,8 +52,21 @@ def cell_for(elements, blocks=BLOCKS):
     return float(np.sqrt(area / elements))
 
 
-def build(cell, solver_cls=ExplicitSolver, blocks=BLOCKS, seed=5, speed=SPEED):
-    """Returns (mesh, solver, is_slug)."""
+def failure_strains(mesh, is_slug, seed=5, binding=None):
+    """The copper's defect field on the mesh (the slug's elements get its fixed strain)."""
+    from .defects import DefectField, Weibull
+    field = DefectField(Weibull(COPPER_STRAIN_SCALE, WEIBULL_MODULUS), REFERENCE_VOLUME,
+                        CORRELATION_LENGTH, seed, binding=binding)
+    try:
+        eps = field.sampleOnto(mesh, depth=DEPTH)
+    finally:
+        field.close()
+    return np.where(is_slug, SLUG_FAILURE_STRAIN, eps)
+
+
+def build(cell, solver_cls=ExplicitSolver, blocks=BLOCKS, seed=5, speed=SPEED, field_binding=None):
+    """Returns (mesh, solver, is_slug). field_binding selects the library that samples the
+    defect field (the HIP library by default)."""
     wall, slug = outlines(blocks)
     mesh = Outline.assemble(cell, *wall, slug)
     # The slug is the last body assembled; its elements are the ones whose centroid lies in
@@ -63,9 +81,7 @@ def build(cell, solver_cls=ExplicitSolver, blocks=BLOCKS, seed=5, speed=SPEED):
     s.setGravity(0.0, -9.81)
     s.setRigidWall(RigidWall.atZ(0.0, 0.4))
     s.setContact(Contact(mesh).withFriction(0.2))
-    rng = np.random.default_rng(seed)
-    eps = np.where(is_slug, SLUG_FAILURE_STRAIN,
-                   COPPER_STRAIN_SCALE * rng.weibull(WEIBULL_MODULUS, mesh.elementCount))
+    eps = failure_strains(mesh, is_slug, seed, field_binding)
     s.setDamage(np.full(mesh.elementCount, FRACTURE_ENERGY), eps)
     s.setErosion(1.0)
     v = np.zeros(mesh.nodeCount)
