@@ -226,6 +226,35 @@ int nf_batch_get_tile(nf_batch* b, int32_t instance, int32_t* threads);
 int nf_batch_run_until(nf_batch* b, double end_time, int64_t max_steps);
 int nf_batch_scatter(nf_batch* b);
 
+/* ---- Film frames (render/Film.java) ----
+ *
+ * Film.capture(solver) (render/Film.java:159-189) for the element quantities the reference's
+ * scenes colour by, computed on the device from the solver's state in one launch:
+ *   PLASTIC_STRAIN  elementPlasticStrain (solver/ExplicitSolver.java:1897-1901)
+ *   DAMAGE          elementDamage (:1764-1772)
+ *   SPEED           elementSpeed (:895-902)
+ *   MISES           von Mises of centroidStress (:1886-1894; Impact.java:229-232)
+ *   STRESS_RR..RZ   centroidStress components
+ *   CENTROID_Z      centroidZ in the current configuration (:1918-1925)
+ *   ALIVE           1 - isEroded (:627-629), the usual aliveIf
+ * r and z receive the deformed node positions (mesh + displacement, Film.java:161-166);
+ * values receives field_count x element_count values, field-major. Arbitrary caller-defined
+ * quantities (the Java lambdas) are host-side and not part of this call. */
+enum nf_film_field {
+    NF_FILM_PLASTIC_STRAIN = 0,
+    NF_FILM_DAMAGE = 1,
+    NF_FILM_SPEED = 2,
+    NF_FILM_MISES = 3,
+    NF_FILM_STRESS_RR = 4,
+    NF_FILM_STRESS_ZZ = 5,
+    NF_FILM_STRESS_TT = 6,
+    NF_FILM_STRESS_RZ = 7,
+    NF_FILM_CENTROID_Z = 8,
+    NF_FILM_ALIVE = 9
+};
+int nf_film_capture(nf_solver* s, int32_t field_count, const int32_t* fields, double* r, double* z,
+                    double* values);
+
 /* ---- Defect-field seeding (core/DefectField.java, core/Weibull.java, core/Normal.java) ----
  *
  * new DefectField(new Weibull(scale, modulus), referenceVolume, correlationLength, seed)

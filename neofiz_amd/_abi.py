@@ -118,6 +118,7 @@ _PROTOTYPES = {
                                 c_double_p]),
     "defect_timing": (C.c_int, [_P, c_double_p]),
     "normal_eval": (C.c_int, [C.c_int32, C.c_int32, c_double_p, c_double_p]),
+    "film_capture": (C.c_int, [_P, C.c_int32, c_int32_p, c_double_p, c_double_p, c_double_p]),
 }
 
 

@@ -66,9 +66,12 @@ __device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t, const n
 // solver/J2.java:332-375, the bracketed Newton solve for the Johnson-Cook multiplier.
 // `thermal` is thermalTerm(temperature), which the caller has already evaluated for the
 // yield check on the same temperature (J2.java:221 and :334 compute the same value twice).
-__device__ __forceinline__ double j2_solve(double norm, double stiffness, double plasticStrain,
-                                           double thermal, double dt, const Mat& j,
-                                           const nfm::Tables& T) {
+// UNIT_RATE: the reference strain rate is 1/s, where rate / eps0 is rate itself; the loop is
+// instantiated for it so that no iteration evaluates a division whose result is discarded.
+template <bool UNIT_RATE>
+__device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, double plasticStrain,
+                                                double thermal, double dt, const Mat& j,
+                                                const nfm::Tables& T) {
     const double tolerance = 1.0e-10 * j.a;
     double lo = 0.0;
     double hi = norm / stiffness;
@@ -83,7 +86,7 @@ __device__ __forceinline__ double j2_solve(double norm, double stiffness, double
         const double pw = nfm::pow(eps, j.n, T);
         const double strain = eps <= 0.0 ? j.a : j.a + j.b * pw;  // strainTerm
         const bool clamped = rate <= j.eps0;
-        const double lg = nfm::log(j.eps0 == 1.0 ? rate : rate / j.eps0, T);
+        const double lg = nfm::log(UNIT_RATE ? rate : rate / j.eps0, T);
         const double rateFactor = clamped ? 1.0 : 1.0 + j.c * lg;  // rateTerm
         const double g = norm - stiffness * dGamma - NFD_SQRT_2_3 * strain * rateFactor * thermal;
         lo = g > 0.0 ? dGamma : lo;
@@ -95,6 +98,13 @@ __device__ __forceinline__ double j2_solve(double norm, double stiffness, double
         dGamma = (next > lo && next < hi) ? next : 0.5 * (lo + hi);
     }
     return dGamma;
+}
+
+__device__ __forceinline__ double j2_solve(double norm, double stiffness, double plasticStrain,
+                                           double thermal, double dt, const Mat& j,
+                                           const nfm::Tables& T) {
+    return j.eps0 == 1.0 ? j2_solve_loop<true>(norm, stiffness, plasticStrain, thermal, dt, j, T)
+                         : j2_solve_loop<false>(norm, stiffness, plasticStrain, thermal, dt, j, T);
 }
 
 // Damage state of one Gauss point (solver/Damage.java:156-192).

@@ -890,6 +890,82 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
     if (mode == MODE_STEP && d.wall && blockIdx.x == 0) reduce_wall_partials(d);
 }
 
+// ------------------------------------------------------------------ film frames
+
+// The element quantities a film frame can carry (enum nf_film_field), at most FILM_MAX.
+constexpr int FILM_MAX = 16;
+struct FilmFields {
+    int n;
+    int id[FILM_MAX];
+};
+
+// Film.capture (render/Film.java:159-189) for the fields the reference's scenes colour by,
+// each restated from its accessor in solver/ExplicitSolver.java with the same summation order:
+// out[0, N) deformed r (mesh.r + ur, :161-166), out[N, 2N) deformed z, then one block of E
+// values per field.
+__global__ void k_film(Dev d, FilmFields ff, double* __restrict__ out) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < d.N) {
+        const double2 x = d.x0[i], u = d.u[i];
+        out[i] = x.x + u.x;
+        out[d.N + i] = x.y + u.y;
+    }
+    if (i >= d.E) return;
+    const int e = i;
+    const int4 c = d.conn[e];
+    const int nodes[4] = {c.x, c.y, c.z, c.w};
+    for (int f = 0; f < ff.n; f++) {
+        double v = 0.0;
+        const int id = ff.id[f];
+        if (id == NF_FILM_PLASTIC_STRAIN) {  // elementPlasticStrain :1897-1901
+            double sum = 0.0;
+            for (int g = 0; g < d.ppe; g++) sum += d.epsP[e * d.ppe + g];
+            v = sum / d.ppe;
+        } else if (id == NF_FILM_DAMAGE) {  // elementDamage :1764-1772
+            if (d.dStrength) {
+                double sum = 0.0;
+                for (int g = 0; g < d.ppe; g++) {
+                    const int p = e * d.ppe + g;
+                    sum += nfd::damage_at(d.dStrength[p], d.dSoftening[p], d.dFailure[p], d.epsP[p]);
+                }
+                v = sum / d.ppe;
+            }
+        } else if (id == NF_FILM_SPEED) {  // elementSpeed :895-902
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double2 w = d.v[nodes[k]];
+                sum += __builtin_sqrt(w.x * w.x + w.y * w.y);
+            }
+            v = 0.25 * sum;
+        } else if (id == NF_FILM_CENTROID_Z) {  // centroidZ, current configuration :1918-1925
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) sum += d.x0[nodes[k]].y + d.u[nodes[k]].y;
+            v = 0.25 * sum;
+        } else if (id == NF_FILM_ALIVE) {  // !isEroded :627-629
+            v = d.gone && d.gone[e] ? 0.0 : 1.0;
+        } else {  // centroidStress :1886-1894, and the scenes' von Mises of it (Impact.java:229-232)
+            double s4[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int g = 0; g < d.ppe; g++) {
+                const int p = e * d.ppe + g;
+                s4[0] += d.sig0[p];
+                s4[1] += d.sig1[p];
+                s4[2] += d.sig2[p];
+                s4[3] += d.sig3[p];
+            }
+            for (int k = 0; k < 4; k++) s4[k] /= d.ppe;
+            if (id == NF_FILM_MISES) {
+                const double a = s4[0] - s4[1], b = s4[1] - s4[2], cc = s4[2] - s4[0];
+                v = __builtin_sqrt(0.5 * (a * a + b * b + cc * cc) + 3.0 * s4[3] * s4[3]);
+            } else {
+                v = s4[id - NF_FILM_STRESS_RR];
+            }
+        }
+        out[2 * d.N + f * d.E + e] = v;
+    }
+}
+
 // ------------------------------------------------------------------ setup and small kernels
 
 // Row-sum lumped mass (solver/ExplicitSolver.java:1559-1597) as a per-node gather over the
@@ -1128,10 +1204,13 @@ struct nf_solver {
     // device buffers owned here
     std::vector<void*> owned;
     double* densityOfElement = nullptr;
+    double* filmBuf = nullptr;  // film frame staging (nf_film_capture), grown on demand
+    size_t filmCap = 0;
 
     ~nf_solver() {
         if (stream) (void)hipStreamSynchronize(stream);
         dropGraphs();
+        if (filmBuf) (void)hipFree(filmBuf);
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
         for (hipEvent_t e : statusEvent)
             if (e) (void)hipEventDestroy(e);
@@ -2138,6 +2217,40 @@ int nf_get_element_field(nf_solver* s, int32_t field, double* out) {
         default: return fail(NF_E_ARGUMENT, "unknown element field");
     }
     HIP_TRY(hipMemcpyAsync(out, src, sizeof(double) * E, hipMemcpyDeviceToHost, s->stream));
+    return s->sync();
+}
+
+int nf_film_capture(nf_solver* s, int32_t field_count, const int32_t* fields, double* r, double* z,
+                    double* values) {
+    g_error.clear();
+    if (!s) return fail(NF_E_ARGUMENT, "null handle");
+    if (field_count < 0 || field_count > FILM_MAX)
+        return fail(NF_E_ARGUMENT, "a frame carries 0 to " + std::to_string(FILM_MAX) + " fields");
+    if (!r || !z || (field_count && (!fields || !values))) return fail(NF_E_ARGUMENT, "null array");
+    FilmFields ff{};
+    ff.n = field_count;
+    for (int f = 0; f < field_count; f++) {
+        if (fields[f] < NF_FILM_PLASTIC_STRAIN || fields[f] > NF_FILM_ALIVE)
+            return fail(NF_E_ARGUMENT, "unknown film field " + std::to_string(fields[f]));
+        ff.id[f] = fields[f];
+    }
+    const size_t N = (size_t)s->N, E = (size_t)s->E;
+    const size_t need = 2 * N + (size_t)field_count * E;
+    if (need > s->filmCap) {
+        if (s->filmBuf) HIP_TRY(hipFree(s->filmBuf));
+        s->filmBuf = nullptr;
+        s->filmCap = 0;
+        HIP_TRY(hipMalloc((void**)&s->filmBuf, sizeof(double) * need));
+        s->filmCap = need;
+    }
+    const int n = (int)std::max(N, E);
+    hipLaunchKernelGGL(k_film, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->d, ff, s->filmBuf);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(r, s->filmBuf, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(z, s->filmBuf + N, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+    if (field_count)
+        HIP_TRY(hipMemcpyAsync(values, s->filmBuf + 2 * N, sizeof(double) * field_count * E,
+                               hipMemcpyDeviceToHost, s->stream));
     return s->sync();
 }
 

@@ -513,6 +513,7 @@ class ExplicitSolver:
             raise IllegalArgumentException(
                 f"failure strain has {eps.size} entries for a mesh of {E} elements")
         self._call("set_damage", _dptr(gf), _dptr(eps))
+        self._failureStrain = eps.copy()
 
     def setErosion(self, threshold):
         self._call("set_erosion", float(threshold))
@@ -702,6 +703,33 @@ class ExplicitSolver:
         for g in range(ppe):
             out += sig[e * ppe + g]
         return out / ppe
+
+    def elementDamage(self, e):  # :1764-1772, with Damage.at (solver/Damage.java:342-347)
+        fail = getattr(self, "_failureStrain", None)
+        if fail is None:
+            return 0.0
+        ppe = self.state().points_per_element
+        strength = self.pointField("damage_strength")
+        softening = self.pointField("damage_softening")
+        ep = self.pointField("plastic_strain")
+        total = 0.0
+        for g in range(ppe):
+            p = e * ppe + g
+            sf = strength[p]
+            if sf <= 0.0:
+                d = 0.0
+            else:
+                d = -softening[p] * (ep[p] - fail[e]) / sf
+                d = 0.0 if d <= 0.0 else (1.0 if d >= 1.0 else d)
+            total += d
+        return total / ppe
+
+    def elementSpeed(self, e):  # :895-902
+        vr, vz = self.velocityR(), self.velocityZ()
+        total = 0.0
+        for n in self.mesh.conn[4 * e:4 * e + 4]:
+            total += math.sqrt(vr[n] * vr[n] + vz[n] * vz[n])
+        return 0.25 * total
 
     def elementPlasticStrain(self, e):  # :1897-1901
         ppe = self.state().points_per_element

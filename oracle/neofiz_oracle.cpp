@@ -1800,6 +1800,80 @@ int nfo_get_point_field(nfo_solver* s, int32_t field, double* out) {
         default: return fail(NF_E_ARGUMENT, "unknown point field");
     }
 }
+// Film.capture (render/Film.java:159-189) with the element accessors of
+// solver/ExplicitSolver.java the scenes colour by; see nf_film_capture.
+int nfo_film_capture(nfo_solver* s, int32_t count, const int32_t* fields, double* r, double* z,
+                     double* values) {
+    if (!s) return fail(NF_E_ARGUMENT, "null handle");
+    if (count < 0 || count > 16) return fail(NF_E_ARGUMENT, "a frame carries 0 to 16 fields");
+    for (int f = 0; f < count; f++)
+        if (fields[f] < NF_FILM_PLASTIC_STRAIN || fields[f] > NF_FILM_ALIVE)
+            return fail(NF_E_ARGUMENT, "unknown film field " + std::to_string(fields[f]));
+    const Mesh& m = s->mesh;
+    for (int i = 0; i < m.nodeCount; i++) {  // :161-166
+        r[i] = m.r[i] + s->ur[i];
+        z[i] = m.z[i] + s->uz[i];
+    }
+    const int ppe = s->pointsPerElement;
+    for (int f = 0; f < count; f++) {
+        double* out = values + (size_t)f * m.elementCount;
+        for (int e = 0; e < m.elementCount; e++) {
+            double v = 0.0;
+            switch (fields[f]) {
+                case NF_FILM_PLASTIC_STRAIN: {  // elementPlasticStrain :1897-1901
+                    double sum = 0.0;
+                    for (int g = 0; g < ppe; g++) sum += s->epsP[e * ppe + g];
+                    v = sum / ppe;
+                    break;
+                }
+                case NF_FILM_DAMAGE: {  // elementDamage :1764-1772
+                    if (s->damage == nullptr) break;
+                    double sum = 0.0;
+                    for (int g = 0; g < ppe; g++) {
+                        const int p = e * ppe + g;
+                        sum += s->damage->at(p, s->epsP[p]);
+                    }
+                    v = sum / ppe;
+                    break;
+                }
+                case NF_FILM_SPEED: {  // elementSpeed :895-902
+                    double sum = 0.0;
+                    for (int k = 0; k < 4; k++) {
+                        const int n = m.conn[e * 4 + k];
+                        sum += std::sqrt(s->vr[n] * s->vr[n] + s->vz[n] * s->vz[n]);
+                    }
+                    v = 0.25 * sum;
+                    break;
+                }
+                case NF_FILM_CENTROID_Z: {  // centroidZ :1918-1925
+                    double sum = 0.0;
+                    for (int k = 0; k < 4; k++) {
+                        const int n = m.conn[e * 4 + k];
+                        sum += m.z[n] + s->uz[n];
+                    }
+                    v = 0.25 * sum;
+                    break;
+                }
+                case NF_FILM_ALIVE: v = (!s->gone.empty() && s->gone[e]) ? 0.0 : 1.0; break;
+                default: {  // centroidStress :1886-1894; mises, Impact.java:229-232
+                    double c[4] = {0.0, 0.0, 0.0, 0.0};
+                    for (int g = 0; g < ppe; g++)
+                        for (int k = 0; k < 4; k++) c[k] += s->sig[4 * (e * ppe + g) + k];
+                    for (int k = 0; k < 4; k++) c[k] /= ppe;
+                    if (fields[f] == NF_FILM_MISES) {
+                        const double a = c[0] - c[1], b = c[1] - c[2], cc = c[2] - c[0];
+                        v = std::sqrt(0.5 * (a * a + b * b + cc * cc) + 3.0 * c[3] * c[3]);
+                    } else {
+                        v = c[fields[f] - NF_FILM_STRESS_RR];
+                    }
+                }
+            }
+            out[e] = v;
+        }
+    }
+    return NF_OK;
+}
+
 int nfo_get_element_field(nfo_solver* s, int32_t field, double* out) {
     const int E = s->mesh.elementCount;
     switch (field) {
