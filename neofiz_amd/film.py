@@ -19,6 +19,7 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _abi
+from .report import java_g
 from .solver import IllegalArgumentException, check
 
 AUTO = math.nan
@@ -235,44 +236,18 @@ def _array(key, values):  # :286-293
 
 
 def number(value):
-    """Film.number (Film.java:295-307): Java's String.format("%.6g") with trailing zeros and
-    a trailing point trimmed and the exponent written as a plain integer; non-finite values
-    and zero print as 0. Java rounds the shortest decimal digits of the double half-up to six
-    significant figures (java.util.Formatter's GENERAL conversion), which is restated here on
-    Python's shortest repr; C's printf, which rounds the exact binary value half-even, differs
-    from it on ties such as 1.000005 or 100000.5. Java's digit generator is not the shortest
-    for subnormal doubles (it prints Double.MIN_VALUE as 4.9e-324, this as 4.94066e-324's
-    shortest form 5e-324); no film quantity is subnormal."""
+    """Film.number (Film.java:295-307): Java's String.format("%.6g") (report.java_g) with
+    trailing zeros and a trailing point trimmed and the exponent written as a plain integer;
+    non-finite values and zero print as 0. Java's digit generator is not the shortest for
+    subnormal doubles (it prints Double.MIN_VALUE as 4.9e-324, this as 5e-324); no film
+    quantity is subnormal."""
     if not math.isfinite(value) or value == 0.0:
         return "0"
-    text = repr(float(value))
-    neg = text.startswith("-")
-    if neg:
-        text = text[1:]
-    mant, _, exp = text.partition("e")
-    e10 = int(exp) if exp else 0
-    whole, _, frac = mant.partition(".")
-    digits = (whole + frac).lstrip("0")
-    # decimal exponent of the first significant digit
-    lead = len(whole.lstrip("0")) - 1 if whole.strip("0") else -(len(frac) - len(frac.lstrip("0")) + 1)
-    point = lead + e10
-    digits = digits.rstrip("0") or "0"
-    if len(digits) > 6:  # round half-up to six significant digits
-        head = int(digits[:6]) + (1 if digits[6] >= "5" else 0)
-        if head == 1000000:
-            head = 100000
-            point += 1
-        digits = str(head)
-    digits = digits.ljust(6, "0")
-    if -4 <= point < 6:  # decimal form
-        if point >= 0:
-            out = digits[:point + 1] + "." + digits[point + 1:]
-        else:
-            out = "0." + "0" * (-point - 1) + digits
-        out = _trim(out) if "." in out else out
-    else:
-        out = _trim(digits[0] + "." + digits[1:]) + "e" + str(point)
-    return "-" + out if neg else out
+    text = java_g(float(value), 6)
+    if "e" in text:
+        mant, exp = text.split("e")
+        return _trim(mant) + "e" + str(int(exp))
+    return _trim(text) if "." in text else text
 
 
 def _trim(text):  # :309-314
