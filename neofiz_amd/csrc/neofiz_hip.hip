@@ -1371,12 +1371,12 @@ struct nf_solver {
     // gathers it and opens the next. The device decides whether either does anything.
     int launchTangent(int mode, int phase, int gatherFirst) {
         if (mode >= 0) {
-            hipLaunchKernelGGL(k_t_element, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, td, q, mode);
+            hipLaunchKernelGGL(k_t_element, dim3(blocks_for(2 * E)), dim3(BLOCK), 0, stream, d, td, q, mode);
             if (mode == MODE_STEP) hipLaunchKernelGGL(k_t_argmin, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, td, q);
         }
         if (phase >= 0) {
-            if (hasWall) hipLaunchKernelGGL(k_t_gn<true>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, td, q, phase, gatherFirst);
-            else hipLaunchKernelGGL(k_t_gn<false>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, td, q, phase, gatherFirst);
+            if (hasWall) hipLaunchKernelGGL(k_t_gn<true>, dim3(blocks_for(2 * N)), dim3(BLOCK), 0, stream, d, td, q, phase, gatherFirst);
+            else hipLaunchKernelGGL(k_t_gn<false>, dim3(blocks_for(2 * N)), dim3(BLOCK), 0, stream, d, td, q, phase, gatherFirst);
         }
         HIP_TRY(hipGetLastError());
         return NF_OK;

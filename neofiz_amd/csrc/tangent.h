@@ -23,24 +23,32 @@
 
 constexpr int NP = 6;  // friction, A, B, n, C, m
 enum { SP_FRICTION = 0, SP_A, SP_B, SP_N, SP_C, SP_M };
+// Each thread carries NL of the NP directions: two threads share an element (or a node),
+// one with directions 0-2, one with 3-5. Both compute the value parts, which are identical;
+// only the first stores them. The derivative of one direction never reads another's, so the
+// split changes no bit of either.
+constexpr int NL = 3;
 
 namespace nft {
 
 struct D {
     double v;
-    double t[NP];
+    double t[NL];
 };
 
 __device__ __forceinline__ D cst(double v) {
     D r;
     r.v = v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = 0.0;
+    for (int k = 0; k < NL; k++) r.t[k] = 0.0;
     return r;
 }
-__device__ __forceinline__ D seed(double v, int k) {
-    D r = cst(v);
-    r.t[k] = 1.0;
+// v seeded in direction k, as seen by a thread that carries directions dir0 .. dir0 + NL - 1.
+__device__ __forceinline__ D seed(double v, int k, int dir0) {
+    D r;
+    r.v = v;
+#pragma unroll
+    for (int m = 0; m < NL; m++) r.t[m] = m == k - dir0 ? 1.0 : 0.0;
     return r;
 }
 
@@ -48,35 +56,35 @@ __device__ __forceinline__ D operator+(const D& a, const D& b) {
     D r;
     r.v = a.v + b.v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a.t[k] + b.t[k];
+    for (int k = 0; k < NL; k++) r.t[k] = a.t[k] + b.t[k];
     return r;
 }
 __device__ __forceinline__ D operator-(const D& a, const D& b) {
     D r;
     r.v = a.v - b.v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a.t[k] - b.t[k];
+    for (int k = 0; k < NL; k++) r.t[k] = a.t[k] - b.t[k];
     return r;
 }
 __device__ __forceinline__ D operator-(const D& a) {
     D r;
     r.v = -a.v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = -a.t[k];
+    for (int k = 0; k < NL; k++) r.t[k] = -a.t[k];
     return r;
 }
 __device__ __forceinline__ D operator*(const D& a, const D& b) {
     D r;
     r.v = a.v * b.v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a.t[k] * b.v + a.v * b.t[k];
+    for (int k = 0; k < NL; k++) r.t[k] = a.t[k] * b.v + a.v * b.t[k];
     return r;
 }
 __device__ __forceinline__ D operator/(const D& a, const D& b) {
     D r;
     r.v = a.v / b.v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = (a.t[k] - r.v * b.t[k]) / b.v;
+    for (int k = 0; k < NL; k++) r.t[k] = (a.t[k] - r.v * b.t[k]) / b.v;
     return r;
 }
 __device__ __forceinline__ D operator+(const D& a, double b) {
@@ -98,35 +106,35 @@ __device__ __forceinline__ D operator-(double a, const D& b) {
     D r;
     r.v = a - b.v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = -b.t[k];
+    for (int k = 0; k < NL; k++) r.t[k] = -b.t[k];
     return r;
 }
 __device__ __forceinline__ D operator*(const D& a, double b) {
     D r;
     r.v = a.v * b;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a.t[k] * b;
+    for (int k = 0; k < NL; k++) r.t[k] = a.t[k] * b;
     return r;
 }
 __device__ __forceinline__ D operator*(double a, const D& b) {
     D r;
     r.v = a * b.v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a * b.t[k];
+    for (int k = 0; k < NL; k++) r.t[k] = a * b.t[k];
     return r;
 }
 __device__ __forceinline__ D operator/(const D& a, double b) {
     D r;
     r.v = a.v / b;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a.t[k] / b;
+    for (int k = 0; k < NL; k++) r.t[k] = a.t[k] / b;
     return r;
 }
 __device__ __forceinline__ D operator/(double a, const D& b) {
     D r;
     r.v = a / b.v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = -r.v * b.t[k] / b.v;
+    for (int k = 0; k < NL; k++) r.t[k] = -r.v * b.t[k] / b.v;
     return r;
 }
 __device__ __forceinline__ D sqrt(const D& a) {
@@ -134,7 +142,7 @@ __device__ __forceinline__ D sqrt(const D& a) {
     r.v = __builtin_sqrt(a.v);
     const double s = r.v > 0.0 ? 0.5 / r.v : 0.0;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a.t[k] * s;
+    for (int k = 0; k < NL; k++) r.t[k] = a.t[k] * s;
     return r;
 }
 // log and pow: the value from the shared table method, the derivative from calculus.
@@ -142,7 +150,7 @@ __device__ __forceinline__ D log(const D& a, const nfm::Tables& T) {
     D r;
     r.v = nfm::log(a.v, T);
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a.t[k] / a.v;
+    for (int k = 0; k < NL; k++) r.t[k] = a.t[k] / a.v;
     return r;
 }
 __device__ __forceinline__ D pow(const D& x, const D& y, const nfm::Tables& T) {
@@ -151,7 +159,7 @@ __device__ __forceinline__ D pow(const D& x, const D& y, const nfm::Tables& T) {
     const double lx = x.v > 0.0 ? nfm::log(x.v, T) : 0.0;
     const double dx = x.v != 0.0 ? r.v * y.v / x.v : 0.0;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = dx * x.t[k] + r.v * lx * y.t[k];
+    for (int k = 0; k < NL; k++) r.t[k] = dx * x.t[k] + r.v * lx * y.t[k];
     return r;
 }
 // Math.min / Math.max: the selected argument, value and derivative.
@@ -170,7 +178,7 @@ struct DMat {
     D a, b, n, c, m;
 };
 
-__device__ __forceinline__ DMat dmat(const Mat& f) {
+__device__ __forceinline__ DMat dmat(const Mat& f, int dir0) {
     DMat j;
     j.lambda = f.lambda;
     j.mu = f.mu;
@@ -178,11 +186,11 @@ __device__ __forceinline__ DMat dmat(const Mat& f) {
     j.troom = f.troom;
     j.tmelt = f.tmelt;
     j.thermalCoefficient = f.thermalCoefficient;
-    j.a = seed(f.a, SP_A);
-    j.b = seed(f.b, SP_B);
-    j.n = seed(f.n, SP_N);
-    j.c = seed(f.c, SP_C);
-    j.m = seed(f.m, SP_M);
+    j.a = seed(f.a, SP_A, dir0);
+    j.b = seed(f.b, SP_B, dir0);
+    j.n = seed(f.n, SP_N, dir0);
+    j.c = seed(f.c, SP_C, dir0);
+    j.m = seed(f.m, SP_M, dir0);
     return j;
 }
 
@@ -278,7 +286,7 @@ __device__ __forceinline__ void j2_update(DPoint& q, const D& dRR, const D& dZZ,
     if (nfd::jabs(g.v) <= 1.0e-10 * j.a.v) {
         dGamma.v = g0;
 #pragma unroll
-        for (int k = 0; k < NP; k++) dGamma.t[k] = -g.t[k] / slope;
+        for (int k = 0; k < NL; k++) dGamma.t[k] = -g.t[k] / slope;
     } else {
         dGamma = j2_solve(norm, stiffness, q.epsP, thermal, strainDt, j, T);
         if (unconverged) atomicAdd(unconverged, 1);
@@ -329,18 +337,18 @@ __device__ __forceinline__ D load(const double* a, size_t stride, size_t i, doub
     D r;
     r.v = v;
 #pragma unroll
-    for (int k = 0; k < NP; k++) r.t[k] = a[k * stride + i];
+    for (int k = 0; k < NL; k++) r.t[k] = a[k * stride + i];
     return r;
 }
 __device__ __forceinline__ void store(double* a, size_t stride, size_t i, const D& x) {
 #pragma unroll
-    for (int k = 0; k < NP; k++) a[k * stride + i] = x.t[k];
+    for (int k = 0; k < NL; k++) a[k * stride + i] = x.t[k];
 }
 __device__ __forceinline__ void load2(const double2* a, size_t stride, size_t i, double2 v, D& x, D& y) {
     x.v = v.x;
     y.v = v.y;
 #pragma unroll
-    for (int k = 0; k < NP; k++) {
+    for (int k = 0; k < NL; k++) {
         const double2 w = a[k * stride + i];
         x.t[k] = w.x;
         y.t[k] = w.y;
@@ -348,18 +356,43 @@ __device__ __forceinline__ void load2(const double2* a, size_t stride, size_t i,
 }
 __device__ __forceinline__ void store2(double2* a, size_t stride, size_t i, const D& x, const D& y) {
 #pragma unroll
-    for (int k = 0; k < NP; k++) a[k * stride + i] = make_double2(x.t[k], y.t[k]);
+    for (int k = 0; k < NL; k++) a[k * stride + i] = make_double2(x.t[k], y.t[k]);
+}
+
+// The derivative arrays as a thread carrying directions dir0 .. dir0 + NL - 1 sees them:
+// every [direction][item] base advanced by dir0 directions.
+__device__ __forceinline__ TDev shifted(const TDev& t, int dir0, size_t N, size_t E, size_t P) {
+    TDev s = t;
+    s.tu += dir0 * N;
+    s.tv += dir0 * N;
+    s.tf += dir0 * N;
+    s.tslot += dir0 * 4 * E;
+    s.tsig += dir0 * 4 * P;
+    s.tep += dir0 * P;
+    s.twork += dir0 * P;
+    s.thg += dir0 * 2 * E;
+    s.tedge += dir0 * E;
+    s.tstep += dir0;
+    return s;
 }
 
 }  // namespace nft
 
+// The value results of one element, stored by the first of its two threads after both have
+// read the state they replace.
+struct TElementOut {
+    double sig0, sig1, sig2, sig3, epsP, work, hgR, hgZ, hgK, edge;
+    double2 slot[4];
+};
+
 // One element under reduced integration (solver/ExplicitSolver.java:1373-1504) over Duals:
-// the value parts are reduced_element's operations exactly.
+// the value parts are reduced_element's operations exactly. Stores this thread's derivative
+// directions (t is the shifted view) and returns the values in o.
 __device__ void tangent_element(const Dev& d, const TDev& t, int e, int4 n, const nft::D& strainDt,
-                                const nfm::Tables& T) {
+                                const nfm::Tables& T, int dir0, int* unconverged, TElementOut& o) {
     using nft::D;
     const Mat fl = d.mat0;
-    const nft::DMat j = nft::dmat(fl);
+    const nft::DMat j = nft::dmat(fl, dir0);
     const double mu = fl.mu;
     const size_t N = d.N, E = d.E, P = d.P;
     const D mid = -0.5 * strainDt;
@@ -428,13 +461,13 @@ __device__ void tangent_element(const Dev& d, const TDev& t, int e, int4 n, cons
             nft::corot_rotate(q.s0, q.s1, q.s3, cs, sn);
         }
     }
-    nft::j2_update(q, dEpsR, dEpsZ, dEpsT, dGamRZ, j, fl, strainDt, T, t.argmin + 2);
-    d.sig0[e] = q.s0.v;
-    d.sig1[e] = q.s1.v;
-    d.sig2[e] = q.s2.v;
-    d.sig3[e] = q.s3.v;
-    d.epsP[e] = q.epsP.v;
-    d.work[e] = q.work.v;
+    nft::j2_update(q, dEpsR, dEpsZ, dEpsT, dGamRZ, j, fl, strainDt, T, unconverged);
+    o.sig0 = q.s0.v;
+    o.sig1 = q.s1.v;
+    o.sig2 = q.s2.v;
+    o.sig3 = q.s3.v;
+    o.epsP = q.epsP.v;
+    o.work = q.work.v;
     nft::store(t.tsig, 4 * P, e, q.s0);
     nft::store(t.tsig + P, 4 * P, e, q.s1);
     nft::store(t.tsig + 2 * P, 4 * P, e, q.s2);
@@ -474,45 +507,70 @@ __device__ void tangent_element(const Dev& d, const TDev& t, int e, int4 n, cons
         const D hgZ = nft::load(t.thg + E, 2 * E, e, d.hgZ[e]);
         const D qr = hgR + k * (y0 * a[0] + y1 * a[1] + y2 * a[2] + y3 * a[3]);
         const D qz = hgZ + k * (y0 * c[0] + y1 * c[1] + y2 * c[2] + y3 * c[3]);
-        d.hgR[e] = qr.v;
-        d.hgZ[e] = qz.v;
+        o.hgR = qr.v;
+        o.hgZ = qz.v;
         nft::store(t.thg, 2 * E, e, qr);
         nft::store(t.thg + E, 2 * E, e, qz);
         g0r = g0r - y0 * qr; g1r = g1r - y1 * qr; g2r = g2r - y2 * qr; g3r = g3r - y3 * qr;
         g0z = g0z - y0 * qz; g1z = g1z - y1 * qz; g2z = g2z - y2 * qz; g3z = g3z - y3 * qz;
-        d.hgK[e] = k.v;
+        o.hgK = k.v;
     }
     const D gr[4] = {g0r, g1r, g2r, g3r}, gz[4] = {g0z, g1z, g2z, g3z};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        d.slot[4 * e + k] = make_double2(gr[k].v, gz[k].v);
+        o.slot[k] = make_double2(gr[k].v, gz[k].v);
         nft::store2(t.tslot, 4 * E, 4 * (size_t)e + k, gr[k], gz[k]);
     }
-    t.edgeVal[e] = minEdge.v;
+    o.edge = minEdge.v;
     nft::store(t.tedge, E, e, minEdge);
 }
 
-__global__ void __launch_bounds__(BLOCK, 2) k_t_element(Dev d, TDev t, int q, int mode) {
+// Two threads per element, in the two halves of the workgroup: the first carries directions
+// 0-2, the second 3-5. Both read the element's value state and compute the same values; after
+// a barrier, which every thread reaches, only the first stores them.
+__global__ void __launch_bounds__(BLOCK) k_t_element(Dev d, TDev t, int q, int mode) {
     Ctl* c = d.ctl;
     Step& st = c->s[q];
     if (mode == MODE_STEP && !st.go) return;
     __shared__ double sLog[97][3];
     __shared__ double sExp[64][2];
     const nfm::Tables T = nfd::stage_tables(sLog, sExp);
+    const int half = threadIdx.x / (BLOCK / 2);
+    const int dir0 = half * NL;
+    const TDev th = nft::shifted(t, dir0, d.N, d.E, d.P);
     nft::D strainDt = nft::cst(0.0);
-    if (mode != MODE_INITIAL) strainDt = nft::load(t.tstep + (size_t)q * 2 * NP, 1, 0, st.dt);
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (mode != MODE_INITIAL) strainDt = nft::load(th.tstep + (size_t)q * 2 * NP, 1, 0, st.dt);
+    const int e = blockIdx.x * (BLOCK / 2) + threadIdx.x % (BLOCK / 2);
+    const bool active = e < d.E;
     bool sawNaN = false;
     double edge = NFD_MAX_VALUE;
-    if (e < d.E) {
-        tangent_element(d, t, e, d.conn[e], strainDt, T);
-        edge = t.edgeVal[e];
+    TElementOut o{};
+    if (active) {
+        tangent_element(d, th, e, d.conn[e], strainDt, T, dir0, half == 0 ? t.argmin + 2 : nullptr, o);
+        edge = o.edge;
         sawNaN = edge != edge;
+    }
+    __syncthreads();
+    if (active && half == 0) {
+        d.sig0[e] = o.sig0;
+        d.sig1[e] = o.sig1;
+        d.sig2[e] = o.sig2;
+        d.sig3[e] = o.sig3;
+        d.epsP[e] = o.epsP;
+        d.work[e] = o.work;
+        if (d.qhg > 0.0) {
+            d.hgR[e] = o.hgR;
+            d.hgZ[e] = o.hgZ;
+            d.hgK[e] = o.hgK;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) d.slot[4 * e + k] = o.slot[k];
+        t.edgeVal[e] = o.edge;
     }
     unsigned long long bits = sawNaN ? MAX_BITS : (unsigned long long)nfm::bits_of(edge);
     for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(bits, off, 64);
-        bits = o < bits ? o : bits;
+        const unsigned long long o2 = __shfl_xor(bits, off, 64);
+        bits = o2 < bits ? o2 : bits;
     }
     const unsigned long long anyNaN = __ballot(sawNaN);
     if ((threadIdx.x & 63) == 0) {
@@ -531,20 +589,30 @@ __global__ void __launch_bounds__(BLOCK) k_t_argmin(Dev d, TDev t, int q) {
 }
 
 // k_gn (PART_ALL) over Duals, without the wall's audit partials.
+// Two threads per node, as in k_t_element: directions 0-2 and 3-5. The node's value state is
+// read by both and written by the first after a barrier.
 template <bool WALL>
 __global__ void __launch_bounds__(BLOCK) k_t_gn(Dev d, TDev t, int q, int phase, int gatherFirst) {
     using nft::D;
     Ctl* c = d.ctl;
     const Step& in = c->s[q];
     Step& out = c->s[q ^ 1];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const double* tin = t.tstep + (size_t)q * 2 * NP;
-    double* tout = t.tstep + (size_t)(q ^ 1) * 2 * NP;
+    const int half = threadIdx.x / (BLOCK / 2);
+    const int dir0 = half * NL;
+    const TDev th = nft::shifted(t, dir0, d.N, d.E, d.P);
+    const bool lead = blockIdx.x == 0 && threadIdx.x % (BLOCK / 2) == 0;  // one per half
+    const double* tin = th.tstep + (size_t)q * 2 * NP;
+    double* tout = th.tstep + (size_t)(q ^ 1) * 2 * NP;
     if (phase == PHASE_STEP && !in.go) {
         if (lead) {
-            out = in;
-            for (int k = 0; k < 2 * NP; k++) tout[k] = tin[k];
-            t.argmin[q ^ 1] = 0x7FFFFFFF;
+            if (half == 0) {
+                out = in;
+                t.argmin[q ^ 1] = 0x7FFFFFFF;
+            }
+            for (int k = 0; k < NL; k++) {
+                tout[k] = tin[k];
+                tout[NP + k] = tin[NP + k];
+            }
         }
         return;
     }
@@ -557,7 +625,7 @@ __global__ void __launch_bounds__(BLOCK) k_t_gn(Dev d, TDev t, int q, int phase,
             D minEdge = nft::cst(NFD_MAX_VALUE);
             if (!in.nanEdge && in.minEdgeBits != MAX_BITS) {
                 const int am = t.argmin[q];
-                minEdge = nft::load(t.tedge, d.E, am, nfm::from_bits(in.minEdgeBits));
+                minEdge = nft::load(th.tedge, d.E, am, nfm::from_bits(in.minEdgeBits));
             }
             dtPrev = dt;
             dt = d.cfl * nft::sqrt(minEdge * d.massScale) / d.fastestWave;
@@ -567,100 +635,108 @@ __global__ void __launch_bounds__(BLOCK) k_t_gn(Dev d, TDev t, int q, int phase,
         next = false;
     }
     if (lead) {
-        Step o = in;
-        o.dt = dt.v;
-        o.dtPrev = dtPrev.v;
-        o.go = next ? 1 : 0;
-        if (next) {
-            o.time = in.time + dt.v;
-            o.steps = in.steps + 1;
+        if (half == 0) {
+            Step o = in;
+            o.dt = dt.v;
+            o.dtPrev = dtPrev.v;
+            o.go = next ? 1 : 0;
+            if (next) {
+                o.time = in.time + dt.v;
+                o.steps = in.steps + 1;
+            }
+            o.nanEdge = 0;
+            o.minEdgeBits = MAX_BITS;
+            out = o;
+            t.argmin[q ^ 1] = 0x7FFFFFFF;
         }
-        o.nanEdge = 0;
-        o.minEdgeBits = MAX_BITS;
-        out = o;
-        for (int k = 0; k < NP; k++) {
+        for (int k = 0; k < NL; k++) {
             tout[k] = dt.t[k];
             tout[NP + k] = dtPrev.t[k];
         }
-        t.argmin[q ^ 1] = 0x7FFFFFFF;
     }
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= d.N) return;
+    const int i = blockIdx.x * (BLOCK / 2) + threadIdx.x % (BLOCK / 2);
+    const bool active = i < d.N;
     const size_t N = d.N, E4 = 4 * (size_t)d.E;
-    D fr, fz;
-    if (gather) {
-        fr = nft::cst(0.0);
-        fz = nft::cst(0.0);
-        for (int k = d.slotStart[i]; k < d.slotStart[i + 1]; k++) {
-            const int s = d.slotOf[k];
-            D sr, sz;
-            nft::load2(t.tslot, E4, s, d.slot[s], sr, sz);
-            fr = fr + sr;
-            fz = fz + sz;
+    D fr = nft::cst(0.0), fz = nft::cst(0.0);
+    if (active) {
+        if (gather) {
+            for (int k = d.slotStart[i]; k < d.slotStart[i + 1]; k++) {
+                const int s = d.slotOf[k];
+                D sr, sz;
+                nft::load2(th.tslot, E4, s, d.slot[s], sr, sz);
+                fr = fr + sr;
+                fz = fz + sz;
+            }
+            const double m = d.mass[i];
+            fr = fr + m * d.gravityR;
+            fz = fz + m * d.gravityZ;
+            if (half == 0) d.f[i] = make_double2(fr.v, fz.v);  // read by no other thread here
+            nft::store2(th.tf, N, i, fr, fz);
+        } else {
+            nft::load2(th.tf, N, i, d.f[i], fr, fz);
         }
-        const double m = d.mass[i];
-        fr = fr + m * d.gravityR;
-        fz = fz + m * d.gravityZ;
-        d.f[i] = make_double2(fr.v, fz.v);
-        nft::store2(t.tf, N, i, fr, fz);
-    } else {
-        nft::load2(t.tf, N, i, d.f[i], fr, fz);
     }
-    if (!next) return;
+    if (!next) return;  // uniform over the grid
     const D dtAvg = 0.5 * (dtPrev + dt);
-    const D half = 0.5 * d.damping * dtAvg;
-    const D decay = (1.0 - half) / (1.0 + half);
-    const D gain = dtAvg / (1.0 + half);
+    const D half_ = 0.5 * d.damping * dtAvg;
+    const D decay = (1.0 - half_) / (1.0 + half_);
+    const D gain = dtAvg / (1.0 + half_);
 
     // nodal_node over Duals (solver/ExplicitSolver.java:998-1062, solver/RigidWall.java:163-279)
-    const double m = d.mass[i];
-    const double invM = 1.0 / m;
-    uint8_t fl = d.flags[i];
-    D vr, vz, ur, uz;
-    nft::load2(t.tv, N, i, d.v[i], vr, vz);
-    nft::load2(t.tu, N, i, d.u[i], ur, uz);
-    const bool resting = WALL && (fl & ON_WALL) && fz.v <= 0.0;
-    if (fl & FIXED_R) {
-        vr = nft::cst(0.0);
-    } else if (WALL && resting && d.wallFrictional) {
-        const D normal = -fz;
-        const D trial = -m * decay * vr / gain - fr;
-        D limit = nft::cst(HUGE_VAL);
-        if (!__builtin_isinf(d.friction)) limit = nft::seed(d.friction, SP_FRICTION) * normal;
-        if (nfd::jabs(trial.v) <= limit.v) {
+    D vr = nft::cst(0.0), vz = nft::cst(0.0), ur = nft::cst(0.0), uz = nft::cst(0.0);
+    uint8_t fl = 0;
+    if (active) {
+        const double m = d.mass[i];
+        const double invM = 1.0 / m;
+        fl = d.flags[i];
+        nft::load2(th.tv, N, i, d.v[i], vr, vz);
+        nft::load2(th.tu, N, i, d.u[i], ur, uz);
+        const bool resting = WALL && (fl & ON_WALL) && fz.v <= 0.0;
+        if (fl & FIXED_R) {
             vr = nft::cst(0.0);
+        } else if (WALL && resting && d.wallFrictional) {
+            const D normal = -fz;
+            const D trial = -m * decay * vr / gain - fr;
+            D limit = nft::cst(HUGE_VAL);
+            if (!__builtin_isinf(d.friction)) limit = nft::seed(d.friction, SP_FRICTION, dir0) * normal;
+            if (nfd::jabs(trial.v) <= limit.v) {
+                vr = nft::cst(0.0);
+            } else {
+                D tangential = limit;
+                tangential.v = __builtin_copysign(limit.v, trial.v);
+                if (__builtin_signbit(trial.v))
+                    for (int k = 0; k < NL; k++) tangential.t[k] = -limit.t[k];
+                vr = decay * vr + gain * (fr + tangential) / m;
+            }
+            ur = ur + vr * dt;
         } else {
-            D tangential = limit;
-            tangential.v = __builtin_copysign(limit.v, trial.v);
-            if (__builtin_signbit(trial.v))
-                for (int k = 0; k < NP; k++) tangential.t[k] = -limit.t[k];
-            vr = decay * vr + gain * (fr + tangential) / m;
+            vr = decay * vr + gain * fr * invM;
+            ur = ur + vr * dt;
         }
-        ur = ur + vr * dt;
-    } else {
-        vr = decay * vr + gain * fr * invM;
-        ur = ur + vr * dt;
-    }
-    if (fl & FIXED_Z) {
-        vz = nft::cst(0.0);
-    } else if (WALL && resting) {
-        // held: nothing moves
-    } else {
-        const D trial = decay * vz + gain * fz * invM;
-        const D zc = d.z0[i] + uz;
-        if (WALL && zc.v + trial.v * dt.v < d.plane) {
-            uz = uz + (d.plane - zc);
+        if (fl & FIXED_Z) {
             vz = nft::cst(0.0);
-            fl |= ON_WALL;
+        } else if (WALL && resting) {
+            // held: nothing moves
         } else {
-            vz = trial;
-            uz = uz + vz * dt;
-            if (WALL) fl &= (uint8_t)~ON_WALL;
+            const D trial = decay * vz + gain * fz * invM;
+            const D zc = d.z0[i] + uz;
+            if (WALL && zc.v + trial.v * dt.v < d.plane) {
+                uz = uz + (d.plane - zc);
+                vz = nft::cst(0.0);
+                fl |= ON_WALL;
+            } else {
+                vz = trial;
+                uz = uz + vz * dt;
+                if (WALL) fl &= (uint8_t)~ON_WALL;
+            }
         }
+        nft::store2(th.tv, N, i, vr, vz);
+        nft::store2(th.tu, N, i, ur, uz);
     }
-    d.v[i] = make_double2(vr.v, vz.v);
-    d.u[i] = make_double2(ur.v, uz.v);
-    nft::store2(t.tv, N, i, vr, vz);
-    nft::store2(t.tu, N, i, ur, uz);
-    if (WALL) d.flags[i] = fl;
+    __syncthreads();  // both threads of a node have read its values
+    if (active && half == 0) {
+        d.v[i] = make_double2(vr.v, vz.v);
+        d.u[i] = make_double2(ur.v, uz.v);
+        if (WALL) d.flags[i] = fl;
+    }
 }
