@@ -151,6 +151,19 @@ __device__ __forceinline__ bool c_skip(const Dev& d, int q, int checkGo) {
 // adopt() :290-322 when the contact has forgotten elements since the last rebuild. The
 // counters were zeroed by whoever raised the stale flag. Every assembly: the grid's
 // reductions are reset here, ahead of k_c_geometry.
+// Appends the lanes whose `take` is set to a compact list with one atomic per wave; returns
+// this lane's slot, or -1. Lanes that have left the caller's loop take no part.
+__device__ __forceinline__ int wave_append(bool take, int* counter) {
+    const unsigned long long mask = __ballot(take);
+    if (mask == 0ULL) return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    return take ? base + __popcll(mask & ((1ULL << lane) - 1ULL)) : -1;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) {
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
@@ -169,10 +182,8 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
     for (int p = blockIdx.x * BLOCK + threadIdx.x; p < 4 * d.E; p += stride) {
         const bool live = seg_live(d, p);
         d.cActive[p] = live ? 1 : 0;
-        if (live) {
-            const int at = atomicAdd(&c->cNAct, 1);
-            if (at < 4 * d.E) d.cAct[at] = p;
-        }
+        const int at = wave_append(live, &c->cNAct);
+        if (at >= 0 && at < 4 * d.E) d.cAct[at] = p;
     }
 }
 
@@ -188,10 +199,8 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild_nodes(Dev d, int q, int che
         d.cSurfFlag[n] = on ? 1 : 0;
         d.cPairOk[n] = 0;
         d.cLast[n] = -1;
-        if (on) {
-            const int at = atomicAdd(&c->cNSurf, 1);
-            if (at < d.N) d.cSurf[at] = n;
-        }
+        const int at = wave_append(on, &c->cNSurf);
+        if (at >= 0 && at < d.N) d.cSurf[at] = n;
     }
 }
 
@@ -214,6 +223,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo)
         const double len = __builtin_sqrt(dr * dr + dz * dz);
         d.cLen[p] = len;
         d.cNrm[p] = len > 0.0 ? make_double2(dz / len, -dr / len) : make_double2(0.0, 0.0);
+        d.cSegX[p] = make_double4(xa.x, xa.y, xb.x, xb.y);
         d.cPairHead[p] = -1;
         // Both ends of every live segment are exactly the surface nodes. NaN never wins a
         // comparison in the reference's loops, so it is left out here.
@@ -307,16 +317,19 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
             atomicOr(&c->cFOverflow, 1);
             continue;
         }
+        // One reservation per segment for all its cells (the order of a cell's list is
+        // immaterial: the search ranks candidates by their walk key).
+        const int count = (i1 - i0 + 1) * (j1 - j0 + 1);
+        const int base = atomicAdd(&c->cFCount, count);
+        if (base < 0 || (long long)base + count > d.cFCap) {  // full (or the counter wrapped)
+            atomicOr(&c->cFOverflow, 1);
+            continue;
+        }
+        int at = base;
         for (int j = j0; j <= j1; j++)
-            for (int i = i0; i <= i1; i++) {
-                const int at = atomicAdd(&c->cFCount, 1);
-                if (at >= d.cFCap) {
-                    atomicOr(&c->cFOverflow, 1);
-                    continue;
-                }
-                d.cFSeg[at] = p;
-                d.cFCell[at] = make_int2(i, j);
-                d.cFNext[at] = atomicExch(&d.cFHead[c_bucket(i, j, fmask)], at);
+            for (int i = i0; i <= i1; i++, at++) {
+                const int next = atomicExch(&d.cFHead[c_bucket(i, j, fmask)], at);
+                d.cFEnt[at] = make_int4(next, p, i, j);
             }
     }
 }
@@ -336,11 +349,10 @@ __device__ __forceinline__ bool c_excluded(const Dev& d, int n, double2 pn, int 
     return false;
 }
 
-// depth() :639-649
-__device__ __forceinline__ double c_depth(const Dev& d, double2 xn, int p, int a, int b) {
+// depth() :639-649, with the segment's end positions given.
+__device__ __forceinline__ double c_depth_at(const Dev& d, double2 xn, int p, double2 xa, double2 xb) {
     const double len = d.cLen[p];
     if (len <= 0.0) return 0.0;
-    const double2 xa = cpos(d, a), xb = cpos(d, b);
     const double tr = (xb.x - xa.x) / len, tz = (xb.y - xa.y) / len;
     const double dr = xn.x - xa.x, dz = xn.y - xa.y;
     const double along = dr * tr + dz * tz;
@@ -348,6 +360,10 @@ __device__ __forceinline__ double c_depth(const Dev& d, double2 xn, int p, int a
     const double2 ns = d.cNrm[p];
     const double gap = dr * ns.x + dz * ns.y;
     return gap < 0.0 ? -gap : 0.0;
+}
+
+__device__ __forceinline__ double c_depth(const Dev& d, double2 xn, int p, int a, int b) {
+    return c_depth_at(d, xn, p, cpos(d, a), cpos(d, b));
 }
 
 // nearestPenetrated :570-606 and accumulate :446-462 for one surface node, with resolve
@@ -376,9 +392,12 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
         // One candidate as the reference's loop body treats it (:585-602). The reference
         // tests the exclusions first, then the depth; both are pure, and a candidate is kept
         // only if it passes both, so testing the cheaper depth first selects the same ones.
-        auto consider = [&](int p, int a, int b, unsigned long long key) {
-            const double dd = c_depth(d, xn, p, a, b);
+        // xa, xb: the segment's current end positions (cSegX, or cpos where that is stale).
+        auto consider = [&](int p, double2 xa, double2 xb, unsigned long long key) {
+            const double dd = c_depth_at(d, xn, p, xa, xb);
             if (dd <= 0.0) return;
+            int a, b;
+            seg_ends(d, p, a, b);
             if (c_excluded(d, n, pn, p, a, b)) return;
             if (dd > C_MAX_DEPTH * d.cLen[p]) {
                 if (p == last) fell = true;
@@ -392,11 +411,11 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
         };
         // Where segment p's first entry falls in the reference's walk over the nine cells
         // around (ci, cj): (cell rank, entry index 2p + end), or ~0 if neither end is there.
-        auto walkKey = [&](int p, int a, int b) {
+        auto walkKey = [&](int p, double2 xa, double2 xb) {
             unsigned long long key = ~0ULL;
-            const int ends[2] = {a, b};
+            const double2 ends[2] = {xa, xb};
             for (int e = 0; e < 2; e++) {
-                const double2 x = cpos(d, ends[e]);
+                const double2 x = ends[e];
                 const int di = clampi(jint((x.x - g.r0) / g.cell), g.cols) - ci;
                 const int dj = clampi(jint((x.y - g.z0) / g.cell), g.rows) - cj;
                 if (di < -1 || di > 1 || dj < -1 || dj > 1) continue;
@@ -410,20 +429,22 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
             // The node's capture cell lists every segment that can be its master; each is
             // considered only if the reference's walk reaches it, at its walk position.
             const int fi = c_fine(xn.x, g.r0, d.cFine), fj = c_fine(xn.y, g.z0, d.cFine);
-            for (int x = d.cFHead[c_bucket(fi, fj, fmask)]; x >= 0; x = d.cFNext[x]) {
-                const int2 cell = d.cFCell[x];
-                if (cell.x != fi || cell.y != fj) continue;
-                const int p = d.cFSeg[x];
-                int a, b;
-                seg_ends(d, p, a, b);
-                const unsigned long long key = walkKey(p, a, b);
-                if (key != ~0ULL) consider(p, a, b, key);
+            for (int x = d.cFHead[c_bucket(fi, fj, fmask)]; x >= 0;) {
+                const int4 ent = d.cFEnt[x];
+                x = ent.x;
+                if (ent.z != fi || ent.w != fj) continue;
+                const int p = ent.y;
+                const double4 sx = d.cSegX[p];  // every listed segment is live this step
+                const double2 xa = make_double2(sx.x, sx.y), xb = make_double2(sx.z, sx.w);
+                const unsigned long long key = walkKey(p, xa, xb);
+                if (key != ~0ULL) consider(p, xa, xb, key);
             }
-            // The previous master may have gone too deep to be captured: the drop test.
+            // The previous master may have gone too deep to be captured: the drop test. It
+            // may no longer be live, so its ends are read from the nodes.
             if (last >= 0 && !fell) {
                 int a, b;
                 seg_ends(d, last, a, b);
-                if (walkKey(last, a, b) != ~0ULL) {
+                if (walkKey(last, cpos(d, a), cpos(d, b)) != ~0ULL) {
                     const double dd = c_depth(d, xn, last, a, b);
                     if (dd > 0.0 && dd > C_MAX_DEPTH * d.cLen[last] && !c_excluded(d, n, pn, last, a, b)) fell = true;
                 }
@@ -441,9 +462,9 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
                         const int2 cell = d.cEntCell[x];
                         if (cell.x != i || cell.y != j) continue;
                         const int p = x >> 1;
-                        int a, b;
-                        seg_ends(d, p, a, b);
-                        consider(p, a, b, rank | (unsigned long long)x);
+                        const double4 sx = d.cSegX[p];
+                        consider(p, make_double2(sx.x, sx.y), make_double2(sx.z, sx.w),
+                                 rank | (unsigned long long)x);
                     }
                 }
             }
@@ -460,9 +481,10 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
         // resolve :660-762
         int a, b;
         seg_ends(d, best, a, b);
-        const double dep = c_depth(d, xn, best, a, b);
+        const double4 bx = d.cSegX[best];
+        const double2 xa = make_double2(bx.x, bx.y), xb = make_double2(bx.z, bx.w);
+        const double dep = c_depth_at(d, xn, best, xa, xb);
         const double len = d.cLen[best];
-        const double2 xa = cpos(d, a), xb = cpos(d, b);
         const double tr0 = (xb.x - xa.x) / len, tz0 = (xb.y - xa.y) / len;
         const double tpar = ((xn.x - xa.x) * tr0 + (xn.y - xa.y) * tz0) / len;  // parameter :652-658
         const double wa = 1.0 - tpar, wb = tpar;

@@ -164,9 +164,8 @@ struct Dev {
     // The capture grid: square cells of side cFine, each listing every segment whose capture
     // rectangle (the region where it can be a node's master) overlaps the cell.
     int* cFHead;             // [cFHashMax] hash heads
-    int* cFNext;             // [cFCap]
-    int* cFSeg;              // [cFCap] segment p of the entry
-    int2* cFCell;            // [cFCap] its cell
+    int4* cFEnt;             // [cFCap] {next entry, segment p, cell i, cell j}: one load a link
+    double4* cSegX;          // [4E] current end positions of each live segment (a.r, a.z, b.r, b.z)
     int cFCap;
     unsigned cFHashMax;
     double cFine;
@@ -1907,9 +1906,8 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
         d.cFHashMax = 64;
         while (d.cFHashMax < 16u * (unsigned)E && d.cFHashMax < (1u << 28)) d.cFHashMax <<= 1;
         TRY(s->alloc(&d.cFHead, d.cFHashMax));
-        TRY(s->alloc(&d.cFNext, (size_t)d.cFCap));
-        TRY(s->alloc(&d.cFSeg, (size_t)d.cFCap));
-        TRY(s->alloc(&d.cFCell, (size_t)d.cFCap));
+        TRY(s->alloc(&d.cFEnt, (size_t)d.cFCap));
+        TRY(s->alloc(&d.cSegX, 4 * (size_t)E));
         // Capture cells twice the median surface segment of the reference mesh.
         {
             std::vector<double> lens;
