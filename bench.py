@@ -52,7 +52,7 @@ def element_kernel_bytes(E, N, P):
 
 
 def step_bytes(E, N):
-    """Algorithmic bytes of one whole step (element + gather + nodal), the survey's
+    """Algorithmic bytes of one whole step (element + gather + nodal):
     B = 352 + 160 N/E per element-step for reduced integration with a rigid wall."""
     return 352.0 * E + 160.0 * N
 
@@ -276,7 +276,7 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; a multi-GPU figure is not "
                                    "produced (ensembles shard across GPUs with no "
-                                   "communication; see DESIGN.md)"}))
+                                   "communication)"}))
         sys.exit(2)
 
     from neofiz_amd import taylor

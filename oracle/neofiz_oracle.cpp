@@ -1,7 +1,7 @@
 // TEST INFRASTRUCTURE -- the oracle. Nothing in the product links, loads or calls this file.
 //
 // A CPU restatement of Neofiz's explicit step loop, written line by line from the Java
-// reference (paths relative to src/main/java/org/neofiz/). It is the checker the HIP path is
+// reference (paths relative to src/main/java/org/neofiz/). It is what the HIP path is
 // compared against, and -- through its threaded build -- the CPU baseline timed beside it.
 // It exports the same C ABI as the product (include/neofiz.h) under the prefix nfo_.
 //

@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE -- Python binding of the oracle (libneofiz_oracle.so, prefix nfo_).
 
-Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this, and only as
-the checker. It exposes the same ExplicitSolver surface as the product over the oracle's
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this, and only to
+check results against. It exposes the same ExplicitSolver surface as the product over the oracle's
 C ABI.
 """
 import ctypes
