@@ -82,11 +82,27 @@ __device__ __forceinline__ double2 cpos(const Dev& d, int n) {
 
 // p is on the surface of the elements the contact still knows: its element is live and no
 // other live element uses the same edge (QuadMesh.surface, mesh/QuadMesh.java:188-226).
+// Every load is issued without waiting on an earlier test (an edge of a quad mesh has at most
+// one partner, so that case is a fixed-length step the compiler can schedule with others),
+// so a segment costs the three dependent loads of the CSR walk.
 __device__ __forceinline__ bool seg_live(const Dev& d, int p) {
-    if (d.cDead[p >> 2]) return false;
-    for (int k = d.cShareStart[p]; k < d.cShareStart[p + 1]; k++)
-        if (!d.cDead[d.cShareOf[k] >> 2]) return false;
-    return true;
+    bool live = d.cDead[p >> 2] == 0;
+    const int k0 = d.cShareStart[p], k1 = d.cShareStart[p + 1];
+    if (k0 < k1) live &= d.cDead[d.cShareOf[k0] >> 2] != 0;
+    for (int k = k0 + 1; k < k1; k++) live &= d.cDead[d.cShareOf[k] >> 2] != 0;
+    return live;
+}
+
+// Whether node n touches a live segment: its segments (eight for an interior node of a quad
+// mesh) tested together, the first eight as a fixed-length step.
+__device__ __forceinline__ bool node_on_surface(const Dev& d, int n) {
+    const int k0 = d.cNodeSegStart[n], k1 = d.cNodeSegStart[n + 1];
+    bool on = false;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (k0 + k < k1) on |= d.cActive[d.cNodeSegOf[k0 + k]] != 0;
+    for (int k = k0 + 8; k < k1; k++) on |= d.cActive[d.cNodeSegOf[k]] != 0;
+    return on;
 }
 
 struct CGrid {
@@ -148,22 +164,51 @@ __device__ __forceinline__ bool c_skip(const Dev& d, int q, int checkGo) {
 
 // ------------------------------------------------------------------ launches
 
+// Exclusive prefix of v over the workgroup in thread order, and the workgroup's total.
+__device__ __forceinline__ int block_exclusive(int v, int* lds, int& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int w = 0; w < BLOCK / 64; w++) {
+        if (w < wave) before += lds[w];
+        total += lds[w];
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
+// The rebuild launches give each thread C_CHUNK items (strided by the workgroup, so loads and
+// flag stores stay coalesced) and reserve a workgroup's appends with one atomic: a counter
+// shared by the whole grid serialises its atomics, so they are kept to a few hundred.
+constexpr int C_CHUNK = 4;
+
+// Appends this thread's items (bit j of mask: item first + j * BLOCK) to list[0, cap) through
+// counter, one reservation per workgroup.
+__device__ __forceinline__ void block_append(unsigned mask, int first, int* counter, int* list, int cap) {
+    __shared__ int lds[BLOCK / 64];
+    __shared__ int ldsBase;
+    int total;
+    int at = block_exclusive(__popc(mask), lds, total);
+    if (threadIdx.x == 0) ldsBase = total ? atomicAdd(counter, total) : 0;
+    __syncthreads();
+    at += ldsBase;
+    for (int j = 0; j < C_CHUNK; j++)
+        if ((mask >> j) & 1u) {
+            if (at >= 0 && at < cap) list[at] = first + j * BLOCK;
+            at++;
+        }
+}
+
 // adopt() :290-322 when the contact has forgotten elements since the last rebuild. The
 // counters were zeroed by whoever raised the stale flag. Every assembly: the grid's
 // reductions are reset here, ahead of k_c_geometry.
-// Appends the lanes whose `take` is set to a compact list with one atomic per wave; returns
-// this lane's slot, or -1. Lanes that have left the caller's loop take no part.
-__device__ __forceinline__ int wave_append(bool take, int* counter) {
-    const unsigned long long mask = __ballot(take);
-    if (mask == 0ULL) return -1;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader, 64);
-    return take ? base + __popcll(mask & ((1ULL << lane) - 1ULL)) : -1;
-}
-
 __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) {
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
@@ -178,13 +223,19 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
         c->cFOverflow = 0;
     }
     if (!c->surfaceStale) return;
-    const int stride = gridDim.x * BLOCK;
-    for (int p = blockIdx.x * BLOCK + threadIdx.x; p < 4 * d.E; p += stride) {
-        const bool live = seg_live(d, p);
-        d.cActive[p] = live ? 1 : 0;
-        const int at = wave_append(live, &c->cNAct);
-        if (at >= 0 && at < 4 * d.E) d.cAct[at] = p;
+    const int E4 = 4 * d.E;
+    const int first = blockIdx.x * BLOCK * C_CHUNK + threadIdx.x;
+    unsigned mask = 0;
+#pragma unroll
+    for (int j = 0; j < C_CHUNK; j++) {
+        const int p = first + j * BLOCK;
+        if (p < E4) {
+            const bool live = seg_live(d, p);
+            d.cActive[p] = live ? 1 : 0;
+            mask |= (live ? 1u : 0u) << j;
+        }
     }
+    block_append(mask, first, &c->cNAct, d.cAct, E4);
 }
 
 // The second half of the rebuild: surface nodes from the live flags just written.
@@ -192,16 +243,20 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild_nodes(Dev d, int q, int che
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
     if (!c->surfaceStale) return;
-    const int stride = gridDim.x * BLOCK;
-    for (int n = blockIdx.x * BLOCK + threadIdx.x; n < d.N; n += stride) {
-        bool on = false;
-        for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1] && !on; k++) on = d.cActive[d.cNodeSegOf[k]] != 0;
-        d.cSurfFlag[n] = on ? 1 : 0;
-        d.cPairOk[n] = 0;
-        d.cLast[n] = -1;
-        const int at = wave_append(on, &c->cNSurf);
-        if (at >= 0 && at < d.N) d.cSurf[at] = n;
+    const int first = blockIdx.x * BLOCK * C_CHUNK + threadIdx.x;
+    unsigned mask = 0;
+#pragma unroll
+    for (int j = 0; j < C_CHUNK; j++) {
+        const int n = first + j * BLOCK;
+        if (n < d.N) {
+            const bool on = node_on_surface(d, n);
+            d.cSurfFlag[n] = on ? 1 : 0;
+            d.cPairOk[n] = 0;
+            d.cLast[n] = -1;
+            mask |= (on ? 1u : 0u) << j;
+        }
     }
+    block_append(mask, first, &c->cNSurf, d.cSurf, d.N);
 }
 
 // geometry() :465-487 per live segment; the grid's box and longest segment; clears.
@@ -242,6 +297,10 @@ __global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo)
             longest = lb > longest ? lb : longest;
         }
     }
+    // Wave shuffles, then the workgroup's waves through LDS: one atomic per workgroup and
+    // key (the keys share a line, so the grid's atomics on them serialise).
+    __shared__ unsigned long long ldsKey[BLOCK / 64][5];
+    const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         unsigned long long v = box[k];
@@ -249,13 +308,24 @@ __global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo)
             const unsigned long long o = __shfl_xor(v, off, 64);
             v = o < v ? o : v;
         }
-        if ((threadIdx.x & 63) == 0 && v != ~0ULL) atomicMin(&c->cBox[k], v);
+        if ((threadIdx.x & 63) == 0) ldsKey[wave][k] = v;
     }
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(longest, off, 64);
         longest = o > longest ? o : longest;
     }
-    if ((threadIdx.x & 63) == 0 && longest) atomicMax(&c->cLongest, longest);
+    if ((threadIdx.x & 63) == 0) ldsKey[wave][4] = longest;
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        const int k = threadIdx.x;
+        unsigned long long v = ldsKey[0][k];
+        for (int w = 1; w < BLOCK / 64; w++) {
+            const unsigned long long o = ldsKey[w][k];
+            v = k < 4 ? (o < v ? o : v) : (o > v ? o : v);
+        }
+        if (k < 4 && v != ~0ULL) atomicMin(&c->cBox[k], v);
+        if (k == 4 && v) atomicMax(&c->cLongest, v);
+    }
     const unsigned H = c_hash_size(nAct);
     for (unsigned h = t0; h < H; h += stride) d.cHead[h] = -1;
     const unsigned HF = c_capture_size(d, nAct);
@@ -300,37 +370,49 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
     // (depth() :639-649 with the MAX_DEPTH cut). That rectangle lies inside the segment's
     // bounding box grown by len / 2 (a hair more, for rounding), and the segment is listed
     // in every cell that box overlaps, so the node's own cell lists every candidate.
+    // Entries are reserved once per workgroup round (the order of a cell's list is
+    // immaterial: the search ranks candidates by their walk key); every thread of a
+    // workgroup runs the same rounds, so the reservation's barriers are uniform.
     const unsigned fmask = c_capture_size(d, nAct) - 1u;
-    for (int t = t0; t < nAct; t += stride) {
-        const int p = d.cAct[t];
-        const double len = d.cLen[p];
-        if (!(len > 0.0) || len > NFD_MAX_VALUE) continue;  // such a segment never wins
-        int a, b;
-        seg_ends(d, p, a, b);
-        const double2 xa = cpos(d, a), xb = cpos(d, b);
-        const double m = 0.5 * len * (1.0 + 1.0e-6) + 1.0e-12;
-        const int i0 = c_fine(nfd::jmin(xa.x, xb.x) - m, g.r0, d.cFine);
-        const int i1 = c_fine(nfd::jmax(xa.x, xb.x) + m, g.r0, d.cFine);
-        const int j0 = c_fine(nfd::jmin(xa.y, xb.y) - m, g.z0, d.cFine);
-        const int j1 = c_fine(nfd::jmax(xa.y, xb.y) + m, g.z0, d.cFine);
-        if ((long long)(i1 - i0 + 1) * (j1 - j0 + 1) > C_MAX_CAPTURE_CELLS) {
-            atomicOr(&c->cFOverflow, 1);
-            continue;
-        }
-        // One reservation per segment for all its cells (the order of a cell's list is
-        // immaterial: the search ranks candidates by their walk key).
-        const int count = (i1 - i0 + 1) * (j1 - j0 + 1);
-        const int base = atomicAdd(&c->cFCount, count);
-        if (base < 0 || (long long)base + count > d.cFCap) {  // full (or the counter wrapped)
-            atomicOr(&c->cFOverflow, 1);
-            continue;
-        }
-        int at = base;
-        for (int j = j0; j <= j1; j++)
-            for (int i = i0; i <= i1; i++, at++) {
-                const int next = atomicExch(&d.cFHead[c_bucket(i, j, fmask)], at);
-                d.cFEnt[at] = make_int4(next, p, i, j);
+    __shared__ int lds[BLOCK / 64];
+    __shared__ int ldsBase;
+    for (int round = blockIdx.x * BLOCK; round < nAct; round += stride) {
+        const int t = round + threadIdx.x;
+        int p = -1, count = 0, i0 = 0, i1 = -1, j0 = 0;
+        if (t < nAct) {
+            p = d.cAct[t];
+            const double len = d.cLen[p];
+            if (len > 0.0 && len <= NFD_MAX_VALUE) {  // any other segment never wins
+                int a, b;
+                seg_ends(d, p, a, b);
+                const double2 xa = cpos(d, a), xb = cpos(d, b);
+                const double m = 0.5 * len * (1.0 + 1.0e-6) + 1.0e-12;
+                i0 = c_fine(nfd::jmin(xa.x, xb.x) - m, g.r0, d.cFine);
+                i1 = c_fine(nfd::jmax(xa.x, xb.x) + m, g.r0, d.cFine);
+                j0 = c_fine(nfd::jmin(xa.y, xb.y) - m, g.z0, d.cFine);
+                const int j1 = c_fine(nfd::jmax(xa.y, xb.y) + m, g.z0, d.cFine);
+                const long long cells = (long long)(i1 - i0 + 1) * (j1 - j0 + 1);
+                if (cells > C_MAX_CAPTURE_CELLS) atomicOr(&c->cFOverflow, 1);
+                else count = (int)cells;
             }
+        }
+        int total;
+        const int offset = block_exclusive(count, lds, total);
+        if (threadIdx.x == 0) ldsBase = total ? atomicAdd(&c->cFCount, total) : 0;
+        __syncthreads();
+        const int base = ldsBase;
+        __syncthreads();  // ldsBase is rewritten by the next round
+        if (count == 0) continue;
+        if (base < 0 || (long long)base + offset + count > d.cFCap) {  // full (or the counter wrapped)
+            atomicOr(&c->cFOverflow, 1);
+            continue;
+        }
+        const int width = i1 - i0 + 1;
+        for (int k = 0, at = base + offset; k < count; k++, at++) {
+            const int i = i0 + k % width, j = j0 + k / width;
+            const int next = atomicExch(&d.cFHead[c_bucket(i, j, fmask)], at);
+            d.cFEnt[at] = make_int4(next, p, i, j);
+        }
     }
 }
 
@@ -349,17 +431,19 @@ __device__ __forceinline__ bool c_excluded(const Dev& d, int n, double2 pn, int 
     return false;
 }
 
-// depth() :639-649, with the segment's end positions given.
-__device__ __forceinline__ double c_depth_at(const Dev& d, double2 xn, int p, double2 xa, double2 xb) {
-    const double len = d.cLen[p];
+// depth() :639-649, with the segment's end positions, length and normal given.
+__device__ __forceinline__ double c_depth_of(double2 xn, double2 xa, double2 xb, double len, double2 ns) {
     if (len <= 0.0) return 0.0;
     const double tr = (xb.x - xa.x) / len, tz = (xb.y - xa.y) / len;
     const double dr = xn.x - xa.x, dz = xn.y - xa.y;
     const double along = dr * tr + dz * tz;
     if (along < 0.0 || along > len) return 0.0;
-    const double2 ns = d.cNrm[p];
     const double gap = dr * ns.x + dz * ns.y;
     return gap < 0.0 ? -gap : 0.0;
+}
+
+__device__ __forceinline__ double c_depth_at(const Dev& d, double2 xn, int p, double2 xa, double2 xb) {
+    return c_depth_of(xn, xa, xb, d.cLen[p], d.cNrm[p]);
 }
 
 __device__ __forceinline__ double c_depth(const Dev& d, double2 xn, int p, int a, int b) {
@@ -393,13 +477,13 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
         // tests the exclusions first, then the depth; both are pure, and a candidate is kept
         // only if it passes both, so testing the cheaper depth first selects the same ones.
         // xa, xb: the segment's current end positions (cSegX, or cpos where that is stale).
-        auto consider = [&](int p, double2 xa, double2 xb, unsigned long long key) {
-            const double dd = c_depth_at(d, xn, p, xa, xb);
+        auto consider = [&](int p, double2 xa, double2 xb, double len, double2 ns, unsigned long long key) {
+            const double dd = c_depth_of(xn, xa, xb, len, ns);
             if (dd <= 0.0) return;
             int a, b;
             seg_ends(d, p, a, b);
             if (c_excluded(d, n, pn, p, a, b)) return;
-            if (dd > C_MAX_DEPTH * d.cLen[p]) {
+            if (dd > C_MAX_DEPTH * len) {
                 if (p == last) fell = true;
                 return;
             }
@@ -429,15 +513,25 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
             // The node's capture cell lists every segment that can be its master; each is
             // considered only if the reference's walk reaches it, at its walk position.
             const int fi = c_fine(xn.x, g.r0, d.cFine), fj = c_fine(xn.y, g.z0, d.cFine);
-            for (int x = d.cFHead[c_bucket(fi, fj, fmask)]; x >= 0;) {
-                const int4 ent = d.cFEnt[x];
-                x = ent.x;
-                if (ent.z != fi || ent.w != fj) continue;
-                const int p = ent.y;
-                const double4 sx = d.cSegX[p];  // every listed segment is live this step
-                const double2 xa = make_double2(sx.x, sx.y), xb = make_double2(sx.z, sx.w);
-                const unsigned long long key = walkKey(p, xa, xb);
-                if (key != ~0ULL) consider(p, xa, xb, key);
+            // The walk keeps the next link's load in flight while a candidate's own data
+            // (end positions, length, normal: three loads issued together) is read.
+            const int4 none = make_int4(-1, 0, 0, 0);
+            int x = d.cFHead[c_bucket(fi, fj, fmask)];
+            int4 ent = x >= 0 ? d.cFEnt[x] : none;
+            while (x >= 0) {
+                const int nx = ent.x;
+                const int4 nent = nx >= 0 ? d.cFEnt[nx] : none;
+                if (ent.z == fi && ent.w == fj) {
+                    const int p = ent.y;
+                    const double4 sx = d.cSegX[p];  // every listed segment is live this step
+                    const double len = d.cLen[p];
+                    const double2 ns = d.cNrm[p];
+                    const double2 xa = make_double2(sx.x, sx.y), xb = make_double2(sx.z, sx.w);
+                    const unsigned long long key = walkKey(p, xa, xb);
+                    if (key != ~0ULL) consider(p, xa, xb, len, ns, key);
+                }
+                x = nx;
+                ent = nent;
             }
             // The previous master may have gone too deep to be captured: the drop test. It
             // may no longer be live, so its ends are read from the nodes.
@@ -463,8 +557,8 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
                         if (cell.x != i || cell.y != j) continue;
                         const int p = x >> 1;
                         const double4 sx = d.cSegX[p];
-                        consider(p, make_double2(sx.x, sx.y), make_double2(sx.z, sx.w),
-                                 rank | (unsigned long long)x);
+                        consider(p, make_double2(sx.x, sx.y), make_double2(sx.z, sx.w), d.cLen[p],
+                                 d.cNrm[p], rank | (unsigned long long)x);
                     }
                 }
             }
