@@ -157,6 +157,9 @@ __device__ __forceinline__ int c_fine(double x, double origin, double h) {
 }
 
 constexpr int C_MAX_CAPTURE_CELLS = 1 << 16;  // per segment; beyond it the step falls back
+// Each capture bucket keeps C_SUB lists (an entry goes to list `entry mod C_SUB`), and the
+// search walks them with C_SUB lanes per surface node, so no lane follows a long chain.
+constexpr int C_SUB = 8;
 
 __device__ __forceinline__ bool c_skip(const Dev& d, int q, int checkGo) {
     return checkGo && !d.ctl->s[q].go;
@@ -328,7 +331,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo)
     }
     const unsigned H = c_hash_size(nAct);
     for (unsigned h = t0; h < H; h += stride) d.cHead[h] = -1;
-    const unsigned HF = c_capture_size(d, nAct);
+    const unsigned HF = c_capture_size(d, nAct) * C_SUB;
     for (unsigned h = t0; h < HF; h += stride) d.cFHead[h] = -1;
 }
 
@@ -410,7 +413,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
         const int width = i1 - i0 + 1;
         for (int k = 0, at = base + offset; k < count; k++, at++) {
             const int i = i0 + k % width, j = j0 + k / width;
-            const int next = atomicExch(&d.cFHead[c_bucket(i, j, fmask)], at);
+            const int next = atomicExch(&d.cFHead[c_bucket(i, j, fmask) * C_SUB + (at & (C_SUB - 1))], at);
             d.cFEnt[at] = make_int4(next, p, i, j);
         }
     }
@@ -462,8 +465,12 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
     const double dt = c->s[q].dt;
     const int stride = gridDim.x * BLOCK;
     long long escaped = 0;
-    for (int t = blockIdx.x * BLOCK + threadIdx.x; t < nSurf; t += stride) {
-        const int n = d.cSurf[t];
+    // C_SUB consecutive lanes per surface node (stride is a multiple of C_SUB, so a group
+    // stays together): lane `sub` walks list `sub` of the node's capture bucket, the group
+    // then keeps the best of its candidates, and its first lane carries on alone.
+    for (int t = blockIdx.x * BLOCK + threadIdx.x; t < C_SUB * nSurf; t += stride) {
+        const int sub = t & (C_SUB - 1);
+        const int n = d.cSurf[t / C_SUB];
         const double2 xn = cpos(d, n);
         const double2 pn = d.cPn[n];
         const int last = d.cLast[n];
@@ -516,7 +523,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
             // The walk keeps the next link's load in flight while a candidate's own data
             // (end positions, length, normal: three loads issued together) is read.
             const int4 none = make_int4(-1, 0, 0, 0);
-            int x = d.cFHead[c_bucket(fi, fj, fmask)];
+            int x = d.cFHead[c_bucket(fi, fj, fmask) * C_SUB + sub];
             int4 ent = x >= 0 ? d.cFEnt[x] : none;
             while (x >= 0) {
                 const int nx = ent.x;
@@ -533,17 +540,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
                 x = nx;
                 ent = nent;
             }
-            // The previous master may have gone too deep to be captured: the drop test. It
-            // may no longer be live, so its ends are read from the nodes.
-            if (last >= 0 && !fell) {
-                int a, b;
-                seg_ends(d, last, a, b);
-                if (walkKey(last, cpos(d, a), cpos(d, b)) != ~0ULL) {
-                    const double dd = c_depth(d, xn, last, a, b);
-                    if (dd > 0.0 && dd > C_MAX_DEPTH * d.cLen[last] && !c_excluded(d, n, pn, last, a, b)) fell = true;
-                }
-            }
-        } else {
+        } else if (sub == 0) {
             // The reference's own walk over the nine cells.
             for (int dj = -1; dj <= 1; dj++) {
                 const int j = cj + dj;
@@ -561,6 +558,34 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
                                  d.cNrm[p], rank | (unsigned long long)x);
                     }
                 }
+            }
+        }
+        // The group's best candidate: deepest, then the earliest walk position (positions are
+        // distinct, so the result does not depend on which lane found what); a fall seen by
+        // any lane counts.
+        int fellAny = fell ? 1 : 0;
+#pragma unroll
+        for (int off = 1; off < C_SUB; off <<= 1) {
+            const double od = __shfl_xor(deepest, off, 64);
+            const unsigned long long ok = __shfl_xor(bestKey, off, 64);
+            const int ob = __shfl_xor(best, off, 64);
+            fellAny |= __shfl_xor(fellAny, off, 64);
+            if (od > deepest || (od == deepest && ok < bestKey)) {
+                deepest = od;
+                bestKey = ok;
+                best = ob;
+            }
+        }
+        fell = fellAny != 0;
+        if (sub != 0) continue;
+        // The previous master may have gone too deep to be captured: the drop test. It may
+        // no longer be live, so its ends are read from the nodes.
+        if (!c->cFOverflow && last >= 0 && !fell) {
+            int a, b;
+            seg_ends(d, last, a, b);
+            if (walkKey(last, cpos(d, a), cpos(d, b)) != ~0ULL) {
+                const double dd = c_depth(d, xn, last, a, b);
+                if (dd > 0.0 && dd > C_MAX_DEPTH * d.cLen[last] && !c_excluded(d, n, pn, last, a, b)) fell = true;
             }
         }
         // accumulate :449-459

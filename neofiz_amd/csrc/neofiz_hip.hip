@@ -163,7 +163,7 @@ struct Dev {
     double cFriction, cDamping, cScale, cRef;
     // The capture grid: square cells of side cFine, each listing every segment whose capture
     // rectangle (the region where it can be a node's master) overlaps the cell.
-    int* cFHead;             // [cFHashMax] hash heads
+    int* cFHead;             // [cFHashMax * C_SUB] hash heads, C_SUB lists per bucket
     int4* cFEnt;             // [cFCap] {next entry, segment p, cell i, cell j}: one load a link
     double4* cSegX;          // [4E] current end positions of each live segment (a.r, a.z, b.r, b.z)
     int cFCap;
@@ -1907,7 +1907,7 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
         d.cFCap = (int)std::min<size_t>(0x7FFFFFFF, std::max<size_t>(1u << 20, 16 * (size_t)E));
         d.cFHashMax = 64;
         while (d.cFHashMax < 16u * (unsigned)E && d.cFHashMax < (1u << 28)) d.cFHashMax <<= 1;
-        TRY(s->alloc(&d.cFHead, d.cFHashMax));
+        TRY(s->alloc(&d.cFHead, (size_t)d.cFHashMax * C_SUB));
         TRY(s->alloc(&d.cFEnt, (size_t)d.cFCap));
         TRY(s->alloc(&d.cSegX, 4 * (size_t)E));
         // Capture cells twice the median surface segment of the reference mesh.
