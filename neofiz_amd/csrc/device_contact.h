@@ -345,7 +345,25 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
     for (int t = t0; t < nSurf; t += stride) {
         const int n = d.cSurf[t];
         double sr = 0.0, sz = 0.0;
-        for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1]; k++) {
+        // the first eight segments' flags and normals loaded side by side, summed in order
+        const int k0 = d.cNodeSegStart[n], k1 = d.cNodeSegStart[n + 1];
+        int seg8[8];
+        double2 nv8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) seg8[k] = k0 + k < k1 ? d.cNodeSegOf[k0 + k] : -1;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool live = seg8[k] >= 0 && d.cActive[seg8[k]];
+            nv8[k] = live ? d.cNrm[seg8[k]] : make_double2(0.0, 0.0);
+            if (!live) seg8[k] = -1;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (seg8[k] >= 0) {
+                sr += nv8[k].x;
+                sz += nv8[k].y;
+            }
+        for (int k = k0 + 8; k < k1; k++) {
             const int p = d.cNodeSegOf[k];
             if (!d.cActive[p]) continue;
             const double2 nv = d.cNrm[p];
@@ -410,8 +428,21 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
             atomicOr(&c->cFOverflow, 1);
             continue;
         }
+        // Up to four cells (the usual count) have their head exchanges issued together
+        // before the entries are stored; larger boxes continue one cell at a time.
         const int width = i1 - i0 + 1;
-        for (int k = 0, at = base + offset; k < count; k++, at++) {
+        const int at0 = base + offset;
+        int next4[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < count) {
+                const int at = at0 + k, i = i0 + k % width, j = j0 + k / width;
+                next4[k] = atomicExch(&d.cFHead[c_bucket(i, j, fmask) * C_SUB + (at & (C_SUB - 1))], at);
+            }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < count) d.cFEnt[at0 + k] = make_int4(next4[k], p, i0 + k % width, j0 + k / width);
+        for (int k = 4, at = at0 + 4; k < count; k++, at++) {
             const int i = i0 + k % width, j = j0 + k / width;
             const int next = atomicExch(&d.cFHead[c_bucket(i, j, fmask) * C_SUB + (at & (C_SUB - 1))], at);
             d.cFEnt[at] = make_int4(next, p, i, j);
