@@ -692,6 +692,77 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
 // The pair terms that touch one node, added to its gathered force in the reference's
 // order: by slave node ascending (accumulate walks the surface nodes in order), and within
 // a pair the normal terms before the tangential ones (resolve :687-759).
+// One pair term on node n: slave `who`, role 0 (n is the slave), 1 or 2 (n is end a or b of
+// the master segment), resolve :687-759.
+__device__ __forceinline__ void c_term(const Dev& d, double2& f, int who, int role, bool friction) {
+    const double* rec = d.cPair + (size_t)who * CPAIR;
+    const double normal = rec[PR_NORMAL], nR = rec[PR_NR], nZ = rec[PR_NZ];
+    const double tangent = rec[PR_TANGENT];
+    const double tR = -nZ, tZ = nR;
+    if (role == 0) {
+        f.x += normal * nR;
+        f.y += normal * nZ;
+        if (friction) {
+            f.x += tangent * tR;
+            f.y += tangent * tZ;
+        }
+    } else {
+        const double w = role == 1 ? rec[PR_WA] : rec[PR_WB];
+        f.x -= w * normal * nR;
+        f.y -= w * normal * nZ;
+        if (friction) {
+            f.x -= w * tangent * tR;
+            f.y -= w * tangent * tZ;
+        }
+    }
+}
+
+// Terms a node keeps in registers; a node with more is handled by c_apply_many. (A build
+// with NF_CONTACT_FAST_TERMS=1 sends every node with two or more terms down that path.)
+#ifndef NF_CONTACT_FAST_TERMS
+#define NF_CONTACT_FAST_TERMS 8
+#endif
+constexpr int C_FAST_TERMS = NF_CONTACT_FAST_TERMS;
+
+// Inserts key into t, kept ascending with unused slots at INT_MAX, by constant indices only
+// (so t stays in registers).
+__device__ __forceinline__ void c_insert(int (&t)[C_FAST_TERMS], int key) {
+#pragma unroll
+    for (int k = C_FAST_TERMS - 1; k > 0; k--) t[k] = t[k - 1] > key ? t[k - 1] : (t[k] > key ? key : t[k]);
+    t[0] = t[0] > key ? key : t[0];
+}
+
+// A node with more than C_FAST_TERMS terms (rare): the terms in slave order by repeated
+// selection of the next larger key over the first C_MAX_TERMS terms in collection order,
+// which needs no per-thread arrays.
+__device__ __forceinline__ void c_apply_many(const Dev& d, int n, bool friction, int m) {
+    const int count = m < C_MAX_TERMS ? m : C_MAX_TERMS;
+    double2 f = d.f[n];
+    int prev = -1;
+    for (int i = 0; i < count; i++) {
+        int best = 0x7FFFFFFF;
+        int c = 0;
+        if (d.cPairOk[n]) {
+            const int key = 4 * n;
+            if (key > prev && key < best) best = key;
+            c++;
+        }
+        for (int k = d.cNodeSegStart[n]; k < d.cNodeSegStart[n + 1] && c < count; k++) {
+            const int p = d.cNodeSegOf[k];
+            if (!d.cActive[p]) continue;
+            int a, b;
+            seg_ends(d, p, a, b);
+            for (int s = d.cPairHead[p]; s >= 0 && c < count; s = d.cPairNext[s], c++) {
+                const int key = 4 * s + (n == a ? 1 : 2);
+                if (key > prev && key < best) best = key;
+            }
+        }
+        c_term(d, f, best >> 2, best & 3, friction);
+        prev = best;
+    }
+    d.f[n] = f;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_c_apply(Dev d, int q, int checkGo) {
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
@@ -703,12 +774,14 @@ __global__ void __launch_bounds__(BLOCK) k_c_apply(Dev d, int q, int checkGo) {
     const int stride = gridDim.x * BLOCK;
     for (int n = blockIdx.x * BLOCK + threadIdx.x; any && n < d.N; n += stride) {
         if (!d.cSurfFlag[n]) continue;
-        int who[C_MAX_TERMS];
-        int role[C_MAX_TERMS];  // 0 slave, 1 end a, 2 end b
+        // The node's terms, keyed 4 who + role (a node meets each pair at most once, so the
+        // keys sort by slave node), in the reference's order: by slave node ascending.
+        int t[C_FAST_TERMS];
+#pragma unroll
+        for (int k = 0; k < C_FAST_TERMS; k++) t[k] = 0x7FFFFFFF;
         int m = 0;
         if (d.cPairOk[n]) {
-            who[m] = n;
-            role[m] = 0;
+            c_insert(t, 4 * n);
             m++;
             const double* rec = d.cPair + (size_t)n * CPAIR;
             pairs++;
@@ -722,51 +795,23 @@ __global__ void __launch_bounds__(BLOCK) k_c_apply(Dev d, int q, int checkGo) {
             int a, b;
             seg_ends(d, p, a, b);
             for (int s = d.cPairHead[p]; s >= 0; s = d.cPairNext[s]) {
-                if (m == C_MAX_TERMS) {
+                if (m == C_MAX_TERMS) {  // the first C_MAX_TERMS are applied
                     c->cOverflow = 1;
                     break;
                 }
-                who[m] = s;
-                role[m] = n == a ? 1 : 2;
+                if (m < C_FAST_TERMS) c_insert(t, 4 * s + (n == a ? 1 : 2));
                 m++;
             }
         }
         if (m == 0) continue;
-        // Insertion sort by slave node; a node meets each pair at most once.
-        for (int i = 1; i < m; i++) {
-            const int w = who[i], r = role[i];
-            int j = i - 1;
-            while (j >= 0 && who[j] > w) {
-                who[j + 1] = who[j];
-                role[j + 1] = role[j];
-                j--;
-            }
-            who[j + 1] = w;
-            role[j + 1] = r;
+        if (m > C_FAST_TERMS) {
+            c_apply_many(d, n, friction, m);
+            continue;
         }
         double2 f = d.f[n];
-        for (int i = 0; i < m; i++) {
-            const double* rec = d.cPair + (size_t)who[i] * CPAIR;
-            const double normal = rec[PR_NORMAL], nR = rec[PR_NR], nZ = rec[PR_NZ];
-            const double tangent = rec[PR_TANGENT];
-            const double tR = -nZ, tZ = nR;
-            if (role[i] == 0) {
-                f.x += normal * nR;
-                f.y += normal * nZ;
-                if (friction) {
-                    f.x += tangent * tR;
-                    f.y += tangent * tZ;
-                }
-            } else {
-                const double w = role[i] == 1 ? rec[PR_WA] : rec[PR_WB];
-                f.x -= w * normal * nR;
-                f.y -= w * normal * nZ;
-                if (friction) {
-                    f.x -= w * tangent * tR;
-                    f.y -= w * tangent * tZ;
-                }
-            }
-        }
+#pragma unroll
+        for (int k = 0; k < C_FAST_TERMS; k++)
+            if (k < m) c_term(d, f, t[k] >> 2, t[k] & 3, friction);
         d.f[n] = f;
     }
     // Fixed-order per-block audit partials; block 0 of the nodal launch adds them up.
