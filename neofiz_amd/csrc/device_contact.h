@@ -187,13 +187,17 @@ __device__ __forceinline__ int block_exclusive(int v, int* lds, int& total) {
     return before + x - v;
 }
 
-// The rebuild launches give each thread C_CHUNK items (strided by the workgroup, so loads and
+// The rebuild launches give each thread a few items (strided by the workgroup, so loads and
 // flag stores stay coalesced) and reserve a workgroup's appends with one atomic: a counter
-// shared by the whole grid serialises its atomics, so they are kept to a few hundred.
-constexpr int C_CHUNK = 4;
+// shared by the whole grid serialises its atomics, so they are kept to a few hundred. The
+// segment pass takes more items per thread (more workgroups would mean more atomics on its
+// counter), the node pass fewer (its dependent loads want more waves in flight).
+constexpr int C_CHUNK_SEG = 8;
+constexpr int C_CHUNK_NODE = 2;
 
 // Appends this thread's items (bit j of mask: item first + j * BLOCK) to list[0, cap) through
 // counter, one reservation per workgroup.
+template <int CHUNK>
 __device__ __forceinline__ void block_append(unsigned mask, int first, int* counter, int* list, int cap) {
     __shared__ int lds[BLOCK / 64];
     __shared__ int ldsBase;
@@ -202,7 +206,7 @@ __device__ __forceinline__ void block_append(unsigned mask, int first, int* coun
     if (threadIdx.x == 0) ldsBase = total ? atomicAdd(counter, total) : 0;
     __syncthreads();
     at += ldsBase;
-    for (int j = 0; j < C_CHUNK; j++)
+    for (int j = 0; j < CHUNK; j++)
         if ((mask >> j) & 1u) {
             if (at >= 0 && at < cap) list[at] = first + j * BLOCK;
             at++;
@@ -227,10 +231,10 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
     }
     if (!c->surfaceStale) return;
     const int E4 = 4 * d.E;
-    const int first = blockIdx.x * BLOCK * C_CHUNK + threadIdx.x;
+    const int first = blockIdx.x * BLOCK * C_CHUNK_SEG + threadIdx.x;
     unsigned mask = 0;
 #pragma unroll
-    for (int j = 0; j < C_CHUNK; j++) {
+    for (int j = 0; j < C_CHUNK_SEG; j++) {
         const int p = first + j * BLOCK;
         if (p < E4) {
             const bool live = seg_live(d, p);
@@ -238,7 +242,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
             mask |= (live ? 1u : 0u) << j;
         }
     }
-    block_append(mask, first, &c->cNAct, d.cAct, E4);
+    block_append<C_CHUNK_SEG>(mask, first, &c->cNAct, d.cAct, E4);
 }
 
 // The second half of the rebuild: surface nodes from the live flags just written.
@@ -246,10 +250,10 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild_nodes(Dev d, int q, int che
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
     if (!c->surfaceStale) return;
-    const int first = blockIdx.x * BLOCK * C_CHUNK + threadIdx.x;
+    const int first = blockIdx.x * BLOCK * C_CHUNK_NODE + threadIdx.x;
     unsigned mask = 0;
 #pragma unroll
-    for (int j = 0; j < C_CHUNK; j++) {
+    for (int j = 0; j < C_CHUNK_NODE; j++) {
         const int n = first + j * BLOCK;
         if (n < d.N) {
             const bool on = node_on_surface(d, n);
@@ -259,7 +263,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild_nodes(Dev d, int q, int che
             mask |= (on ? 1u : 0u) << j;
         }
     }
-    block_append(mask, first, &c->cNSurf, d.cSurf, d.N);
+    block_append<C_CHUNK_NODE>(mask, first, &c->cNSurf, d.cSurf, d.N);
 }
 
 // geometry() :465-487 per live segment; the grid's box and longest segment; clears.

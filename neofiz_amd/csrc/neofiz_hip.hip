@@ -1086,8 +1086,8 @@ __global__ void k_math_eval(int fn, const double* x, const double* y, double* ou
 }
 
 inline int blocks_for(int n) { return (n + BLOCK - 1) / BLOCK; }
-// Workgroups of the rebuild launches, C_CHUNK items per thread.
-inline int blocks_for_chunks(int n) { return std::max(1, (n + BLOCK * C_CHUNK - 1) / (BLOCK * C_CHUNK)); }
+// Workgroups of a launch that gives each thread `chunk` items.
+inline int blocks_for_chunks(int n, int chunk) { return std::max(1, (n + BLOCK * chunk - 1) / (BLOCK * chunk)); }
 
 template <typename T>
 int dev_alloc(T** p, size_t count) {
@@ -1347,8 +1347,8 @@ struct nf_solver {
         const int g4 = std::min(blocks_for(4 * E), CONTACT_GRID);
         const int gn = std::min(blocks_for(N), CONTACT_GRID);
         if (a) HIP_TRY(hipEventRecord(a, stream));
-        hipLaunchKernelGGL(k_c_rebuild, dim3(blocks_for_chunks(4 * E)), dim3(BLOCK), 0, stream, d, parity, checkGo);
-        hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(blocks_for_chunks(N)), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_rebuild, dim3(blocks_for_chunks(4 * E, C_CHUNK_SEG)), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(blocks_for_chunks(N, C_CHUNK_NODE)), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_geometry, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_grid, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_search, dim3(gn), dim3(BLOCK), 0, stream, d, parity, checkGo);
@@ -1958,8 +1958,8 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
     d.cDamping = damping;
     d.cScale = scale;
     d.cRef = s->dt0;
-    hipLaunchKernelGGL(k_c_rebuild, dim3(blocks_for_chunks(4 * E)), dim3(BLOCK), 0, s->stream, d, s->q, 0);
-    hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(blocks_for_chunks(N)), dim3(BLOCK), 0, s->stream, d, s->q, 0);
+    hipLaunchKernelGGL(k_c_rebuild, dim3(blocks_for_chunks(4 * E, C_CHUNK_SEG)), dim3(BLOCK), 0, s->stream, d, s->q, 0);
+    hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(blocks_for_chunks(N, C_CHUNK_NODE)), dim3(BLOCK), 0, s->stream, d, s->q, 0);
     HIP_TRY(hipGetLastError());
     // The lists are built: clear the flag so the first assembly does not append them again.
     TRY(s->sync());
