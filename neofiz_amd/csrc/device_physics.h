@@ -49,18 +49,18 @@ struct Mat {
 // core/JohnsonCook.java:138-197
 // T: the log/exp tables, read from LDS in the kernels (nfm::Tables).
 __device__ __forceinline__ double jc_strainTerm(const Mat& j, double e, const nfm::Tables& T) {
-    return e <= 0.0 ? j.a : j.a + j.b * nfm::pow(e, j.n, T);
+    return e <= 0.0 ? j.a : j.a + j.b * nfm::pow_ordinary(e, j.n, T);
 }
 __device__ __forceinline__ double jc_rateTerm(const Mat& j, double rate, const nfm::Tables& T) {
     if (rate <= j.eps0) return 1.0;
     // x / 1.0 is x exactly, so the division is skipped at the usual reference rate of 1/s.
-    return 1.0 + j.c * nfm::log(j.eps0 == 1.0 ? rate : rate / j.eps0, T);
+    return 1.0 + j.c * nfm::log_ordinary(j.eps0 == 1.0 ? rate : rate / j.eps0, T);
 }
 __device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t, const nfm::Tables& T) {
     const double homologous = (t - j.troom) / (j.tmelt - j.troom);
     if (homologous <= 0.0) return 1.0;
     if (homologous >= 1.0) return 0.0;
-    return 1.0 - nfm::pow(homologous, j.m, T);
+    return 1.0 - nfm::pow_ordinary(homologous, j.m, T);
 }
 
 // solver/J2.java:332-375, the bracketed Newton solve for the Johnson-Cook multiplier.
@@ -83,10 +83,10 @@ __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, d
         const double dEps = NFD_SQRT_2_3 * dGamma;
         const double eps = plasticStrain + dEps;
         const double rate = dt > 0.0 ? dEps / dt : 0.0;
-        const double pw = nfm::pow(eps, j.n, T);
+        const double pw = nfm::pow_ordinary(eps, j.n, T);
         const double strain = eps <= 0.0 ? j.a : j.a + j.b * pw;  // strainTerm
         const bool clamped = rate <= j.eps0;
-        const double lg = nfm::log(UNIT_RATE ? rate : rate / j.eps0, T);
+        const double lg = nfm::log_ordinary(UNIT_RATE ? rate : rate / j.eps0, T);
         const double rateFactor = clamped ? 1.0 : 1.0 + j.c * lg;  // rateTerm
         const double g = norm - stiffness * dGamma - NFD_SQRT_2_3 * strain * rateFactor * thermal;
         lo = g > 0.0 ? dGamma : lo;

@@ -201,18 +201,23 @@ NF_HD double exp_dd(double a0, double b, const double (*ET)[2]) {
 
 NF_HD double exp(double x) { return exp_dd(x, 0.0, NFM_EXP_TBL); }
 
-// x^y for the cases the solver needs: x > 0 finite, y finite. Special values follow IEEE pow
-// for x = 0, x = 1, y = 0, y = 1 (exactly x), NaN; a negative base yields NaN. Branch-free.
-NF_HD double pow(double x, double y, const Tables& T) {
-    const bool regular = x > 0.0 && x < inf() && !is_nan(y);
+// The main path of pow below: exp(y log x) with log x as a double-double.
+NF_HD double pow_main(double x, double y, const Tables& T) {
     double l_hi, l_lo;
-    log_dd(regular ? x : 1.0, l_hi, l_lo, T.log);
+    log_dd(x, l_hi, l_lo, T.log);
     double p_hi, p_lo;
     two_prod(y, l_hi, p_hi, p_lo);
     p_lo = p_lo + y * l_lo;
     double t_hi, t_lo;
     fast_two_sum(p_hi, p_lo, t_hi, t_lo);
-    const double v = exp_dd(t_hi, t_lo, T.exp);
+    return exp_dd(t_hi, t_lo, T.exp);
+}
+
+// x^y for the cases the solver needs: x > 0 finite, y finite. Special values follow IEEE pow
+// for x = 0, x = 1, y = 0, y = 1 (exactly x), NaN; a negative base yields NaN. Branch-free.
+NF_HD double pow(double x, double y, const Tables& T) {
+    const bool regular = x > 0.0 && x < inf() && !is_nan(y);
+    const double v = pow_main(regular ? x : 1.0, y, T);
     double special;
     if (is_nan(x) || is_nan(y)) special = qnan();
     else if (x == 0.0) special = y > 0.0 ? 0.0 : inf();
@@ -223,6 +228,25 @@ NF_HD double pow(double x, double y, const Tables& T) {
 }
 
 NF_HD double pow(double x, double y) { return pow(x, y, Tables{NFM_LOG_TBL, NFM_EXP_TBL}); }
+
+// pow and log with their special inputs out of line, for loops that almost never meet one
+// (the Johnson-Cook Newton solve): an ordinary input (x positive, finite and not 1; for pow,
+// y neither NaN, 0 nor 1) takes the main path alone, with none of the selects that pick the
+// special values; any other input calls pow / log themselves. For every input the result is
+// pow's or log's own, bit for bit: on ordinary inputs those return exactly the main path.
+NF_HD double pow_ordinary(double x, double y, const Tables& T) {
+    if (x > 0.0 && x < inf() && x != 1.0 && !is_nan(y) && y != 0.0 && y != 1.0) return pow_main(x, y, T);
+    return pow(x, y, T);
+}
+
+NF_HD double log_ordinary(double x, const Tables& T) {
+    if (x > 0.0 && x < inf()) {
+        double hi, lo;
+        log_dd(x, hi, lo, T.log);
+        return hi + lo;
+    }
+    return log(x, T);
+}
 
 #define NFM_PI 3.141592653589793
 #define NFM_PI_2_HI 0x1.921fb54442d18p0    // pi/2 rounded

@@ -73,3 +73,26 @@ def test_special_values(oracle):
     assert np.array_equal(evaluate(b, 9, tiny, None), tiny)      # log1p x = x below 2^-54
     assert evaluate(b, 9, np.array([-1.0]), None)[0] == -math.inf
     assert math.isnan(evaluate(b, 9, np.array([-1.5]), None)[0])
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+def test_ordinary_input_variants_are_pow_and_log(oracle):
+    """pow_ordinary / log_ordinary (functions 10, 11: the Newton loop's pow and log, with the
+    special inputs out of line) return pow's and log's own bits on every input: ordinary ones
+    over many binades, and every special base and exponent."""
+    b = oracle.binding()
+    rng = np.random.default_rng(11)
+    specials = np.array([0.0, -0.0, 1.0, -1.0, 2.0, 0.5, 5e-324, 2.2250738585072014e-308, 1e-310,
+                         1.7976931348623157e308, math.inf, -math.inf, math.nan, -2.5])
+    xs = np.concatenate([np.exp(rng.uniform(-740, 709, 3000)), rng.uniform(0.5, 2.0, 1000), specials])
+    ys = np.concatenate([rng.uniform(-3, 3, 2000), rng.uniform(0.05, 1.5, 2000), specials])
+    x = np.repeat(xs, 4)[: 4 * xs.size]
+    y = np.resize(ys, x.size)
+    xx, yy = np.meshgrid(specials, specials)
+    x = np.concatenate([x, xx.ravel()])
+    y = np.concatenate([y, yy.ravel()])
+    assert np.array_equal(_bits(evaluate(b, 10, x, y)), _bits(evaluate(b, 2, x, y)))
+    assert np.array_equal(_bits(evaluate(b, 11, xs, None)), _bits(evaluate(b, 0, xs, None)))
