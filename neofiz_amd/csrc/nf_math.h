@@ -97,11 +97,13 @@ struct Tables {
     const double (*exp)[2];
 };
 
-NF_HD void log_dd(double x, double& hi, double& lo, const double (*LT)[3]) {
+// MAY_BE_SUB = false is the same computation for a normal x, without the subnormal select.
+template <bool MAY_BE_SUB>
+NF_HD void log_dd_impl(double x, double& hi, double& lo, const double (*LT)[3]) {
     // Branch-free: every range decision is a select, so the compiler can interleave this
     // chain with independent work (the Newton loop evaluates a pow and a log side by side).
     const uint64_t b0 = bits_of(x);
-    const bool sub = b0 < 0x0010000000000000ULL;  // subnormal (or zero)
+    const bool sub = MAY_BE_SUB && b0 < 0x0010000000000000ULL;  // subnormal (or zero)
     const double xs = sub ? x * 0x1p54 : x;
     const uint64_t b = bits_of(xs);
     int k = (int)(b >> 52) - 1023 - (sub ? 54 : 0);
@@ -135,6 +137,10 @@ NF_HD void log_dd(double x, double& hi, double& lo, const double (*LT)[3]) {
     two_sum(s1, r, s2, e2);
     const double l = (e1 + e2) + ((kd * NFM_LN2_LO + LT[j][2]) + (r_lo + P));
     fast_two_sum(s2, l, hi, lo);
+}
+
+NF_HD void log_dd(double x, double& hi, double& lo, const double (*LT)[3]) {
+    log_dd_impl<true>(x, hi, lo, LT);
 }
 
 NF_HD double log(double x, const Tables& T) {
@@ -171,11 +177,8 @@ NF_HD double log1p(double x) {
 // scheme and the table value carried as hi + lo. Branch-free: the scaling by 2^k is two
 // exact multiplications by powers of two (one rounding at most, into the subnormal range),
 // and the out-of-range results are selected at the end.
-NF_HD double exp_dd(double a0, double b, const double (*ET)[2]) {
-    const bool nan = is_nan(a0);
-    const bool over = a0 > 709.782712893384;
-    const bool under = a0 < -745.2;
-    const double a = (nan || over || under) ? 0.0 : a0;
+// The main path of exp_dd for a in [-745.2, 709.782712893384].
+NF_HD double exp_dd_main(double a, double b, const double (*ET)[2]) {
     const double shifter = 0x1.8p52;
     const double nd = (a * NFM_INV_L64 + shifter) - shifter;
     const int n = (int)nd;
@@ -195,7 +198,15 @@ NF_HD double exp_dd(double a0, double b, const double (*ET)[2]) {
     const double y = th + (tl + (th * r + (th * E + tl * p)));
     const int k1 = k / 2;  // |k1|, |k - k1| <= 538: both factors are normal doubles
     const int k2 = k - k1;
-    const double v = (y * from_bits((uint64_t)(k1 + 1023) << 52)) * from_bits((uint64_t)(k2 + 1023) << 52);
+    return (y * from_bits((uint64_t)(k1 + 1023) << 52)) * from_bits((uint64_t)(k2 + 1023) << 52);
+}
+
+NF_HD double exp_dd(double a0, double b, const double (*ET)[2]) {
+    const bool nan = is_nan(a0);
+    const bool over = a0 > 709.782712893384;
+    const bool under = a0 < -745.2;
+    const double a = (nan || over || under) ? 0.0 : a0;
+    const double v = exp_dd_main(a, b, ET);
     return nan ? a0 : (over ? inf() : (under ? 0.0 : v));
 }
 
@@ -230,19 +241,30 @@ NF_HD double pow(double x, double y, const Tables& T) {
 NF_HD double pow(double x, double y) { return pow(x, y, Tables{NFM_LOG_TBL, NFM_EXP_TBL}); }
 
 // pow and log with their special inputs out of line, for loops that almost never meet one
-// (the Johnson-Cook Newton solve): an ordinary input (x positive, finite and not 1; for pow,
-// y neither NaN, 0 nor 1) takes the main path alone, with none of the selects that pick the
-// special values; any other input calls pow / log themselves. For every input the result is
-// pow's or log's own, bit for bit: on ordinary inputs those return exactly the main path.
+// (the Johnson-Cook Newton solve). An ordinary input takes the main path alone, with none of
+// the selects that pick the special values: x normal, finite and (for pow) not 1; y neither
+// NaN, 0 nor 1; and y log x inside exp's range. Any other input calls pow / log / exp_dd
+// themselves. For every input the result is pow's or log's own, bit for bit: on ordinary
+// inputs those return exactly the main path.
 NF_HD double pow_ordinary(double x, double y, const Tables& T) {
-    if (x > 0.0 && x < inf() && x != 1.0 && !is_nan(y) && y != 0.0 && y != 1.0) return pow_main(x, y, T);
+    if (x >= 0x1p-1022 && x < inf() && x != 1.0 && !is_nan(y) && y != 0.0 && y != 1.0) {
+        double l_hi, l_lo;
+        log_dd_impl<false>(x, l_hi, l_lo, T.log);
+        double p_hi, p_lo;
+        two_prod(y, l_hi, p_hi, p_lo);
+        p_lo = p_lo + y * l_lo;
+        double t_hi, t_lo;
+        fast_two_sum(p_hi, p_lo, t_hi, t_lo);
+        if (t_hi >= -745.2 && t_hi <= 709.782712893384) return exp_dd_main(t_hi, t_lo, T.exp);
+        return exp_dd(t_hi, t_lo, T.exp);
+    }
     return pow(x, y, T);
 }
 
 NF_HD double log_ordinary(double x, const Tables& T) {
-    if (x > 0.0 && x < inf()) {
+    if (x >= 0x1p-1022 && x < inf()) {
         double hi, lo;
-        log_dd(x, hi, lo, T.log);
+        log_dd_impl<false>(x, hi, lo, T.log);
         return hi + lo;
     }
     return log(x, T);
