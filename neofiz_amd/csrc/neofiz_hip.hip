@@ -1082,6 +1082,13 @@ __global__ void k_math_eval(int fn, const double* x, const double* y, double* ou
         case 9: r = nfm::log1p(x[i]); break;
         case 10: r = nfm::pow_ordinary(x[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}); break;
         case 11: r = nfm::log_ordinary(x[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}); break;
+        case 12:
+        case 13: {
+            double lx;
+            const double pw = nfm::pow_log_ordinary(x[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, lx);
+            r = fn == 12 ? pw : lx;
+            break;
+        }
         default: r = 0.0;
     }
     out[i] = r;
@@ -2584,7 +2591,7 @@ int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_
 int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n) {
     g_error.clear();
     if (n <= 0) return NF_OK;
-    if (fn < 0 || fn > 11) return fail(NF_E_ARGUMENT, "unknown function");
+    if (fn < 0 || fn > 13) return fail(NF_E_ARGUMENT, "unknown function");
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;
     TRY(dev_alloc(&dx, n));
     TRY(dev_alloc(&dy, n));

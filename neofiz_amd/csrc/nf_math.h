@@ -261,6 +261,25 @@ NF_HD double pow_ordinary(double x, double y, const Tables& T) {
     return pow(x, y, T);
 }
 
+// pow_ordinary(x, y) together with log(x) (into lx), sharing the logarithm's double-double
+// when the input is ordinary: lx is log's own bits either way.
+NF_HD double pow_log_ordinary(double x, double y, const Tables& T, double& lx) {
+    if (x >= 0x1p-1022 && x < inf() && x != 1.0 && !is_nan(y) && y != 0.0 && y != 1.0) {
+        double l_hi, l_lo;
+        log_dd_impl<false>(x, l_hi, l_lo, T.log);
+        lx = l_hi + l_lo;
+        double p_hi, p_lo;
+        two_prod(y, l_hi, p_hi, p_lo);
+        p_lo = p_lo + y * l_lo;
+        double t_hi, t_lo;
+        fast_two_sum(p_hi, p_lo, t_hi, t_lo);
+        if (t_hi >= -745.2 && t_hi <= 709.782712893384) return exp_dd_main(t_hi, t_lo, T.exp);
+        return exp_dd(t_hi, t_lo, T.exp);
+    }
+    lx = log(x, T);
+    return pow(x, y, T);
+}
+
 NF_HD double log_ordinary(double x, const Tables& T) {
     if (x >= 0x1p-1022 && x < inf()) {
         double hi, lo;

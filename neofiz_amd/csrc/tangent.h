@@ -148,15 +148,16 @@ __device__ __forceinline__ D sqrt(const D& a) {
 // log and pow: the value from the shared table method, the derivative from calculus.
 __device__ __forceinline__ D log(const D& a, const nfm::Tables& T) {
     D r;
-    r.v = nfm::log(a.v, T);
+    r.v = nfm::log_ordinary(a.v, T);
 #pragma unroll
     for (int k = 0; k < NL; k++) r.t[k] = a.t[k] / a.v;
     return r;
 }
 __device__ __forceinline__ D pow(const D& x, const D& y, const nfm::Tables& T) {
     D r;
-    r.v = nfm::pow(x.v, y.v, T);
-    const double lx = x.v > 0.0 ? nfm::log(x.v, T) : 0.0;
+    double logx;
+    r.v = nfm::pow_log_ordinary(x.v, y.v, T, logx);
+    const double lx = x.v > 0.0 ? logx : 0.0;
     const double dx = x.v != 0.0 ? r.v * y.v / x.v : 0.0;
 #pragma unroll
     for (int k = 0; k < NL; k++) r.t[k] = dx * x.t[k] + r.v * lx * y.t[k];

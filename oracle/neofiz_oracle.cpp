@@ -2067,6 +2067,13 @@ extern "C" int nfo_math_eval(int32_t fn, const double* x, const double* y, doubl
             case 9: r = nfm::log1p(x[i]); break;
             case 10: r = nfm::pow_ordinary(x[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}); break;
             case 11: r = nfm::log_ordinary(x[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}); break;
+            case 12:
+            case 13: {
+                double lx;
+                const double pw = nfm::pow_log_ordinary(x[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, lx);
+                r = fn == 12 ? pw : lx;
+                break;
+            }
             default: return fail(NF_E_ARGUMENT, "unknown function");
         }
         out[i] = r;

@@ -57,12 +57,14 @@ def rel(a, b):
     (5, 0.0, 6.3, None),        # cos over the whole turn (the white noise's angle)
     (10, -30.0, 3.0, (0.05, 1.5)),  # the Newton loop's pow of exp(U)
     (11, -40.0, 40.0, None),    # the Newton loop's log of exp(U)
+    (12, -30.0, 3.0, (0.05, 1.5)),  # the derivative pass's pow of exp(U)
+    (13, -30.0, 3.0, (0.05, 1.5)),  # and the log it shares
 ])
 def test_device_math_matches_host(hip, oracle, fn, lo, hi, second):
     rng = np.random.default_rng(100 + fn)
     n = 200_000
     x = rng.uniform(lo, hi, n)
-    if fn in (0, 2, 7, 10, 11):
+    if fn in (0, 2, 7, 10, 11, 12, 13):
         x = np.exp(x)
     y = rng.uniform(*second, n) if second else np.zeros(n)
     import ctypes as C

@@ -80,9 +80,10 @@ def _bits(a):
 
 
 def test_ordinary_input_variants_are_pow_and_log(oracle):
-    """pow_ordinary / log_ordinary (functions 10, 11: the Newton loop's pow and log, with the
-    special inputs out of line) return pow's and log's own bits on every input: ordinary ones
-    over many binades, and every special base and exponent."""
+    """pow_ordinary / log_ordinary / pow_log_ordinary (functions 10-13: the Newton loop's pow
+    and log with the special inputs out of line, and the derivative pass's pow with its shared
+    log) return pow's and log's own bits on every input: ordinary ones over many binades, and
+    every special base and exponent."""
     b = oracle.binding()
     rng = np.random.default_rng(11)
     specials = np.array([0.0, -0.0, 1.0, -1.0, 2.0, 0.5, 5e-324, 2.2250738585072014e-308, 1e-310,
@@ -96,3 +97,6 @@ def test_ordinary_input_variants_are_pow_and_log(oracle):
     y = np.concatenate([y, yy.ravel()])
     assert np.array_equal(_bits(evaluate(b, 10, x, y)), _bits(evaluate(b, 2, x, y)))
     assert np.array_equal(_bits(evaluate(b, 11, xs, None)), _bits(evaluate(b, 0, xs, None)))
+    # the derivative pass's pow with the logarithm it shares (functions 12, 13)
+    assert np.array_equal(_bits(evaluate(b, 12, x, y)), _bits(evaluate(b, 2, x, y)))
+    assert np.array_equal(_bits(evaluate(b, 13, x, y)), _bits(evaluate(b, 0, x, None)))
