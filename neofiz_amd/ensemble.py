@@ -75,6 +75,30 @@ def results(mesh, solver):
     return (z_max - z_min) / taylor.ROD_LENGTH, 2.0 * radius
 
 
+def predicted_cost(inst):
+    """Element-steps to the end time, up to a factor common to the ensemble: the stable step
+    scales with the cell size, the rod's diameter over nr, so the step count scales with nr."""
+    return inst.elements * inst.nr
+
+
+def shard(instances, devices):
+    """Assigns each instance to one of `devices` GPUs, which then run their shares as
+    independent batches with no communication: costliest first (predicted_cost), each to the
+    device with the least cost so far, ties to the lowest device. Returns one ascending list
+    of instance indices per device. The heaviest device carries at most the mean load plus
+    one instance (the longest-processing-time bound)."""
+    if devices < 1:
+        raise ValueError("at least one device")
+    load = [0] * devices
+    out = [[] for _ in range(devices)]
+    order = sorted(range(len(instances)), key=lambda i: (-predicted_cost(instances[i]), i))
+    for i in order:
+        d = min(range(devices), key=lambda k: (load[k], k))
+        load[d] += predicted_cost(instances[i])
+        out[d].append(i)
+    return [sorted(x) for x in out]
+
+
 def tune(batch, instances, end_time, trial_steps=24):
     """The batch's autotuner: instances are bucketed by mesh size, and each bucket's tile
     size (threads per tile, so how many of its tiles share a workgroup) is chosen in turn
