@@ -537,19 +537,17 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
         };
         // Where segment p's first entry falls in the reference's walk over the nine cells
         // around (ci, cj): (cell rank, entry index 2p + end), or ~0 if neither end is there.
+        // (Each end through its own call, so no array of the two ends reaches scratch memory.)
+        auto endKey = [&](double2 x, unsigned long long entry) {
+            const int di = clampi(jint((x.x - g.r0) / g.cell), g.cols) - ci;
+            const int dj = clampi(jint((x.y - g.z0) / g.cell), g.rows) - cj;
+            if (di < -1 || di > 1 || dj < -1 || dj > 1) return ~0ULL;
+            return ((unsigned long long)((dj + 1) * 3 + (di + 1)) << 40) | entry;
+        };
         auto walkKey = [&](int p, double2 xa, double2 xb) {
-            unsigned long long key = ~0ULL;
-            const double2 ends[2] = {xa, xb};
-            for (int e = 0; e < 2; e++) {
-                const double2 x = ends[e];
-                const int di = clampi(jint((x.x - g.r0) / g.cell), g.cols) - ci;
-                const int dj = clampi(jint((x.y - g.z0) / g.cell), g.rows) - cj;
-                if (di < -1 || di > 1 || dj < -1 || dj > 1) continue;
-                const unsigned long long k =
-                    ((unsigned long long)((dj + 1) * 3 + (di + 1)) << 40) | (unsigned long long)(2 * p + e);
-                key = k < key ? k : key;
-            }
-            return key;
+            const unsigned long long ka = endKey(xa, (unsigned long long)(2 * p));
+            const unsigned long long kb = endKey(xb, (unsigned long long)(2 * p + 1));
+            return kb < ka ? kb : ka;
         };
         if (!c->cFOverflow) {
             // The node's capture cell lists every segment that can be its master; each is
@@ -557,12 +555,13 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
             const int fi = c_fine(xn.x, g.r0, d.cFine), fj = c_fine(xn.y, g.z0, d.cFine);
             // The walk keeps the next link's load in flight while a candidate's own data
             // (end positions, length, normal: three loads issued together) is read.
-            const int4 none = make_int4(-1, 0, 0, 0);
+            // (An exhausted link loads entry 0 and ignores it: a select between a load and a
+            // constant would put the constant in scratch memory.)
             int x = d.cFHead[c_bucket(fi, fj, fmask) * C_SUB + sub];
-            int4 ent = x >= 0 ? d.cFEnt[x] : none;
+            int4 ent = d.cFEnt[x >= 0 ? x : 0];
             while (x >= 0) {
                 const int nx = ent.x;
-                const int4 nent = nx >= 0 ? d.cFEnt[nx] : none;
+                const int4 nent = d.cFEnt[nx >= 0 ? nx : 0];
                 if (ent.z == fi && ent.w == fj) {
                     const int p = ent.y;
                     const double4 sx = d.cSegX[p];  // every listed segment is live this step
