@@ -143,10 +143,11 @@ __device__ __forceinline__ unsigned c_bucket(int i, int j, unsigned mask) {
     return h & mask;
 }
 
-// Capture-grid hash table size: 16 heads per live segment, within what was allocated.
+// Capture-grid hash table size: 4 buckets per live segment (each bucket keeps C_SUB lists,
+// so 4 C_SUB heads), within what was allocated. The heads are cleared every assembly.
 __device__ __forceinline__ unsigned c_capture_size(const Dev& d, int nAct) {
     unsigned h = 64;
-    while (h < 16u * (unsigned)nAct && h < d.cFHashMax) h <<= 1;
+    while (h < 4u * (unsigned)nAct && h < d.cFHashMax) h <<= 1;
     return h;
 }
 
@@ -340,13 +341,18 @@ __global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo)
 }
 
 // Node normals :488-499 (the node's live segments in surface order) and the grid :534-551.
+// The launch is three equal thirds, one per independent part (node normals, the coarse grid,
+// the capture grid), so that their dependent loads overlap instead of queueing in each thread.
 __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
     const int nAct = c->cNAct, nSurf = c->cNSurf;
-    const int stride = gridDim.x * BLOCK;
-    const int t0 = blockIdx.x * BLOCK + threadIdx.x;
-    for (int t = t0; t < nSurf; t += stride) {
+    const int third = gridDim.x / 3;
+    const int part = blockIdx.x / third;
+    const int block = blockIdx.x - part * third;
+    const int stride = third * BLOCK;
+    const int t0 = block * BLOCK + threadIdx.x;
+    if (part == 0) for (int t = t0; t < nSurf; t += stride) {
         const int n = d.cSurf[t];
         double sr = 0.0, sz = 0.0;
         // the first eight segments' flags and normals loaded side by side, summed in order
@@ -377,9 +383,9 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
         const double len = __builtin_sqrt(sr * sr + sz * sz);
         d.cPn[n] = len > 0.0 ? make_double2(sr / len, sz / len) : make_double2(0.0, 0.0);
     }
-    if (nAct == 0) return;
+    if (part == 0 || nAct == 0) return;
     const CGrid g = c_grid(d);
-    for (int t = t0; t < 2 * nAct; t += stride) {
+    if (part == 1) for (int t = t0; t < 2 * nAct; t += stride) {
         const int p = d.cAct[t >> 1];
         int a, b;
         seg_ends(d, p, a, b);
@@ -398,10 +404,11 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
     // Entries are reserved once per workgroup round (the order of a cell's list is
     // immaterial: the search ranks candidates by their walk key); every thread of a
     // workgroup runs the same rounds, so the reservation's barriers are uniform.
+    if (part != 2) return;
     const unsigned fmask = c_capture_size(d, nAct) - 1u;
     __shared__ int lds[BLOCK / 64];
     __shared__ int ldsBase;
-    for (int round = blockIdx.x * BLOCK; round < nAct; round += stride) {
+    for (int round = block * BLOCK; round < nAct; round += stride) {
         const int t = round + threadIdx.x;
         int p = -1, count = 0, i0 = 0, i1 = -1, j0 = 0;
         if (t < nAct) {

@@ -1359,7 +1359,7 @@ struct nf_solver {
         hipLaunchKernelGGL(k_c_rebuild, dim3(blocks_for_chunks(4 * E, C_CHUNK_SEG)), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(blocks_for_chunks(N, C_CHUNK_NODE)), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_geometry, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
-        hipLaunchKernelGGL(k_c_grid, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        hipLaunchKernelGGL(k_c_grid, dim3(3 * g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_search, dim3(gn), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_apply, dim3(d.cApplyBlocks), dim3(BLOCK), 0, stream, d, parity, checkGo);
         if (b) HIP_TRY(hipEventRecord(b, stream));
