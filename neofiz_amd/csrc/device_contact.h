@@ -82,25 +82,37 @@ __device__ __forceinline__ double2 cpos(const Dev& d, int n) {
 
 // p is on the surface of the elements the contact still knows: its element is live and no
 // other live element uses the same edge (QuadMesh.surface, mesh/QuadMesh.java:188-226).
-// Every load is issued without waiting on an earlier test (an edge of a quad mesh has at most
-// one partner, so that case is a fixed-length step the compiler can schedule with others),
-// so a segment costs the three dependent loads of the CSR walk.
+// An edge of a conforming quad mesh has at most one partner, read from cPartner: two
+// dependent loads. An edge with several partners walks the CSR.
 __device__ __forceinline__ bool seg_live(const Dev& d, int p) {
     bool live = d.cDead[p >> 2] == 0;
-    const int k0 = d.cShareStart[p], k1 = d.cShareStart[p + 1];
-    if (k0 < k1) live &= d.cDead[d.cShareOf[k0] >> 2] != 0;
-    for (int k = k0 + 1; k < k1; k++) live &= d.cDead[d.cShareOf[k] >> 2] != 0;
+    const int partner = d.cPartner[p];
+    if (partner >= 0) {
+        live &= d.cDead[partner >> 2] != 0;
+    } else if (partner == -2) {
+        for (int k = d.cShareStart[p]; k < d.cShareStart[p + 1]; k++) live &= d.cDead[d.cShareOf[k] >> 2] != 0;
+    }
     return live;
 }
 
-// Whether node n touches a live segment: its segments (eight for an interior node of a quad
-// mesh) tested together, the first eight as a fixed-length step.
+// Node n's first eight segments (all of them on a quad mesh) from the fixed-width copy, -1
+// where it has fewer.
+__device__ __forceinline__ void node_segs8(const Dev& d, int n, int (&seg)[8]) {
+    const int4 a = d.cNodeSeg8[2 * n], b = d.cNodeSeg8[2 * n + 1];
+    seg[0] = a.x; seg[1] = a.y; seg[2] = a.z; seg[3] = a.w;
+    seg[4] = b.x; seg[5] = b.y; seg[6] = b.z; seg[7] = b.w;
+}
+
+// Whether node n touches a live segment: the first eight segments tested together, any
+// further ones through the CSR.
 __device__ __forceinline__ bool node_on_surface(const Dev& d, int n) {
-    const int k0 = d.cNodeSegStart[n], k1 = d.cNodeSegStart[n + 1];
+    int seg[8];
+    node_segs8(d, n, seg);
     bool on = false;
 #pragma unroll
     for (int k = 0; k < 8; k++)
-        if (k0 + k < k1) on |= d.cActive[d.cNodeSegOf[k0 + k]] != 0;
+        if (seg[k] >= 0) on |= d.cActive[seg[k]] != 0;
+    const int k0 = d.cNodeSegStart[n], k1 = d.cNodeSegStart[n + 1];
     for (int k = k0 + 8; k < k1; k++) on |= d.cActive[d.cNodeSegOf[k]] != 0;
     return on;
 }
@@ -359,8 +371,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
         const int k0 = d.cNodeSegStart[n], k1 = d.cNodeSegStart[n + 1];
         int seg8[8];
         double2 nv8[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) seg8[k] = k0 + k < k1 ? d.cNodeSegOf[k0 + k] : -1;
+        node_segs8(d, n, seg8);
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const bool live = seg8[k] >= 0 && d.cActive[seg8[k]];

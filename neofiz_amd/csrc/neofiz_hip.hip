@@ -147,6 +147,9 @@ struct Dev {
     const int* cShareOf;
     const int* cNodeSegStart;  // [N+1] CSR: potential segments touching a node, ascending
     const int* cNodeSegOf;
+    const int* cPartner;       // [4E] the one other potential segment with the same edge, -1
+                               // if none, -2 if several (then the CSR above is walked)
+    const int4* cNodeSeg8;     // [2N] a node's first eight CSR entries, -1 padded
     double* cLen;            // [4E]
     double2* cNrm;           // [4E] outward unit normal
     double2* cPn;            // [N] node normal
@@ -1947,6 +1950,25 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
         d.cShareOf = shareOfD;
         d.cNodeSegStart = nodeStartD;
         d.cNodeSegOf = nodeOfD;
+        // Fixed-width copies of the two CSRs for the common case, so that a test costs one
+        // dependent load less.
+        std::vector<int> partner(4 * (size_t)E);
+        for (int p = 0; p < 4 * E; p++) {
+            const int c = shareStart[p + 1] - shareStart[p];
+            partner[p] = c == 0 ? -1 : (c == 1 ? shareOf[shareStart[p]] : -2);
+        }
+        std::vector<int> seg8(8 * (size_t)N, -1);
+        for (int n = 0; n < N; n++)
+            for (int k = nodeStart[n]; k < nodeStart[n + 1] && k < nodeStart[n] + 8; k++)
+                seg8[8 * (size_t)n + (k - nodeStart[n])] = nodeOf[k];
+        int* partnerD = nullptr;
+        int4* seg8D = nullptr;
+        TRY(s->alloc(&partnerD, 4 * (size_t)E));
+        TRY(s->alloc(&seg8D, 2 * (size_t)N));
+        HIP_TRY(hipMemcpy(partnerD, partner.data(), sizeof(int) * partner.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(seg8D, seg8.data(), sizeof(int) * seg8.size(), hipMemcpyHostToDevice));
+        d.cPartner = partnerD;
+        d.cNodeSeg8 = seg8D;
     }
     // new Contact(mesh) :273-280: nothing forgotten, no history, audits from zero.
     HIP_TRY(hipMemsetAsync(d.cDead, 0, E, s->stream));
