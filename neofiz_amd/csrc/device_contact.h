@@ -2,16 +2,17 @@
 //
 // Included by neofiz_hip.hip after the Dev and Ctl definitions.
 //
-// Each assembly that gathers forces runs six launches after the gather and before the
+// Each assembly that gathers forces runs five launches after the gather and before the
 // nodal update (Contact.begin / search / accumulate, :424-462):
 //
-//   k_c_rebuild, k_c_rebuild_nodes
-//                 only when an element was forgotten since the last rebuild (adopt,
-//                 :290-322): live flags of every potential segment, then of every surface
-//                 node, compact lists of both, lastMaster cleared.
-//   k_c_geometry  segment lengths and outward normals (:465-487), the grid's bounding box
-//                 and longest segment (exact min/max on order-preserving bit keys), and the
-//                 clearing of the hash heads and per-segment pair lists.
+//   k_c_rebuild   only when an element was forgotten since the last rebuild (adopt,
+//                 :290-322): live flags and the compact list of every potential segment.
+//   k_c_nodes_geometry
+//                 the rest of that rebuild (live flags and compact list of surface nodes,
+//                 lastMaster cleared) beside geometry: segment lengths and outward normals
+//                 (:465-487), the grid's bounding box and longest segment (exact min/max on
+//                 order-preserving bit keys), and the clearing of the hash heads and
+//                 per-segment pair lists.
 //   k_c_grid      node normals in the reference's order (:488-499) and every segment end
 //                 filed under its grid cell (:515-551).
 //   k_c_search    per surface node, the deepest penetrated segment (:570-658) and the
@@ -228,7 +229,7 @@ __device__ __forceinline__ void block_append(unsigned mask, int first, int* coun
 
 // adopt() :290-322 when the contact has forgotten elements since the last rebuild. The
 // counters were zeroed by whoever raised the stale flag. Every assembly: the grid's
-// reductions are reset here, ahead of k_c_geometry.
+// reductions are reset here, ahead of the geometry part of k_c_nodes_geometry.
 __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) {
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
@@ -258,12 +259,11 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
     block_append<C_CHUNK_SEG>(mask, first, &c->cNAct, d.cAct, E4);
 }
 
-// The second half of the rebuild: surface nodes from the live flags just written.
-__global__ void __launch_bounds__(BLOCK) k_c_rebuild_nodes(Dev d, int q, int checkGo) {
-    if (c_skip(d, q, checkGo)) return;
+// The second half of the rebuild: surface nodes from the live flags just written (workgroup
+// `block` of the node part).
+__device__ void c_rebuild_nodes(const Dev& d, int block) {
     Ctl* c = d.ctl;
-    if (!c->surfaceStale) return;
-    const int first = blockIdx.x * BLOCK * C_CHUNK_NODE + threadIdx.x;
+    const int first = block * BLOCK * C_CHUNK_NODE + threadIdx.x;
     unsigned mask = 0;
 #pragma unroll
     for (int j = 0; j < C_CHUNK_NODE; j++) {
@@ -279,14 +279,13 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild_nodes(Dev d, int q, int che
     block_append<C_CHUNK_NODE>(mask, first, &c->cNSurf, d.cSurf, d.N);
 }
 
-// geometry() :465-487 per live segment; the grid's box and longest segment; clears.
-__global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo) {
-    if (c_skip(d, q, checkGo)) return;
+// geometry() :465-487 per live segment; the grid's box and longest segment; clears
+// (workgroup `block` of `nblocks` in the geometry part).
+__device__ void c_geometry(const Dev& d, int block, int nblocks) {
     Ctl* c = d.ctl;
     const int nAct = c->cNAct;
-    const int stride = gridDim.x * BLOCK;
-    const int t0 = blockIdx.x * BLOCK + threadIdx.x;
-    if (t0 == 0) c->surfaceStale = 0;
+    const int stride = nblocks * BLOCK;
+    const int t0 = block * BLOCK + threadIdx.x;
     unsigned long long box[4] = {~0ULL, ~0ULL, ~0ULL, ~0ULL};
     unsigned long long longest = 0ULL;
     for (int t = t0; t < nAct; t += stride) {
@@ -352,12 +351,25 @@ __global__ void __launch_bounds__(BLOCK) k_c_geometry(Dev d, int q, int checkGo)
     for (unsigned h = t0; h < HF; h += stride) d.cFHead[h] = -1;
 }
 
+// The rest of the rebuild (surface nodes, only after an erosion) and geometry() side by side:
+// they read different outputs of k_c_rebuild and write different arrays, so the first
+// nodeBlocks workgroups do the one and the others the other, and their loads overlap.
+__global__ void __launch_bounds__(BLOCK) k_c_nodes_geometry(Dev d, int q, int checkGo, int nodeBlocks) {
+    if (c_skip(d, q, checkGo)) return;
+    if ((int)blockIdx.x < nodeBlocks) {
+        if (d.ctl->surfaceStale) c_rebuild_nodes(d, blockIdx.x);
+        return;
+    }
+    c_geometry(d, blockIdx.x - nodeBlocks, gridDim.x - nodeBlocks);
+}
+
 // Node normals :488-499 (the node's live segments in surface order) and the grid :534-551.
 // The launch is three equal thirds, one per independent part (node normals, the coarse grid,
 // the capture grid), so that their dependent loads overlap instead of queueing in each thread.
 __global__ void __launch_bounds__(BLOCK) k_c_grid(Dev d, int q, int checkGo) {
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->surfaceStale = 0;  // the rebuild has been read
     const int nAct = c->cNAct, nSurf = c->cNSurf;
     const int third = gridDim.x / 3;
     const int part = blockIdx.x / third;

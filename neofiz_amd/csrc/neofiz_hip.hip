@@ -1360,8 +1360,8 @@ struct nf_solver {
         const int gn = std::min(blocks_for(N), CONTACT_GRID);
         if (a) HIP_TRY(hipEventRecord(a, stream));
         hipLaunchKernelGGL(k_c_rebuild, dim3(blocks_for_chunks(4 * E, C_CHUNK_SEG)), dim3(BLOCK), 0, stream, d, parity, checkGo);
-        hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(blocks_for_chunks(N, C_CHUNK_NODE)), dim3(BLOCK), 0, stream, d, parity, checkGo);
-        hipLaunchKernelGGL(k_c_geometry, dim3(g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        const int nb = blocks_for_chunks(N, C_CHUNK_NODE);
+        hipLaunchKernelGGL(k_c_nodes_geometry, dim3(nb + g4), dim3(BLOCK), 0, stream, d, parity, checkGo, nb);
         hipLaunchKernelGGL(k_c_grid, dim3(3 * g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_search, dim3(gn), dim3(BLOCK), 0, stream, d, parity, checkGo);
         hipLaunchKernelGGL(k_c_apply, dim3(d.cApplyBlocks), dim3(BLOCK), 0, stream, d, parity, checkGo);
@@ -1990,7 +1990,8 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
     d.cScale = scale;
     d.cRef = s->dt0;
     hipLaunchKernelGGL(k_c_rebuild, dim3(blocks_for_chunks(4 * E, C_CHUNK_SEG)), dim3(BLOCK), 0, s->stream, d, s->q, 0);
-    hipLaunchKernelGGL(k_c_rebuild_nodes, dim3(blocks_for_chunks(N, C_CHUNK_NODE)), dim3(BLOCK), 0, s->stream, d, s->q, 0);
+    const int nb = blocks_for_chunks(N, C_CHUNK_NODE);  // the node part alone: no geometry yet
+    hipLaunchKernelGGL(k_c_nodes_geometry, dim3(nb), dim3(BLOCK), 0, s->stream, d, s->q, 0, nb);
     HIP_TRY(hipGetLastError());
     // The lists are built: clear the flag so the first assembly does not append them again.
     TRY(s->sync());
