@@ -2,11 +2,12 @@
 //
 // Included by neofiz_hip.hip after the Dev and Ctl definitions.
 //
-// Each assembly that gathers forces runs five launches after the gather and before the
+// Each assembly that gathers forces runs four launches after the gather and before the
 // nodal update (Contact.begin / search / accumulate, :424-462):
 //
-//   k_c_rebuild   only when an element was forgotten since the last rebuild (adopt,
-//                 :290-322): live flags and the compact list of every potential segment.
+//   (c_rebuild, on extra workgroups of the gather launch) only when an element was forgotten
+//                 since the last rebuild (adopt, :290-322): live flags and the compact list of
+//                 every potential segment.
 //   k_c_nodes_geometry
 //                 the rest of that rebuild (live flags and compact list of surface nodes,
 //                 lastMaster cleared) beside geometry: segment lengths and outward normals
@@ -227,13 +228,14 @@ __device__ __forceinline__ void block_append(unsigned mask, int first, int* coun
         }
 }
 
-// adopt() :290-322 when the contact has forgotten elements since the last rebuild. The
-// counters were zeroed by whoever raised the stale flag. Every assembly: the grid's
-// reductions are reset here, ahead of the geometry part of k_c_nodes_geometry.
-__global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) {
-    if (c_skip(d, q, checkGo)) return;
+// adopt() :290-322 when the contact has forgotten elements since the last rebuild (workgroup
+// `block` of the segment part). The counters were zeroed by whoever raised the stale flag.
+// Every assembly: the grid's reductions are reset here, ahead of the geometry part of
+// k_c_nodes_geometry. Inside a step this runs on extra workgroups of the gather launch,
+// since it reads only the elements' forgotten flags.
+__device__ void c_rebuild(const Dev& d, int block) {
     Ctl* c = d.ctl;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (block == 0 && threadIdx.x == 0) {
         const unsigned long long hi = order_key(NFD_MAX_VALUE);
         c->cBox[0] = hi;
         c->cBox[1] = hi;
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
     }
     if (!c->surfaceStale) return;
     const int E4 = 4 * d.E;
-    const int first = blockIdx.x * BLOCK * C_CHUNK_SEG + threadIdx.x;
+    const int first = block * BLOCK * C_CHUNK_SEG + threadIdx.x;
     unsigned mask = 0;
 #pragma unroll
     for (int j = 0; j < C_CHUNK_SEG; j++) {
@@ -257,6 +259,11 @@ __global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) 
         }
     }
     block_append<C_CHUNK_SEG>(mask, first, &c->cNAct, d.cAct, E4);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_c_rebuild(Dev d, int q, int checkGo) {
+    if (c_skip(d, q, checkGo)) return;
+    c_rebuild(d, blockIdx.x);
 }
 
 // The second half of the rebuild: surface nodes from the live flags just written (workgroup

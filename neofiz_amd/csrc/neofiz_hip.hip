@@ -321,6 +321,10 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
     const bool gathered = phase == PHASE_STEP || gatherFirst;
     const bool gather = gathered && part != PART_REST;
     if (part == PART_GATHER) {
+        if ((int)blockIdx.x >= d.nodalBlocks) {  // the workgroups past the nodes: the contact rebuild
+            c_rebuild(d, blockIdx.x - d.nodalBlocks);
+            return;
+        }
         const int i = blockIdx.x * BLOCK + threadIdx.x;
         if (gather && i < d.N) d.f[i] = gather_node(d, i);
         return;
@@ -1341,8 +1345,10 @@ struct nf_solver {
 
     template <bool WALL>
     int launchGNT(int phase, int parity, int gatherFirst, int part, hipEvent_t a, hipEvent_t b) {
-        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part);
-        else hipLaunchKernelGGL(k_gn<WALL>, dim3(blocks_for(N)), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part);
+        // the gather before the contact launches carries the contact's segment rebuild
+        const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0);
+        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part);
+        else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
@@ -1359,7 +1365,6 @@ struct nf_solver {
         const int g4 = std::min(blocks_for(4 * E), CONTACT_GRID);
         const int gn = std::min(blocks_for(N), CONTACT_GRID);
         if (a) HIP_TRY(hipEventRecord(a, stream));
-        hipLaunchKernelGGL(k_c_rebuild, dim3(blocks_for_chunks(4 * E, C_CHUNK_SEG)), dim3(BLOCK), 0, stream, d, parity, checkGo);
         const int nb = blocks_for_chunks(N, C_CHUNK_NODE);
         hipLaunchKernelGGL(k_c_nodes_geometry, dim3(nb + g4), dim3(BLOCK), 0, stream, d, parity, checkGo, nb);
         hipLaunchKernelGGL(k_c_grid, dim3(3 * g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
