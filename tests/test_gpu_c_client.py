@@ -1,0 +1,55 @@
+"""The library driven from plain C (tests/c_client/step_loop.c, built by
+__graft_entry__.build()): the 16 x 160 Johnson-Cook rod set up and stepped through
+include/neofiz.h alone ends bit for bit where the same run through the Python host mirror
+ends. The C client stands in for the Java FFM binding, which needs a JDK this image lacks."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from neofiz_amd import taylor
+from neofiz_amd.solver import ExplicitSolver, QuadMesh
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLIENT = os.path.join(ROOT, "tests", "c_client", "_build", "step_loop")
+NR, NZ, STEPS = 16, 160, 600
+
+
+def test_c_client_matches_the_python_mirror(tmp_path):
+    if not os.path.exists(CLIENT):
+        pytest.fail(f"{CLIENT} is not built; run __graft_entry__.build() first")
+    mesh, solver, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER, taylor.ROD_SPEED,
+                                   NR, NZ, friction=taylor.ROD_JC_FRICTION, solver_cls=ExplicitSolver)
+    solver.run(STEPS)
+    want = [solver.nodeField(k) for k in ("ur", "uz", "vr", "vz")]
+    want_time = solver.time()
+    solver.shutdown()
+
+    m = taylor.ROD_JC.to_c()
+    j = m.johnson_cook
+    axis = np.asarray(QuadMesh.axisNodes(NR, NZ), dtype=np.int32)
+    path_in, path_out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(path_in, "wb") as f:
+        f.write(struct.pack("<5i", mesh.nodeCount, mesh.elementCount, axis.size, m.has_johnson_cook, STEPS))
+        f.write(np.ascontiguousarray(mesh.r, "<f8").tobytes())
+        f.write(np.ascontiguousarray(mesh.z, "<f8").tobytes())
+        f.write(np.ascontiguousarray(mesh.conn, "<i4").tobytes())
+        f.write(axis.astype("<i4").tobytes())
+        f.write(struct.pack("<6d", m.youngs_modulus, m.poisson_ratio, m.density, m.yield_stress,
+                            m.isotropic_hardening, m.kinematic_hardening))
+        f.write(struct.pack("<10d", j.a, j.b, j.n, j.c, j.m, j.reference_strain_rate, j.room_temperature,
+                            j.melt_temperature, j.specific_heat, j.taylor_quinney))
+        f.write(struct.pack("<2d", taylor.ROD_SPEED, taylor.ROD_JC_FRICTION))
+    print(f"C client: {NR}x{NZ} rod, {STEPS} steps", flush=True)
+    done = subprocess.run([CLIENT, str(path_in), str(path_out)], capture_output=True, text=True, timeout=120)
+    assert done.returncode == 0, done.stderr
+    got = np.fromfile(path_out, dtype="<f8")
+    n = mesh.nodeCount
+    assert got.size == 4 * n + 1
+    for k in range(4):
+        assert np.array_equal(got[k * n:(k + 1) * n].view(np.uint64), want[k].view(np.uint64)), k
+    assert got[-1] == want_time
