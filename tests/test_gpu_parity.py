@@ -184,6 +184,27 @@ def test_run_until_matches_step_count_at_largest_mesh():
     assert_parity(g, r, "JC 128x1280 prefix")
 
 
+def test_benchmark_rod_2000_steps_bitwise():
+    """The benchmark's 128 x 1280 Johnson-Cook rod (164k elements) through its first 2000
+    steps, the bench's warmup: the plastic zone is established, tens of thousands of points
+    run the Newton solve each step, and every state array is bit-identical to the oracle's
+    (threaded on the host's cores)."""
+    import os
+    from oracle.pyoracle import OracleSolver
+    out = []
+    for cls in (ExplicitSolver, OracleSolver):
+        solver = rod_builder(taylor.ROD_JC, 128, 1280, 0.09)(cls)
+        if cls is OracleSolver:
+            solver.setThreads(min(16, os.cpu_count() or 1))
+        solver.run(2000)
+        out.append((snapshot(solver), solver.state()))
+        print(f"{cls.__name__}: 2000 steps done", flush=True)
+        solver.shutdown()
+    (g, gs), (r, rs) = out
+    assert gs.steps == rs.steps == 2000 and gs.time == rs.time
+    assert_parity(g, r, "JC 128x1280 2000 steps")
+
+
 @pytest.mark.parametrize("with_damage", [False, True])
 def test_point_paths_bitwise(hip, oracle, with_damage):
     """J2 / Johnson-Cook / damage at 4096 random material points: device == oracle."""
