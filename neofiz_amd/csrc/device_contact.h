@@ -174,7 +174,7 @@ __device__ __forceinline__ int c_fine(double x, double origin, double h) {
 constexpr int C_MAX_CAPTURE_CELLS = 1 << 16;  // per segment; beyond it the step falls back
 // Each capture bucket keeps C_SUB lists (an entry goes to list `entry mod C_SUB`), and the
 // search walks them with C_SUB lanes per surface node, so no lane follows a long chain.
-constexpr int C_SUB = 8;
+constexpr int C_SUB = 16;
 
 __device__ __forceinline__ bool c_skip(const Dev& d, int q, int checkGo) {
     return checkGo && !d.ctl->s[q].go;
