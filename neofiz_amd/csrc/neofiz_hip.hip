@@ -1922,8 +1922,10 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
         TRY(s->alloc(&d.cPair, (size_t)CPAIR * N));
         // Capture grid: entries for four cells per segment on average, at least 2^20.
         d.cFCap = (int)std::min<size_t>(0x7FFFFFFF, std::max<size_t>(1u << 20, 16 * (size_t)E));
+        // buckets for 4 per live segment (at most 4E segments), capped so that the heads
+        // (C_SUB per bucket) stay within 1 GiB; past the cap buckets are shared more often
         d.cFHashMax = 64;
-        while (d.cFHashMax < 16u * (unsigned)E && d.cFHashMax < (1u << 28)) d.cFHashMax <<= 1;
+        while (d.cFHashMax < 16u * (unsigned)E && d.cFHashMax < (1u << 24)) d.cFHashMax <<= 1;
         TRY(s->alloc(&d.cFHead, (size_t)d.cFHashMax * C_SUB));
         TRY(s->alloc(&d.cFEnt, (size_t)d.cFCap));
         TRY(s->alloc(&d.cSegX, 4 * (size_t)E));
