@@ -6,10 +6,11 @@
 // (solver/Damage.java:315, :321-322). Library implementations differ between glibc and the
 // AMD device library in the last bit, and a plastic run amplifies a last-bit difference over
 // tens of thousands of steps. Both the CPU restatement and the HIP kernels therefore call the
-// functions below, which use only IEEE-754 add, subtract, multiply, divide and square root
-// (each correctly rounded on both targets), integer bit manipulation and comparisons. The
-// file must be compiled with floating-point contraction off (-ffp-contract=off) on every
-// target, so that no multiply-add is fused on one side and not the other.
+// functions below, which use only IEEE-754 add, subtract, multiply, divide, square root and
+// scaleB by a power of two (each correctly rounded on both targets), integer bit manipulation
+// and comparisons. The file must be compiled with floating-point contraction off
+// (-ffp-contract=off) on every target, so that no multiply-add is fused on one side and not
+// the other.
 //
 // Methods (all from the mathematics; the tables in nf_math_tables.h are correctly rounded
 // values computed by gen_math_tables.py with mpmath):
@@ -174,9 +175,9 @@ NF_HD double log1p(double x) {
 
 // exp(a + b) with |b| <= ulp(a). Table method: a + b = (64 k + j) ln2/64 + r, |r| <= ln2/128;
 // exp = 2^k 2^(j/64) (1 + r + r^2 (1/2 + r/6 + ... + r^5/5040)), the polynomial by Estrin's
-// scheme and the table value carried as hi + lo. Branch-free: the scaling by 2^k is two
-// exact multiplications by powers of two (one rounding at most, into the subnormal range),
-// and the out-of-range results are selected at the end.
+// scheme and the table value carried as hi + lo. Branch-free: the scaling by 2^k is IEEE's
+// scaleB (ldexp: exact, or one rounding into the subnormal range), and the out-of-range
+// results are selected at the end.
 // The main path of exp_dd for a in [-745.2, 709.782712893384].
 NF_HD double exp_dd_main(double a, double b, const double (*ET)[2]) {
     const double shifter = 0x1.8p52;
@@ -196,9 +197,8 @@ NF_HD double exp_dd_main(double a, double b, const double (*ET)[2]) {
     const double p = r + E;
     const double th = ET[j][0], tl = ET[j][1];
     const double y = th + (tl + (th * r + (th * E + tl * p)));
-    const int k1 = k / 2;  // |k1|, |k - k1| <= 538: both factors are normal doubles
-    const int k2 = k - k1;
-    return (y * from_bits((uint64_t)(k1 + 1023) << 52)) * from_bits((uint64_t)(k2 + 1023) << 52);
+    // y 2^k with one correct rounding (IEEE scaleB: exact unless the result is subnormal)
+    return __builtin_ldexp(y, k);
 }
 
 NF_HD double exp_dd(double a0, double b, const double (*ET)[2]) {

@@ -46,6 +46,7 @@ def rel(a, b):
 @pytest.mark.parametrize("fn,lo,hi,second", [
     (0, -40.0, 40.0, None),     # log of exp(U)
     (1, -50.0, 50.0, None),     # exp
+    (1, -745.2, -700.0, None),  # exp into the subnormal range (the scaling's one rounding)
     (2, -30.0, 3.0, (0.05, 1.5)),  # pow(exp(U), y)
     (3, -2.0, 2.0, (-2.0, 2.0)),   # atan2
     (4, -1.6, 1.6, None),       # sin
