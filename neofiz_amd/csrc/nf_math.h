@@ -112,8 +112,7 @@ NF_HD void log_dd_impl(double x, double& hi, double& lo, const double (*LT)[3]) 
     const bool high = m1 >= 1.5;
     const double m = high ? m1 * 0.5 : m1;
     k += high ? 1 : 0;
-    const double shifter = 0x1.8p52;
-    const double jd = (m * 128.0 + shifter) - shifter;  // nearest integer, 96..192
+    const double jd = __builtin_rint(m * 128.0);  // nearest integer (ties to even), 96..192
     const int j = (int)jd - 96;
     const double F = jd * 0x1p-7;
     const double f = m - F;  // exact (Sterbenz)
@@ -180,8 +179,7 @@ NF_HD double log1p(double x) {
 // results are selected at the end.
 // The main path of exp_dd for a in [-745.2, 709.782712893384].
 NF_HD double exp_dd_main(double a, double b, const double (*ET)[2]) {
-    const double shifter = 0x1.8p52;
-    const double nd = (a * NFM_INV_L64 + shifter) - shifter;
+    const double nd = __builtin_rint(a * NFM_INV_L64);  // nearest integer (ties to even)
     const int n = (int)nd;
     const int j = n & 63;
     const int k = (n - j) / 64;
