@@ -133,8 +133,11 @@ NF_HD void log_dd_impl(double x, double& hi, double& lo, const double (*LT)[3]) 
     const double P = r2 * q;
     const double kd = (double)k;
     double s1, e1, s2, e2;
-    two_sum(kd * NFM_LN2_HI, LT[j][1], s1, e1);
-    two_sum(s1, r, s2, e2);
+    // Both sums are exact with the three-operation form: |k ln2| >= 0.69 > 0.41 >= |log F|
+    // when k != 0, and for k = 0 and F != 1, |log F| >= log(128/127) > 2^-7.5 >= |r| (for
+    // F = 1 the first operand is 0).
+    fast_two_sum(kd * NFM_LN2_HI, LT[j][1], s1, e1);
+    fast_two_sum(s1, r, s2, e2);
     const double l = (e1 + e2) + ((kd * NFM_LN2_LO + LT[j][2]) + (r_lo + P));
     fast_two_sum(s2, l, hi, lo);
 }
