@@ -107,18 +107,25 @@ NF_HD void log_dd_impl(double x, double& hi, double& lo, const double (*LT)[3]) 
     const bool sub = MAY_BE_SUB && b0 < 0x0010000000000000ULL;  // subnormal (or zero)
     const double xs = sub ? x * 0x1p54 : x;
     const uint64_t b = bits_of(xs);
-    int k = (int)(b >> 52) - 1023 - (sub ? 54 : 0);
-    const double m1 = from_bits((b & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);
-    const bool high = m1 >= 1.5;
-    const double m = high ? m1 * 0.5 : m1;
-    k += high ? 1 : 0;
+    // x = 2^k m, m in [3/4, 3/2), in exact integer arithmetic on the positive normal xs:
+    // subtracting the bits of 3/4 leaves k in the exponent field (a mantissa of 1.5 or more
+    // carries into it, which is the case m = mantissa / 2), and taking k back out of the
+    // exponent leaves m.
+    // The low word of the bits does not take part.
+    const uint32_t bh = (uint32_t)(b >> 32);
+    const uint32_t th = bh - 0x3FE80000u;
+    const int k = ((int32_t)th >> 20) - (sub ? 54 : 0);
+    const double m = from_bits(((uint64_t)(bh - (th & 0xFFF00000u)) << 32) | (b & 0xFFFFFFFFULL));
     const double jd = __builtin_rint(m * 128.0);  // nearest integer (ties to even), 96..192
-    const int j = (int)jd - 96;
+    // The byte offset of F = jd / 128's row, 24 (jd - 96), in double arithmetic on small
+    // integers (exact), which the device computes at full rate, unlike an integer multiply.
+    const int off = (int)(jd * 24.0 - 2304.0);
+    const double* row = (const double*)((const char*)&LT[0][0] + off);
     const double F = jd * 0x1p-7;
     const double f = m - F;  // exact (Sterbenz)
     // r = f (1/F) and the remainder f - r F to full precision: r F = p + e exactly (e by an
     // exact fma), f - p exactly (Sterbenz), so r_lo = ((f - p) - e) / F carries the rest.
-    const double invF = LT[j][0];
+    const double invF = row[0];
     const double r = f * invF;
     double p, e;
     two_prod(r, F, p, e);
@@ -136,9 +143,9 @@ NF_HD void log_dd_impl(double x, double& hi, double& lo, const double (*LT)[3]) 
     // Both sums are exact with the three-operation form: |k ln2| >= 0.69 > 0.41 >= |log F|
     // when k != 0, and for k = 0 and F != 1, |log F| >= log(128/127) > 2^-7.5 >= |r| (for
     // F = 1 the first operand is 0).
-    fast_two_sum(kd * NFM_LN2_HI, LT[j][1], s1, e1);
+    fast_two_sum(kd * NFM_LN2_HI, row[1], s1, e1);
     fast_two_sum(s1, r, s2, e2);
-    const double l = (e1 + e2) + ((kd * NFM_LN2_LO + LT[j][2]) + (r_lo + P));
+    const double l = (e1 + e2) + ((kd * NFM_LN2_LO + row[2]) + (r_lo + P));
     fast_two_sum(s2, l, hi, lo);
 }
 
