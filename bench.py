@@ -14,24 +14,30 @@ Inputs are resident in HBM before the timed region starts.
 
 Extra objects on the line:
   roofline      the element kernel's algorithmic bytes per launch over its mean launch
-                duration in a profiled pass over the --profile-steps steps that follow the
-                timed ones (start/stop events recorded by each dispatch on the library's
-                stream), against the 8 TB/s HBM peak; traffic (HBM bytes per launch) from
-                the committed rocprofv3 PMC summary.
+                duration on the timed steps (replayed on an identical twin of the rod, whose
+                trajectory is bit-identical, with start/stop events recorded by each dispatch
+                on the library's stream), against the 8 TB/s HBM peak; traffic (HBM bytes per
+                launch) from the committed rocprofv3 PMC summary.
+  full_run      the same rod from t = 0 to 80 us: steps, element-updates/s, HBM fraction,
+                L/L0 and mushroom diameter, so the figure does not rest on one window.
+  ladder        the rods of the refinement ladder to 80 us: 8x80 linear hardening without
+                friction, 16x160 to 64x640 Johnson-Cook with mu = 0.09.
   cpu_baseline  the oracle (the C++ restatement, threaded like the reference's Parallel
-                pool) on the host's cores over a bounded sample of the same workload.
+                pool) on the host's cores: warm-up, then the best of three bursts.
   defect_field  device-side defect-field seeding of the ~200k-element perforation assembly
                 (kernel time per call, inputs resident) and the size-effect population of
                 3000 seeds, with the serial oracle on the same mesh beside it.
   perforation, sensitivity, batched
                 the multi-body contact scene, the derivative's cost, the 256-instance
-                ensemble against one-by-one runs.
+                ensemble against one-by-one runs (and its instances that reach 80 us alone).
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -40,15 +46,17 @@ HBM_PEAK_GBPS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "element_kernel_traffic.txt")
 
 
-def element_kernel_bytes(E, N, P):
+def element_kernel_bytes(E, N, kinematic=False):
     """Algorithmic bytes one launch of the reduced-integration element kernel must move.
 
-    Per element: connectivity 16; point state read and written 2 x 96 (stress 32, back
-    stress 32, plastic strain 8, plastic work 8, hourglass force 16); hourglass stiffness
-    written 8; four force slots written 64. Per node, read once: reference coordinates 16,
-    displacement 16, velocity 16.
+    Per element: connectivity 16; point state read and written 2 x 64 (stress 32, plastic
+    strain 8, plastic work 8, hourglass force 16), plus 2 x 32 of back stress when some
+    material hardens kinematically (without it the back stress is +0 forever and the kernel
+    neither loads nor stores it); hourglass stiffness written 8; four force slots written 64.
+    Per node, read once: reference coordinates 16, displacement 16, velocity 16.
     """
-    return E * (16 + 2 * 96 + 8 + 64) + N * 48
+    state = 96 if kinematic else 64
+    return E * (16 + 2 * state + 8 + 64) + N * 48
 
 
 def step_bytes(E, N):
@@ -57,23 +65,42 @@ def step_bytes(E, N):
     return 352.0 * E + 160.0 * N
 
 
-def cpu_baseline(nr, nz, seconds, threads):
-    """The oracle on the host's cores: steps from t = 0 of the same rod until `seconds` pass."""
+def cpu_baseline(nr, nz, seconds, threads, bursts=3):
+    """The oracle on the host's cores, measured as the reference measures (M0Cylinder.java:
+    931-942, Bench.java:123-132): a warm-up, then `bursts` timed bursts of about
+    `seconds` / bursts each, continuing the same run from t = 0; the best burst's rate."""
     from neofiz_amd import taylor
     from oracle.pyoracle import OracleSolver
     mesh, solver, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER,
                                    taylor.ROD_SPEED, nr, nz, friction=taylor.ROD_JC_FRICTION,
                                    solver_cls=OracleSolver)
     solver.setThreads(threads)
-    solver.run(2)  # warm the pool and the caches
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        solver.run(5)
-        steps += 5
-    dt = time.perf_counter() - t0
+    solver.run(20)  # warm-up: the pool, the caches, the impact onset
+    rates = []
+    for _ in range(bursts):
+        steps = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds / bursts:
+            solver.run(5)
+            steps += 5
+        rates.append(mesh.elementCount * steps / (time.perf_counter() - t0))
+    last = solver.steps()
     solver.shutdown()
-    return mesh.elementCount * steps / dt, steps
+    return max(rates), rates, last
+
+
+def host_cpu():
+    """(CPU model, logical CPUs of the host, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
 
 
 def stream_copy_gbps():
@@ -96,31 +123,40 @@ def ensemble_leg(count, cap, per_size):
     `per_size` instances of each mesh size through the single-instance path with the same
     end test; the one-by-one time of the whole ensemble is then estimated per mesh size as
     (measured seconds per step) x (the steps every instance of that size took, known exactly
-    from the batched run, whose states are bit-identical to solo runs)."""
+    from the batched run, whose states are bit-identical to solo runs).
+
+    `reached`: the same comparison restricted to the instances that reach 80 us within the
+    cap, as the ensemble is specified (each instance stops at 80 us): those instances run
+    again as a batch of their own with no cap, and one by one, `per_size` of each mesh size
+    measured and the rest estimated from them as above."""
     from neofiz_amd import ensemble, taylor
     insts = ensemble.draw(count)
     print(f"# ensemble: {count} instances, batched, cap {cap} steps", file=sys.stderr, flush=True)
     meshes, solvers, batched_s, tiles = ensemble.run_batched(insts, taylor.ROD_DURATION, cap)
     steps = [s.steps() for s in solvers]
-    reached = sum(1 for s in solvers if s.time() >= taylor.ROD_DURATION)
+    done = [s.time() >= taylor.ROD_DURATION for s in solvers]
+    reached = sum(done)
     work = sum(m.elementCount * k for m, k in zip(meshes, steps))
     sample_checks = []
-    by_size = {}
-    for inst, k in zip(insts, steps):
-        by_size.setdefault(inst.elements, []).append(k)
-    per_step = {}
     print(f"# ensemble: one-by-one, {per_size} per mesh size", file=sys.stderr, flush=True)
-    for e in sorted(by_size):
-        picks = [(i, n) for n, i in enumerate(insts) if i.elements == e][:per_size]
-        solo = ensemble.run_one_by_one([i for i, _ in picks], taylor.ROD_DURATION, cap)
-        per_step[e] = sum(sec for _, sec, _ in solo) / sum(k for k, _, _ in solo)
-        for (inst, n), (k, _, res) in zip(picks, solo):
-            batched_res = ensemble.results(meshes[n], solvers[n])
-            sample_checks.append(k == steps[n] and res == batched_res)
-    one_by_one_s = sum(per_step[e] * sum(ks) for e, ks in by_size.items())
-    for s in solvers:
-        s.shutdown()
-    return {
+
+    def one_by_one(indices):
+        """Seconds per step per mesh size, measured on `per_size` instances of each."""
+        by_size = {}
+        for n in indices:
+            by_size.setdefault(insts[n].elements, []).append(n)
+        per_step = {}
+        for e, ns in sorted(by_size.items()):
+            picks = ns[:per_size]
+            solo = ensemble.run_one_by_one([insts[n] for n in picks], taylor.ROD_DURATION, cap)
+            per_step[e] = sum(sec for _, sec, _ in solo) / sum(k for k, _, _ in solo)
+            for n, (k, _, res) in zip(picks, solo):
+                sample_checks.append(k == steps[n] and res == ensemble.results(meshes[n], solvers[n]))
+        estimate = sum(per_step[insts[n].elements] * steps[n] for n in indices)
+        return per_step, estimate, {e: len(ns) for e, ns in by_size.items()}
+
+    per_step, one_by_one_s, sizes = one_by_one(range(count))
+    out = {
         "instances": count, "step_cap": cap, "reached_end_time": reached,
         "element_steps": work, "value": work / batched_s, "seconds": batched_s,
         "unit": "element-updates/s",
@@ -128,9 +164,57 @@ def ensemble_leg(count, cap, per_size):
         "one_by_one_us_per_step": {str(e): v * 1e6 for e, v in per_step.items()},
         "speedup": one_by_one_s / batched_s,
         "tiles": {str(k): v for k, v in (tiles or {}).items()},
-        "solo_equals_batched": all(sample_checks),
+        "instances_per_size": {str(k): v for k, v in sizes.items()},
         "note": f"one-by-one measured on {per_size} instances per mesh size, scaled by the "
                 "ensemble's exact step counts"}
+    hit = [n for n in range(count) if done[n]]
+    if hit:
+        print(f"# ensemble: the {len(hit)} instances that reach 80 us, batched alone", file=sys.stderr, flush=True)
+        sub = [insts[n] for n in hit]
+        m2, s2, sec2, _ = ensemble.run_batched(sub, taylor.ROD_DURATION, 0)
+        same = all(b.steps() == steps[n] and ensemble.results(mb, b) == ensemble.results(meshes[n], solvers[n])
+                   for n, mb, b in zip(hit, m2, s2))
+        for b in s2:
+            b.shutdown()
+        per_hit, est_hit, sizes_hit = one_by_one(hit)
+        work_hit = sum(meshes[n].elementCount * steps[n] for n in hit)
+        out["reached"] = {"instances": len(hit), "element_steps": work_hit,
+                          "batched_seconds": sec2, "value": work_hit / sec2,
+                          "one_by_one_seconds_estimated": est_hit,
+                          "one_by_one_us_per_step": {str(e): v * 1e6 for e, v in per_hit.items()},
+                          "instances_per_size": {str(k): v for k, v in sizes_hit.items()},
+                          "speedup": est_hit / sec2, "same_results_as_full_batch": same}
+    out["solo_equals_batched"] = all(sample_checks)
+    for s_ in solvers:
+        s_.shutdown()
+    return out
+
+
+def rod_run(nr, nz, johnson_cook, cap):
+    """A whole rod from t = 0 to 80 us (or `cap` steps) through the single-instance path:
+    steps, wall seconds, element-updates/s and the harness's L/L0 and mushroom diameter
+    (validate/TaylorImpactCase.java:264-311)."""
+    from neofiz_amd import taylor
+    from neofiz_amd.solver import ExplicitSolver
+    mat = taylor.ROD_JC if johnson_cook else taylor.ROD_LINEAR
+    mu = taylor.ROD_JC_FRICTION if johnson_cook else 0.0
+    mesh, solver, _ = taylor.build(mat, taylor.ROD_LENGTH, taylor.ROD_DIAMETER, taylor.ROD_SPEED,
+                                   nr, nz, friction=mu, solver_cls=ExplicitSolver)
+    a0 = solver.audits()
+    t0 = time.perf_counter()
+    steps = solver.runUntil(taylor.ROD_DURATION, cap)
+    sec = time.perf_counter() - t0
+    m = taylor.measure(mesh, solver, taylor.ROD_LENGTH, a0.axial_momentum, a0.kinetic_energy)
+    st = solver.state()
+    solver.shutdown()
+    return {"mesh": f"{nr}x{nz}", "law": "johnson-cook, mu 0.09" if johnson_cook else "linear, mu 0",
+            "elements": mesh.elementCount, "steps": steps, "time_us": st.time * 1e6,
+            "reached_80us": st.time >= taylor.ROD_DURATION, "seconds": sec,
+            "element_updates_per_s": mesh.elementCount * steps / sec,
+            "L_over_L0": m["lengthRatio"], "mushroom_diameter_mm": m["mushroomDiameter"] * 1e3,
+            "max_plastic_strain": m["maxPlasticStrain"],
+            "momentum_balance_error": m["momentumBalanceError"],
+            "energy_balance_error": m["energyBalanceError"]}
 
 
 def sensitivity_leg(nr, nz):
@@ -257,7 +341,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--nr", type=int, default=128)
     ap.add_argument("--nz", type=int, default=1280)
-    ap.add_argument("--profile-steps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
@@ -265,12 +348,16 @@ def main():
     ap.add_argument("--no-sensitivity", action="store_true")
     ap.add_argument("--no-perforation", action="store_true")
     ap.add_argument("--no-defect", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true")
+    ap.add_argument("--no-ladder", action="store_true")
+    ap.add_argument("--full-run-cap", type=int, default=400000,
+                    help="step cap of the whole 80 us run of the benchmark rod")
     ap.add_argument("--defect-reps", type=int, default=20)
     ap.add_argument("--perforation-steps", type=int, default=2000)
     ap.add_argument("--perforation-warmup", type=int, default=1500)
     ap.add_argument("--ensemble", type=int, default=256, help="instances in the batched ensemble")
     ap.add_argument("--ensemble-cap", type=int, default=20000, help="step cap per instance")
-    ap.add_argument("--ensemble-solo", type=int, default=4, help="one-by-one sample per mesh size")
+    ap.add_argument("--ensemble-solo", type=int, default=8, help="one-by-one sample per mesh size")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -279,13 +366,17 @@ def main():
                                    "communication)"}))
         sys.exit(2)
 
+    import ctypes as C
     from neofiz_amd import taylor
     from neofiz_amd.solver import ExplicitSolver
 
+    def rod():
+        return taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER,
+                            taylor.ROD_SPEED, args.nr, args.nz,
+                            friction=taylor.ROD_JC_FRICTION, solver_cls=ExplicitSolver)
+
     print(f"# building the {args.nr}x{args.nz} rod", file=sys.stderr, flush=True)
-    mesh, solver, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER,
-                                   taylor.ROD_SPEED, args.nr, args.nz,
-                                   friction=taylor.ROD_JC_FRICTION, solver_cls=ExplicitSolver)
+    mesh, solver, _ = rod()
     E, N = mesh.elementCount, mesh.nodeCount
     solver.run(args.warmup)  # synchronous: returns with the device idle
     print(f"# warmup {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
@@ -294,23 +385,28 @@ def main():
     elapsed = time.perf_counter() - t0
     st = solver.state()
     assert st.steps == args.warmup + args.steps, "step count mismatch"
-
-    # Kernel durations over the next --profile-steps steps, in a separate pass: each launch
-    # carries start/stop events recorded by the dispatch itself (hipExtLaunchKernelGGL).
-    # Kept out of the timed region, which the event bookkeeping would slow.
-    import ctypes as C
-    b = solver._b
-    ms = (C.c_double * 3)()
-    counts = (C.c_int64 * 3)()
-    b.profile(solver._h, 1)
-    b.profile_read(solver._h, ms, counts)
-    solver.run(args.profile_steps)
-    b.profile_read(solver._h, ms, counts)
-    b.profile(solver._h, 0)
-    kernel_ms = [ms[k] / max(1, counts[k]) for k in range(3)]
+    final_u = solver.axialDisplacement()
     solver.shutdown()
 
-    elem_bytes = element_kernel_bytes(E, N, E)
+    # Kernel durations over the same steps: a twin of the rod, built and warmed identically
+    # (the step loop is deterministic, so its trajectory is the timed one bit for bit), runs
+    # the timed window again with start/stop events recorded by each dispatch
+    # (hipExtLaunchKernelGGL), which the timed run itself does without.
+    _, twin, _ = rod()
+    twin.run(args.warmup)
+    b = twin._b
+    ms = (C.c_double * 3)()
+    counts = (C.c_int64 * 3)()
+    b.profile(twin._h, 1)
+    b.profile_read(twin._h, ms, counts)
+    twin.run(args.steps)
+    b.profile_read(twin._h, ms, counts)
+    b.profile(twin._h, 0)
+    kernel_ms = [ms[k] / max(1, counts[k]) for k in range(3)]
+    same_window = bool(np.array_equal(twin.axialDisplacement().view(np.uint64), final_u.view(np.uint64)))
+    twin.shutdown()
+
+    elem_bytes = element_kernel_bytes(E, N, kinematic=False)
     achieved = elem_bytes / (kernel_ms[1] * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -319,7 +415,10 @@ def main():
             for line in f:
                 if ":" in line and not line.startswith("#"):
                     k, v = line.split(":", 1)
-                    t[k.strip()] = float(v)
+                    try:
+                        t[k.strip()] = float(v)
+                    except ValueError:
+                        pass
         if t.get("nr") == args.nr and t.get("nz") == args.nz:
             traffic = int(t["bytes_per_launch"])
 
@@ -342,16 +441,27 @@ def main():
                    "steps_from_t0": [args.warmup, args.warmup + args.steps],
                    "parallelism": "single-gpu"},
         "hbm_fraction_step": value * (step_bytes(E, N) / E) / (HBM_PEAK_GBPS * 1e9),
-        "kernel_ms": {"gather_nodal": kernel_ms[0], "element": kernel_ms[1]},
+        "kernel_ms": {"gather_nodal": kernel_ms[0], "element": kernel_ms[1],
+                      "window": "the timed steps, replayed on an identical twin with events on",
+                      "twin_trajectory_identical": same_window},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "bytes_per_launch": elem_bytes},
     }
     if not args.no_copy:
         try:
             line["stream_copy_GBps"] = stream_copy_gbps()
-        except Exception as exc:  # torch missing or no device: report, do not hide
+        except Exception as exc:  # no device: report, do not hide
             line["stream_copy_GBps"] = None
             line["stream_copy_error"] = str(exc)
+    if not args.no_full_run:
+        print(f"# full run: the {args.nr}x{args.nz} rod from t = 0 to 80 us", file=sys.stderr, flush=True)
+        fr = rod_run(args.nr, args.nz, True, args.full_run_cap)
+        fr["hbm_fraction_step"] = fr["element_updates_per_s"] * (step_bytes(E, N) / E) / (HBM_PEAK_GBPS * 1e9)
+        line["full_run"] = fr
+    if not args.no_ladder:
+        print("# ladder: 8x80 linear, 16x160 .. 64x640 Johnson-Cook to 80 us", file=sys.stderr, flush=True)
+        line["ladder"] = [rod_run(8, 80, False, 0)] + [rod_run(n, 10 * n, True, 0) for n in (16, 32, 64)]
     if not args.no_defect:
         line["defect_field"] = defect_leg(args.defect_reps, not args.no_cpu)
     if not args.no_perforation:
@@ -361,15 +471,22 @@ def main():
     if not args.no_ensemble:
         line["batched"] = ensemble_leg(args.ensemble, args.ensemble_cap, args.ensemble_solo)
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        threads = min(threads, 16)
-        print(f"# CPU baseline: oracle on {threads} threads for ~{args.cpu_seconds} s",
+        model, host_cpus, allowed = host_cpu()
+        # The pool is sized to the CPU share this process is given: OMP_NUM_THREADS where the
+        # host sets it (the GPU host allots its GPU's share of cores that way), otherwise every
+        # CPU this process may run on.
+        share = os.environ.get("OMP_NUM_THREADS")
+        threads = min(allowed, int(share)) if share and share.isdigit() and int(share) > 0 else allowed
+        print(f"# CPU baseline: oracle on {threads} threads, 3 bursts over ~{args.cpu_seconds} s",
               file=sys.stderr, flush=True)
-        rate, cpu_steps = cpu_baseline(args.nr, args.nz, args.cpu_seconds, threads)
+        rate, rates, last = cpu_baseline(args.nr, args.nz, args.cpu_seconds, threads)
         line["cpu_baseline"] = {
             "value": rate, "unit": "element-updates/s", "cores": threads, "kind": "port",
-            "sample": f"the same {args.nr}x{args.nz} rod, steps 2..{2 + cpu_steps} from t = 0, "
-                      f"oracle (C++ restatement) threaded with the reference's cursor schedule"}
+            "sample": f"the same {args.nr}x{args.nz} rod from t = 0: 20 warm-up steps, then the "
+                      f"best of 3 bursts of ~{args.cpu_seconds / 3:.0f} s (to step {last}), oracle "
+                      "(C++ restatement) threaded with the reference's cursor schedule",
+            "bursts": rates, "cpu_model": model, "host_cpus": host_cpus,
+            "cpus_allowed": allowed}
     print(json.dumps(line), flush=True)
 
 

@@ -109,6 +109,14 @@ int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double fric
  * :487-489, solver/Contact.java:273-364; enabled = 0 is setContact(null). */
 int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double damping,
                    double stiffness_scale);
+/* Test hook with no counterpart in the reference: lowers two capacities of the contact
+ * search so that its rare paths can be exercised. capture_capacity caps the entries of the
+ * capture grid (0 sends every assembly down the reference's own nine-cell walk, the path
+ * taken when the grid overflows; the pairs and forces are the same); max_terms caps the pair
+ * terms one node may take in one step (at most 32; a node over the cap makes the run call
+ * fail with NF_E_STATE after its steps). -1 restores either default. NF_E_STATE without
+ * contact. */
+int nf_contact_set_limits(nf_solver* s, int32_t capture_capacity, int32_t max_terms);
 /* setGravity :509-512 */
 int nf_set_gravity(nf_solver* s, double acceleration_r, double acceleration_z);
 /* setDamage(double[] fractureEnergy, double[] failureStrain) :729-791, one per element. */
@@ -143,10 +151,20 @@ int nf_put_node_field(nf_solver* s, int32_t field, const double* in);
 
 /* Gauss-point fields: stress() :930 (4 per point {rr, zz, tt, rz}), back stress (4 per
  * point), plasticStrain() :925, plasticWorkDensity() :1712, and the damage state
- * (solver/Damage.java:188-192: strength, softening, band width). */
+ * (solver/Damage.java:188-192: strength, softening, band width). The rest serve the damage()
+ * object (:794) and temperature(int) (:1832) in bulk:
+ *   TEMPERATURE            temperature(p) at every point (JohnsonCook.temperature,
+ *                          core/JohnsonCook.java:218-221); NaN at a point whose material has
+ *                          no Johnson-Cook law (where temperature(p) throws)
+ *   DAMAGE                 Damage.at(p, plasticStrain()[p]) (solver/Damage.java:342-347)
+ *   DAMAGE_FAILURE_STRAIN  the point's failure strain, DAMAGE_FRACTURE_ENERGY
+ *                          Damage.fractureEnergy(p) (solver/Damage.java:266-268)
+ * The damage fields read 0 when no damage model is set (damage() returns null). */
 enum nf_point_field {
     NF_POINT_STRESS = 0, NF_POINT_BACK, NF_POINT_PLASTIC_STRAIN, NF_POINT_PLASTIC_WORK,
     NF_POINT_DAMAGE_STRENGTH, NF_POINT_DAMAGE_SOFTENING, NF_POINT_DAMAGE_WIDTH,
+    NF_POINT_TEMPERATURE, NF_POINT_DAMAGE, NF_POINT_DAMAGE_FAILURE_STRAIN,
+    NF_POINT_DAMAGE_FRACTURE_ENERGY,
     NF_POINT_FIELD_COUNT
 };
 int nf_get_point_field(nf_solver* s, int32_t field, double* out);
@@ -173,6 +191,25 @@ typedef struct nf_state {
     int64_t contact_escaped_nodes;
 } nf_state;
 int nf_get_state(nf_solver* s, nf_state* out);
+
+/* Single-value queries of ExplicitSolver (solver/ExplicitSolver.java):
+ *   nf_temperature          temperature(int point) :1832-1840; NF_E_STATE when the point's
+ *                           material has no Johnson-Cook law, NF_E_ARGUMENT out of range
+ *   nf_plastic_front_radius plasticFrontRadius() :1867-1877 (largest reference-mesh corner
+ *                           radius of an element with a yielded point)
+ *   nf_most_damaged_element mostDamagedElement() :1778-1794 (-1 without damage or before any
+ *                           softening)
+ *   nf_bore_pressure        borePressure() :826-828 (the ramped pressure at the current time)
+ *   nf_mesh_z               meshZ(int node) :1928-1930 */
+int nf_temperature(nf_solver* s, int32_t point, double* out);
+int nf_plastic_front_radius(nf_solver* s, double* out);
+int nf_most_damaged_element(nf_solver* s, int32_t* out);
+int nf_bore_pressure(nf_solver* s, double* out);
+int nf_mesh_z(nf_solver* s, int32_t node, double* out);
+
+/* setDamage(double fractureEnergy, double[] failureStrain) :713-717: one fracture energy
+ * for every element, then as nf_set_damage. */
+int nf_set_damage_uniform(nf_solver* s, double fracture_energy, const double* failure_strain);
 
 /* Audits, solver/ExplicitSolver.java:875-886, :1610-1860. max_temperature is NaN when no
  * material has a Johnson-Cook law (temperature() throws there, :1834-1837). */

@@ -53,7 +53,9 @@ class DefectParamsC(C.Structure):
 NODE_FIELDS = {"ur": 0, "uz": 1, "vr": 2, "vz": 3, "fr": 4, "fz": 5, "mass": 6, "r": 7,
                "z": 8, "on_wall": 9, "fixed_r": 10, "fixed_z": 11}
 POINT_FIELDS = {"stress": 0, "back": 1, "plastic_strain": 2, "plastic_work": 3,
-                "damage_strength": 4, "damage_softening": 5, "damage_width": 6}
+                "damage_strength": 4, "damage_softening": 5, "damage_width": 6,
+                "temperature": 7, "damage": 8, "damage_failure_strain": 9,
+                "damage_fracture_energy": 10}
 ELEMENT_FIELDS = {"hg_r": 0, "hg_z": 1, "hg_stiffness": 2, "eroded": 3}
 
 _P = C.c_void_p
@@ -75,6 +77,7 @@ _PROTOTYPES = {
     "set_rigid_wall": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
     "set_contact": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_double]),
     "set_gravity": (C.c_int, [_P, C.c_double, C.c_double]),
+    "contact_set_limits": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "set_damage": (C.c_int, [_P, c_double_p, c_double_p]),
     "set_erosion": (C.c_int, [_P, C.c_double]),
     "set_mass_scaling": (C.c_int, [_P, C.c_double]),
@@ -89,6 +92,12 @@ _PROTOTYPES = {
     "get_element_field": (C.c_int, [_P, C.c_int32, c_double_p]),
     "get_state": (C.c_int, [_P, C.POINTER(StateC)]),
     "get_audits": (C.c_int, [_P, C.POINTER(AuditsC)]),
+    "temperature": (C.c_int, [_P, C.c_int32, c_double_p]),
+    "plastic_front_radius": (C.c_int, [_P, c_double_p]),
+    "most_damaged_element": (C.c_int, [_P, c_int32_p]),
+    "bore_pressure": (C.c_int, [_P, c_double_p]),
+    "mesh_z": (C.c_int, [_P, C.c_int32, c_double_p]),
+    "set_damage_uniform": (C.c_int, [_P, C.c_double, c_double_p]),
     "contact_pairs": (C.c_int, [_P, C.c_int32, c_int32_p, c_int32_p]),
     "contact_surface": (C.c_int, [_P, C.c_int32, c_int32_p, c_int32_p]),
     "enable_sensitivity": (C.c_int, [_P]),

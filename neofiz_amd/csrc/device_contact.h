@@ -776,7 +776,7 @@ __device__ __forceinline__ void c_insert(int (&t)[C_FAST_TERMS], int key) {
 // selection of the next larger key over the first C_MAX_TERMS terms in collection order,
 // which needs no per-thread arrays.
 __device__ __forceinline__ void c_apply_many(const Dev& d, int n, bool friction, int m) {
-    const int count = m < C_MAX_TERMS ? m : C_MAX_TERMS;
+    const int count = m < d.cMaxTerms ? m : d.cMaxTerms;
     double2 f = d.f[n];
     int prev = -1;
     for (int i = 0; i < count; i++) {
@@ -835,7 +835,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_apply(Dev d, int q, int checkGo) {
             int a, b;
             seg_ends(d, p, a, b);
             for (int s = d.cPairHead[p]; s >= 0; s = d.cPairNext[s]) {
-                if (m == C_MAX_TERMS) {  // the first C_MAX_TERMS are applied
+                if (m == d.cMaxTerms) {  // the first cMaxTerms (C_MAX_TERMS by default) are applied
                     c->cOverflow = 1;
                     break;
                 }

@@ -172,6 +172,7 @@ struct Dev {
     int cFCap;
     unsigned cFHashMax;
     double cFine;
+    int cMaxTerms;           // pair terms one node may take in one step (C_MAX_TERMS unless lowered)
 };
 
 __device__ __forceinline__ double pressureAt(const Dev& d, double t) {  // :1203-1207
@@ -1210,6 +1211,7 @@ struct nf_solver {
     bool anyJC = false;
     bool hasKin = false;  // some material has kinematic hardening
     bool tangent = false;  // parameter sensitivities are carried (nf_enable_sensitivity)
+    int cFCapAlloc = 0;    // capture-grid entries allocated (nf_contact_set_limits may use fewer)
     TDev td{};
     double wallPlane = 0.0, wallFriction = 0.0;
     // Optional per-kernel timing with HIP events on the handle's stream (nf_profile).
@@ -1471,8 +1473,19 @@ struct nf_solver {
 
     // run steps while time < endTime and fewer than maxSteps have been taken.
     int runSteps(double endTime, long long maxSteps, long long* taken) {
+        if (tangent && (hasContact || hasDamage || hasErosion))
+            return fail(NF_E_STATE, "the sensitivity kernels do not carry contact, damage or erosion");
         Step h;
-        TRY(readStep(h));
+        {
+            // The term-overflow flag reports on this call's steps alone.
+            Ctl full;
+            TRY(readCtl(full));
+            if (full.cOverflow) {
+                full.cOverflow = 0;
+                TRY(writeCtl(full));
+            }
+            h = full.s[q];
+        }
         const long long start = h.steps;
         if (taken) *taken = 0;
         if (!(h.time < endTime) || maxSteps == 0) return NF_OK;
@@ -1525,7 +1538,8 @@ struct nf_solver {
         TRY(harvest());
         if (taken) *taken = h.steps - start;
         if (full.cOverflow)
-            return fail(NF_E_STATE, "contact: a node received more than 32 pair terms in one step");
+            return fail(NF_E_STATE, "contact: a node received more than " + std::to_string(d.cMaxTerms) +
+                                        " pair terms in one step");
         return NF_OK;
     }
 };
@@ -1842,8 +1856,18 @@ int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double fric
     h.wallContact = h.wallStuck = 0;
     return s->writeCtl(h);
 }
+// A solver carrying parameter sensitivities differentiates the Taylor step alone: contact,
+// damage and erosion have no derivative kernels, so switching them on afterwards is refused
+// (switching contact off is allowed: it is already off).
+static int refuse_if_sensitive(const nf_solver* s, const char* what) {
+    if (!s->tangent) return NF_OK;
+    return fail(NF_E_STATE, std::string(what) + " cannot be set on a solver carrying sensitivities; "
+                "they cover reduced integration of one Johnson-Cook material without damage, erosion or contact");
+}
+
 int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double damping, double scale) {
     g_error.clear();
+    if (enabled) TRY(refuse_if_sensitive(s, "contact"));
     if (!enabled) {
         s->hasContact = false;
         s->d.contact = 0;
@@ -1922,6 +1946,8 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
         TRY(s->alloc(&d.cPair, (size_t)CPAIR * N));
         // Capture grid: entries for four cells per segment on average, at least 2^20.
         d.cFCap = (int)std::min<size_t>(0x7FFFFFFF, std::max<size_t>(1u << 20, 16 * (size_t)E));
+        s->cFCapAlloc = d.cFCap;
+        d.cMaxTerms = C_MAX_TERMS;
         // buckets for 4 per live segment (at most 4E segments), capped so that the heads
         // (C_SUB per bucket) stay within 1 GiB; past the cap buckets are shared more often
         d.cFHashMax = 64;
@@ -2009,6 +2035,17 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
     s->graphDirty = true;
     return NF_OK;
 }
+int nf_contact_set_limits(nf_solver* s, int32_t capture_capacity, int32_t max_terms) {
+    g_error.clear();
+    if (!s->hasContact) return fail(NF_E_STATE, "contact is not set");
+    if (max_terms == 0 || max_terms > C_MAX_TERMS || max_terms < -1)
+        return fail(NF_E_ARGUMENT, "max_terms lies in [1, " + std::to_string(C_MAX_TERMS) + "], or -1 for the default");
+    if (capture_capacity < -1) return fail(NF_E_ARGUMENT, "capture_capacity is a count, or -1 for the default");
+    s->d.cFCap = capture_capacity < 0 ? s->cFCapAlloc : std::min(capture_capacity, s->cFCapAlloc);
+    s->d.cMaxTerms = max_terms < 0 ? C_MAX_TERMS : max_terms;
+    s->graphDirty = true;
+    return NF_OK;
+}
 int nf_set_gravity(nf_solver* s, double ar, double az) {
     s->d.gravityR = ar;
     s->d.gravityZ = az;
@@ -2017,6 +2054,7 @@ int nf_set_gravity(nf_solver* s, double ar, double az) {
 }
 int nf_set_damage(nf_solver* s, const double* gf, const double* eps) {  // :729-791
     g_error.clear();
+    TRY(refuse_if_sensitive(s, "damage"));
     if (!gf || !eps) return fail(NF_E_ARGUMENT, "fracture energy and failure strain are required");
     for (const HostMaterial& m : s->materials) {
         const double worst = 2.0 * m.mu() + (2.0 / 3.0) * (m.raw.kinematic_hardening - m.raw.youngs_modulus);
@@ -2084,6 +2122,7 @@ int nf_set_damage(nf_solver* s, const double* gf, const double* eps) {  // :729-
 }
 int nf_set_erosion(nf_solver* s, double threshold) {  // :549-561
     g_error.clear();
+    TRY(refuse_if_sensitive(s, "erosion"));
     if (!s->hasDamage) return fail(NF_E_STATE, "erosion needs a failure model; call setDamage first");
     if (!(threshold > 0.0) || threshold > 1.0)
         return fail(NF_E_ARGUMENT, "the erosion threshold is a damage, so it lies in (0, 1]");
@@ -2215,6 +2254,39 @@ int nf_get_point_field(nf_solver* s, int32_t field, double* out) {
         }
         return NF_OK;
     }
+    if (field == NF_POINT_TEMPERATURE) {  // JohnsonCook.temperature, core/JohnsonCook.java:218-221
+        std::vector<double> work(P);
+        TRY(copy(d.work, work.data()));
+        TRY(s->sync());
+        for (int p = 0; p < P; p++) {
+            const HostMaterial& m = s->materials[s->materialOf.empty() ? 0 : s->materialOf[p / s->ppe]];
+            const nf_johnson_cook& j = m.raw.johnson_cook;
+            out[p] = m.raw.has_johnson_cook ? j.room_temperature + j.taylor_quinney * work[p] / (m.raw.density * j.specific_heat)
+                                            : NAN;
+        }
+        return NF_OK;
+    }
+    if (field == NF_POINT_DAMAGE) {  // Damage.at, solver/Damage.java:342-347
+        if (!d.dStrength) {
+            std::memset(out, 0, sizeof(double) * P);
+            return NF_OK;
+        }
+        std::vector<double> st(P), so(P), fs(P), ep(P);
+        TRY(copy(d.dStrength, st.data()));
+        TRY(copy(d.dSoftening, so.data()));
+        TRY(copy(d.dFailure, fs.data()));
+        TRY(copy(d.epsP, ep.data()));
+        TRY(s->sync());
+        for (int p = 0; p < P; p++) {
+            double v = 0.0;
+            if (st[p] > 0.0) {
+                const double dd = -so[p] * (ep[p] - fs[p]) / st[p];
+                v = dd <= 0.0 ? 0.0 : (dd >= 1.0 ? 1.0 : dd);
+            }
+            out[p] = v;
+        }
+        return NF_OK;
+    }
     const double* src = nullptr;
     switch (field) {
         case NF_POINT_PLASTIC_STRAIN: src = d.epsP; break;
@@ -2222,6 +2294,8 @@ int nf_get_point_field(nf_solver* s, int32_t field, double* out) {
         case NF_POINT_DAMAGE_STRENGTH: src = d.dStrength; break;
         case NF_POINT_DAMAGE_SOFTENING: src = d.dSoftening; break;
         case NF_POINT_DAMAGE_WIDTH: src = d.dWidth; break;
+        case NF_POINT_DAMAGE_FAILURE_STRAIN: src = d.dFailure; break;
+        case NF_POINT_DAMAGE_FRACTURE_ENERGY: src = d.dGf; break;
         default: return fail(NF_E_ARGUMENT, "unknown point field");
     }
     if (!src) {
@@ -2372,6 +2446,85 @@ int nf_contact_surface(nf_solver* s, int32_t capacity, int32_t* out, int32_t* co
     }
     *count = k;
     return NF_OK;
+}
+
+// ExplicitSolver's single-value queries (solver/ExplicitSolver.java:826, :1778-1794,
+// :1832-1840, :1867-1877, :1928-1930), from the device state copied out.
+int nf_temperature(nf_solver* s, int32_t point, double* out) {
+    g_error.clear();
+    if (point < 0 || point >= s->P) return fail(NF_E_ARGUMENT, "point out of range");
+    const HostMaterial& m = s->materials[s->materialOf.empty() ? 0 : s->materialOf[point / s->ppe]];
+    if (!m.raw.has_johnson_cook) return fail(NF_E_STATE, "this material has no thermal model; temperature is not defined");
+    double w = 0.0;
+    HIP_TRY(hipMemcpyAsync(&w, s->d.work + point, sizeof w, hipMemcpyDeviceToHost, s->stream));
+    TRY(s->sync());
+    const nf_johnson_cook& j = m.raw.johnson_cook;
+    *out = j.room_temperature + j.taylor_quinney * w / (m.raw.density * j.specific_heat);
+    return NF_OK;
+}
+
+int nf_plastic_front_radius(nf_solver* s, double* out) {
+    g_error.clear();
+    std::vector<double> ep(s->P);
+    TRY(nf_get_point_field(s, NF_POINT_PLASTIC_STRAIN, ep.data()));
+    double front = 0.0;
+    for (int e = 0; e < s->E; e++) {
+        bool yielded = false;
+        for (int g = 0; g < s->ppe; g++)
+            if (ep[e * s->ppe + g] > 0.0) yielded = true;
+        if (!yielded) continue;
+        const int* c = &s->conn[4 * e];  // outerRadius :1932-1936, reference coordinates
+        const double outer = std::max(std::max(s->r[c[0]], s->r[c[1]]), std::max(s->r[c[2]], s->r[c[3]]));
+        front = std::max(front, outer);
+    }
+    *out = front;
+    return NF_OK;
+}
+
+int nf_most_damaged_element(nf_solver* s, int32_t* out) {
+    g_error.clear();
+    *out = -1;
+    if (!s->hasDamage) return NF_OK;
+    std::vector<double> dmg(s->P);
+    TRY(nf_get_point_field(s, NF_POINT_DAMAGE, dmg.data()));
+    double max = 0.0;
+    for (int e = 0; e < s->E; e++)
+        for (int g = 0; g < s->ppe; g++) {
+            const double v = dmg[e * s->ppe + g];
+            if (v > max) {
+                max = v;
+                *out = e;
+            }
+        }
+    return NF_OK;
+}
+
+int nf_bore_pressure(nf_solver* s, double* out) {  // pressureAt(time) :1203-1207
+    g_error.clear();
+    Step h;
+    TRY(s->readStep(h));
+    const Dev& d = s->d;
+    if (d.rampDuration <= 0.0) {
+        *out = d.targetPressure;
+        return NF_OK;
+    }
+    const double t = h.time / d.rampDuration;
+    const double sm = t != t ? t : (t < 1.0 ? t : 1.0);  // Math.min(1.0, t)
+    *out = d.targetPressure * sm * sm * (3.0 - 2.0 * sm);
+    return NF_OK;
+}
+
+int nf_mesh_z(nf_solver* s, int32_t node, double* out) {
+    g_error.clear();
+    if (node < 0 || node >= s->N) return fail(NF_E_ARGUMENT, "node out of range");
+    *out = s->z[node];
+    return NF_OK;
+}
+
+int nf_set_damage_uniform(nf_solver* s, double fracture_energy, const double* failure_strain) {  // :713-717
+    g_error.clear();
+    const std::vector<double> gf(s->E, fracture_energy);
+    return nf_set_damage(s, gf.data(), failure_strain);
 }
 
 // Audits from the device state, in the reference's summation order

@@ -147,10 +147,10 @@ def run_batched(instances, end_time=taylor.ROD_DURATION, max_steps=0, autotune=T
     taken = 0
     if autotune:
         choice, _, taken = tune(batch, instances, end_time)
-    if max_steps > 0:
-        max_steps = max(max_steps - taken, 0)
-    if max_steps > 0 or taken == 0:
-        batch.runUntil(end_time, max_steps)
+    if max_steps <= 0:
+        batch.runUntil(end_time, 0)  # no cap: every instance to end_time
+    elif max_steps > taken:
+        batch.runUntil(end_time, max_steps - taken)
     seconds = time.perf_counter() - t0
     batch.scatter()
     batch.close()

@@ -110,6 +110,9 @@ class Material:
     def dilatationalWaveSpeed(self):
         return math.sqrt((self.lambda_() + 2.0 * self.mu()) / self.density)
 
+    def barWaveSpeed(self):  # :127-129
+        return math.sqrt(self.youngsModulus / self.density)
+
     def to_c(self):
         out = _abi.MaterialC()
         out.youngs_modulus = self.youngsModulus
@@ -498,6 +501,12 @@ class ExplicitSolver:
             contact._solver = self
         self._contact = contact
 
+    def setContactLimits(self, captureCapacity=-1, maxTerms=-1):
+        """nf_contact_set_limits (HIP library only; no counterpart in the reference): caps
+        the capture grid's entries (0 forces the reference's nine-cell walk) and the pair
+        terms a node may take in one step; -1 restores a default."""
+        self._call("contact_set_limits", int(captureCapacity), int(maxTerms))
+
     def contact(self):
         """The contact model, or None if none was set (:492-494)."""
         return self._contact
@@ -506,14 +515,32 @@ class ExplicitSolver:
         self._call("set_gravity", float(ar), float(az))
 
     def setDamage(self, fractureEnergy, failureStrain):
+        """setDamage(double, double[]) :713-717 for a scalar fracture energy, setDamage(double[],
+        double[]) :729-791 for one per element."""
         E = self.mesh.elementCount
-        gf = np.ascontiguousarray(np.broadcast_to(np.asarray(fractureEnergy, np.float64), (E,)))
         eps = np.ascontiguousarray(failureStrain, dtype=np.float64)
-        if eps.size != E:
-            raise IllegalArgumentException(
-                f"failure strain has {eps.size} entries for a mesh of {E} elements")
-        self._call("set_damage", _dptr(gf), _dptr(eps))
+        if np.ndim(fractureEnergy) == 0:
+            if eps.size != E:
+                raise IllegalArgumentException(
+                    f"failure strain has {eps.size} entries for a mesh of {E} elements")
+            self._call("set_damage_uniform", float(fractureEnergy), _dptr(eps))
+        else:
+            gf = np.ascontiguousarray(fractureEnergy, dtype=np.float64)
+            if gf.size != E:
+                raise IllegalArgumentException(
+                    f"fracture energy has {gf.size} entries for a mesh of {E} elements")
+            if eps.size != E:
+                raise IllegalArgumentException(
+                    f"failure strain has {eps.size} entries for a mesh of {E} elements")
+            self._call("set_damage", _dptr(gf), _dptr(eps))
         self._failureStrain = eps.copy()
+
+    def damage(self):
+        """The damage state (:794-796), or None if none was set: a read-only view of
+        solver/Damage.java over the solver's current arrays."""
+        if getattr(self, "_failureStrain", None) is None:
+            return None
+        return Damage(self)
 
     def setErosion(self, threshold):
         self._call("set_erosion", float(threshold))
@@ -666,6 +693,52 @@ class ExplicitSolver:
     def maxDamage(self):
         return self.audits().max_damage
 
+    def damagedFraction(self):  # :1748-1751
+        return self.audits().damaged_fraction
+
+    def failedFraction(self):  # :1753-1756
+        return self.audits().failed_fraction
+
+    def temperature(self, point):
+        """Adiabatic temperature at a Gauss point (:1832-1840); IllegalStateException when
+        its material has no Johnson-Cook law."""
+        out = C.c_double()
+        self._call("temperature", int(point), C.byref(out))
+        return out.value
+
+    def plasticFrontRadius(self):  # :1867-1877
+        out = C.c_double()
+        self._call("plastic_front_radius", C.byref(out))
+        return out.value
+
+    def mostDamagedElement(self):  # :1778-1794
+        out = C.c_int32()
+        self._call("most_damaged_element", C.byref(out))
+        return out.value
+
+    def borePressure(self):  # :826-828
+        out = C.c_double()
+        self._call("bore_pressure", C.byref(out))
+        return out.value
+
+    def meshZ(self, node):  # :1928-1930
+        out = C.c_double()
+        self._call("mesh_z", int(node), C.byref(out))
+        return out.value
+
+    def centroidRadius(self, e):  # :1904-1908, reference coordinates
+        c = self.mesh.conn[4 * e:4 * e + 4]
+        r = self.mesh.r
+        return 0.25 * (r[c[0]] + r[c[1]] + r[c[2]] + r[c[3]])
+
+    def centroidZ(self, e):  # :1917-1925, current configuration
+        c = self.mesh.conn[4 * e:4 * e + 4]
+        uz = self.axialDisplacement()
+        total = 0.0
+        for n in c:
+            total += self.mesh.z[n] + uz[n]
+        return 0.25 * total
+
     # RigidWall audits live in the solver's state on this side of the boundary.
     def wallImpulse(self):
         return self.state().wall_impulse
@@ -763,6 +836,80 @@ class ExplicitSolver:
         out = np.empty(max(1, 3 * n.value), np.int32)
         self._call("contact_pairs", n.value, _iptr(out), C.byref(n))
         return out[:3 * n.value].reshape(-1, 3)
+
+
+class Damage:
+    """solver/Damage.java's queries over a solver's damage arrays (fracture energy, failure
+    strain, and the onset strength, softening modulus and band width written by the return
+    map), read from the solver when the view is made."""
+
+    def __init__(self, solver):
+        self._fe = solver.pointField("damage_fracture_energy")
+        self._fs = solver.pointField("damage_failure_strain")
+        self._st = solver.pointField("damage_strength")
+        self._so = solver.pointField("damage_softening")
+        self._w = solver.pointField("damage_width")
+        youngs = np.array([m.youngsModulus for m in solver.materials.distinct])
+        ppe = solver.state().points_per_element
+        idx = (np.zeros(solver.mesh.elementCount, np.int64) if solver.materials.index is None
+               else np.asarray(solver.materials.index))
+        self._E = np.repeat(youngs[idx], ppe)
+
+    def fractureEnergy(self, p):  # :266-268
+        return float(self._fe[p])
+
+    def points(self):  # :270-272
+        return int(self._fs.size)
+
+    def hasBegun(self, p):  # :275-277
+        return bool(self._st[p] > 0.0)
+
+    def bandWidth(self, p):  # :280-282
+        return float(self._w[p])
+
+    def strengthAtOnset(self, p):  # :285-287
+        return float(self._st[p])
+
+    def softeningModulus(self, p):  # :290-292
+        return float(self._so[p])
+
+    def at(self, p, plasticStrain):  # :342-347
+        sf = self._st[p]
+        if sf <= 0.0:
+            return 0.0
+        d = -self._so[p] * (plasticStrain - self._fs[p]) / sf
+        return 0.0 if d <= 0.0 else (1.0 if d >= 1.0 else float(d))
+
+    def dissipation(self, p, plasticStrain):  # :363-372
+        sf = self._st[p]
+        if sf <= 0.0:
+            return 0.0
+        slope = self._so[p]
+        full = -sf / slope
+        excess = plasticStrain - self._fs[p]
+        if excess <= 0.0:
+            return 0.0
+        if excess > full:
+            excess = full
+        return float(excess * (sf + 0.5 * slope * excess))
+
+    def maximum(self, plasticStrain):  # :375-382
+        m = 0.0
+        for p in range(self.points()):
+            d = self.at(p, plasticStrain[p])
+            if d > m:
+                m = d
+        return m
+
+    def begunFraction(self):  # :385-389
+        return int(np.count_nonzero(self._st > 0.0)) / self.points()
+
+    def failedFraction(self, plasticStrain):  # :392-398
+        n = sum(1 for p in range(self.points()) if self.at(p, plasticStrain[p]) >= 1.0)
+        return n / self.points()
+
+    def clampedPoints(self):  # :406-412
+        return int(np.count_nonzero(self._so == -self._E))
 
 
 class Batch:

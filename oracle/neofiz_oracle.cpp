@@ -1786,17 +1786,30 @@ int nfo_get_point_field(nfo_solver* s, int32_t field, double* out) {
         case NF_POINT_PLASTIC_WORK: std::memcpy(out, s->wPlastic.data(), sizeof(double) * P); return NF_OK;
         case NF_POINT_DAMAGE_STRENGTH:
         case NF_POINT_DAMAGE_SOFTENING:
-        case NF_POINT_DAMAGE_WIDTH: {
+        case NF_POINT_DAMAGE_WIDTH:
+        case NF_POINT_DAMAGE_FAILURE_STRAIN:
+        case NF_POINT_DAMAGE_FRACTURE_ENERGY: {
             if (!s->damage) {
                 std::memset(out, 0, sizeof(double) * P);
                 return NF_OK;
             }
             const std::vector<double>& v = field == NF_POINT_DAMAGE_STRENGTH ? s->damage->strength
                                            : field == NF_POINT_DAMAGE_SOFTENING ? s->damage->softening
-                                                                                : s->damage->width;
+                                           : field == NF_POINT_DAMAGE_WIDTH ? s->damage->width
+                                           : field == NF_POINT_DAMAGE_FAILURE_STRAIN ? s->damage->failureStrain
+                                                                                     : s->damage->fractureEnergy;
             std::memcpy(out, v.data(), sizeof(double) * P);
             return NF_OK;
         }
+        case NF_POINT_DAMAGE:  // Damage.at(p, epsP[p]) :342-347
+            for (int p = 0; p < P; p++) out[p] = s->damage ? s->damage->at(p, s->epsP[p]) : 0.0;
+            return NF_OK;
+        case NF_POINT_TEMPERATURE:  // temperature(p) :1832-1840, NaN where it throws
+            for (int p = 0; p < P; p++) {
+                const Material& m = s->materialAt(p / s->pointsPerElement);
+                out[p] = m.has_jc ? m.jc.temperature(s->wPlastic[p], m.rho) : NAN;
+            }
+            return NF_OK;
         default: return fail(NF_E_ARGUMENT, "unknown point field");
     }
 }
@@ -1872,6 +1885,60 @@ int nfo_film_capture(nfo_solver* s, int32_t count, const int32_t* fields, double
         }
     }
     return NF_OK;
+}
+
+// ExplicitSolver's single-value queries.
+int nfo_temperature(nfo_solver* s, int32_t point, double* out) {  // :1832-1840
+    if (point < 0 || point >= (int)s->wPlastic.size()) return fail(NF_E_ARGUMENT, "point out of range");
+    const Material& m = s->materialAt(point / s->pointsPerElement);
+    if (!m.has_jc) return fail(NF_E_STATE, "this material has no thermal model; temperature is not defined");
+    *out = m.jc.temperature(s->wPlastic[point], m.rho);
+    return NF_OK;
+}
+int nfo_plastic_front_radius(nfo_solver* s, double* out) {  // :1867-1877
+    const int ppe = s->pointsPerElement;
+    double front = 0.0;
+    for (int e = 0; e < s->mesh.elementCount; e++) {
+        bool yielded = false;
+        for (int g = 0; g < ppe; g++)
+            if (s->epsP[e * ppe + g] > 0.0) yielded = true;
+        if (yielded) {
+            const int* c = &s->mesh.conn[4 * e];  // outerRadius :1932-1936
+            const double outer = jmax(jmax(s->mesh.r[c[0]], s->mesh.r[c[1]]), jmax(s->mesh.r[c[2]], s->mesh.r[c[3]]));
+            front = jmax(front, outer);
+        }
+    }
+    *out = front;
+    return NF_OK;
+}
+int nfo_most_damaged_element(nfo_solver* s, int32_t* out) {  // :1778-1794
+    *out = -1;
+    if (!s->damage) return NF_OK;
+    const int ppe = s->pointsPerElement;
+    double max = 0.0;
+    for (int e = 0; e < s->mesh.elementCount; e++)
+        for (int g = 0; g < ppe; g++) {
+            const int p = e * ppe + g;
+            const double d = s->damage->at(p, s->epsP[p]);
+            if (d > max) {
+                max = d;
+                *out = e;
+            }
+        }
+    return NF_OK;
+}
+int nfo_bore_pressure(nfo_solver* s, double* out) {  // :826-828
+    *out = s->pressureAt(s->time);
+    return NF_OK;
+}
+int nfo_mesh_z(nfo_solver* s, int32_t node, double* out) {  // :1928-1930
+    if (node < 0 || node >= s->mesh.nodeCount) return fail(NF_E_ARGUMENT, "node out of range");
+    *out = s->mesh.z[node];
+    return NF_OK;
+}
+int nfo_set_damage_uniform(nfo_solver* s, double fracture_energy, const double* failure_strain) {  // :713-717
+    const std::vector<double> gf(s->mesh.elementCount, fracture_energy);
+    return nfo_set_damage(s, gf.data(), failure_strain);
 }
 
 int nfo_get_element_field(nfo_solver* s, int32_t field, double* out) {

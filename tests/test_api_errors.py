@@ -91,3 +91,23 @@ def test_mass_scaling_scales_the_step(oracle):
     assert abs(s.nodeField("mass").sum() / m0 - 16.0) < 1e-12
     assert s.state().dt0 == 4.0 * dt
     s.shutdown()
+
+
+@pytest.mark.gpu
+def test_sensitivity_refuses_contact_damage_and_erosion_set_later():
+    """A solver carrying sensitivities has derivative kernels for the Taylor step only, so
+    contact, damage and erosion are refused after enableSensitivity as well as before it."""
+    from neofiz_amd import taylor
+    mesh, s, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER,
+                              taylor.ROD_SPEED, 2, 8, friction=taylor.ROD_JC_FRICTION)
+    s.enableSensitivity()
+    with pytest.raises(IllegalStateException):
+        s.setContact(Contact(mesh).withFriction(0.2))
+    with pytest.raises(IllegalStateException):
+        s.setDamage(1e4, np.full(mesh.elementCount, 0.5))
+    with pytest.raises(IllegalStateException):
+        s.setErosion(1.0)
+    s.setContact(None)  # switching contact off is harmless
+    s.run(5)
+    assert s.steps() == 5
+    s.shutdown()

@@ -239,3 +239,51 @@ def test_stacked_wall_perforation_bitwise():
     g, r, seen = lockstep(build, 30, 50, damage=True, label="stacked wall")
     assert seen > 0
     assert r.eroded_elements > 100
+
+
+# ------------------------------------------------------------------ rare contact paths
+
+def with_limits(build, capture_capacity):
+    """The scene with the HIP side's capture grid capped (the oracle has none)."""
+    def make(cls):
+        s = build(cls)
+        if cls is ExplicitSolver:
+            s.setContactLimits(capture_capacity, -1)
+        return s
+    return make
+
+
+@pytest.mark.parametrize("scene", ["collision", "slide", "perforation"])
+def test_nine_cell_walk_fallback_bitwise(scene):
+    """With the capture grid given no room, every assembly overflows it and the search walks
+    the reference's nine coarse cells instead (the path a real overflow takes): pairs, forces
+    and state stay bit-identical to the oracle."""
+    if scene == "collision":
+        mesh = two_blocks()
+        build, chunks, steps = (lambda cls: colliding(cls, mesh, Contact(mesh))), 200, 10
+    elif scene == "slide":
+        mesh = Outline.mesh(MM, Outline.rectangle(0.0, 0.0, 20.0 * MM, 5.0 * MM),
+                            Outline.rectangle(5.0 * MM, 6.0 * MM, 13.0 * MM, 11.0 * MM))
+        build, chunks, steps = (lambda cls: sliding(cls, mesh, 0.4)), 150, 10
+    else:
+        build, chunks, steps = mini_wall, 60, 20
+    _, r, seen = lockstep(with_limits(build, 0), chunks, steps, damage=scene == "perforation",
+                          label=f"nine-cell walk, {scene}")
+    assert seen > 0
+
+
+def test_term_cap_is_a_state_error_and_clears():
+    """A node given more pair terms in one step than the cap makes the run call fail with
+    IllegalStateException after its steps; the next call, with the cap restored, runs."""
+    from neofiz_amd.solver import IllegalStateException
+    s = mini_wall(ExplicitSolver)
+    s.run(50)  # the slug is in the wall: 35 pairs, a node with four terms (the oracle's count)
+    assert s.state().contact_pairs > 0
+    s.setContactLimits(-1, 1)
+    with pytest.raises(IllegalStateException, match="pair terms"):
+        s.run(40)
+    s.setContactLimits(-1, -1)
+    s.run(20)
+    with pytest.raises(Exception):
+        s.setContactLimits(-1, 33)
+    s.shutdown()
