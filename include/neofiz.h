@@ -345,6 +345,19 @@ int nf_defect_sample_onto_device(nf_defect* f, int32_t seed_count, const int64_t
                                  int32_t element_count, const int32_t* conn,
                                  int32_t volume_rule, double depth, int32_t flags, double* out);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * inside a captured stream, tight serving loops, harnesses that forbid hipMalloc). The same
+ * as nf_defect_sample_onto_device, bit for bit, but the call allocates nothing: its bounding
+ * box, status word, seed hashes and white-noise grid live in the library's static 16 MiB
+ * device array (one per GPU, held until the process ends), not in the handle's grown
+ * buffers. seed_count is at most 16384 (else NF_E_ARGUMENT); a box whose grid exceeds the
+ * array even for one seed hashes every tap instead (the same values). Calls that overlap on
+ * one GPU share that array and race: the caller serialises them. */
+int nf_defect_sample_onto_static(nf_defect* f, int32_t seed_count, const int64_t* seeds,
+                                 int32_t node_count, const double* r, const double* z,
+                                 int32_t element_count, const int32_t* conn,
+                                 int32_t volume_rule, double depth, int32_t flags, double* out);
+
 /* Point evaluations at n points (host arrays):
  *   GAUSSIAN          gaussianAt(r, z) :172-174                      (w unused)
  *   STRAIN_AREA       failureStrainAt(r, z, crossSectionArea = w) :217-219

@@ -123,6 +123,9 @@ _PROTOTYPES = {
     "defect_sample_onto_device": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.c_int32, _P,
                                             _P, C.c_int32, _P, C.c_int32, C.c_double, C.c_int32,
                                             _P]),
+    "defect_sample_onto_static": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.c_int32, _P,
+                                            _P, C.c_int32, _P, C.c_int32, C.c_double, C.c_int32,
+                                            _P]),
     "defect_points": (C.c_int, [_P, C.c_int32, C.c_int32, c_double_p, c_double_p, c_double_p,
                                 c_double_p]),
     "defect_timing": (C.c_int, [_P, c_double_p]),

@@ -417,11 +417,31 @@ inline const char* message(int code) {
     }
 }
 
+// Clears one int (a fill kernel rather than a memset node, so that the call records no
+// memset if a caller captures its stream).
+__global__ void k_df_zero_int(int* __restrict__ p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
 // Largest grid, in doubles, that one chunk of seeds may hold (1 GiB); a box too large for
 // even one seed hashes every tap instead.
 constexpr int64_t GRID_BUDGET = int64_t(1) << 27;
 
 }  // namespace dfd
+
+// The device scratch one sampling call works in: the bounding box and its per-workgroup
+// partials, the status word, the seeds' hashes and the white-noise grid (grid_cap doubles,
+// h0_cap hashes). A handle's own call grows its buffers to fit; the no-allocation call hands
+// in fixed slices of the library's static scratch array instead.
+struct DfScratch {
+    double* box;
+    double* boxPartial;
+    int* status;
+    uint64_t* h0;
+    size_t h0Cap;
+    double* grid;
+    size_t gridCap;
+};
 
 struct nf_defect {
     dfd::Params p{};
@@ -465,8 +485,8 @@ struct nf_defect {
 
     // The grid box of cacheFor :351-366 from the device bounding box; false when the grid is
     // not used (flag, empty, or beyond the budget for a single seed).
-    int box_for(const double* r_d, const double* z_d, int n, bool uncached, dfd::Grid& g,
-                bool& cached) {
+    int box_for(const DfScratch& w, const double* r_d, const double* z_d, int n, bool uncached,
+                dfd::Grid& g, bool& cached) {
         cached = false;
         g = dfd::Grid{nullptr, 0, 0, 0, 0};
         HIP_TRY(hipEventRecord(ev[0], stream));
@@ -477,13 +497,13 @@ struct nf_defect {
         const int blocks = std::max(1, std::min(dfd::BOX_BLOCKS, (n + 4 * dfd::BOUNDS_THREADS - 1) /
                                                                      (4 * dfd::BOUNDS_THREADS)));
         hipLaunchKernelGGL(dfd::k_df_bounds, dim3(blocks), dim3(dfd::BOUNDS_THREADS), 0, stream, r_d, z_d, n,
-                           boxPartial);
+                           w.boxPartial);
         hipLaunchKernelGGL(dfd::k_df_bounds_final, dim3(1), dim3(dfd::BOUNDS_THREADS), 0, stream,
-                           (const double*)boxPartial, blocks, box);
+                           (const double*)w.boxPartial, blocks, w.box);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ev[1], stream));
         double hb[4];
-        HIP_TRY(hipMemcpyAsync(hb, box, sizeof hb, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(hb, w.box, sizeof hb, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         const int32_t i0 = dfd::jint(std::floor(hb[0] / p.cell)) - dfd::RADIUS;
         const int32_t j0 = dfd::jint(std::floor(hb[2] / p.cell)) - dfd::RADIUS;
@@ -496,47 +516,71 @@ struct nf_defect {
         return NF_OK;
     }
 
-    int seeds_to_device(int count, const int64_t* seeds) {
-        std::vector<uint64_t> hh((size_t)count);
-        for (int s = 0; s < count; s++) hh[(size_t)s] = dfd::seed_hash(seeds ? seeds[s] : seed);
-        TRY(grow(&h0, h0Cap, (size_t)count));
-        HIP_TRY(hipMemcpyAsync(h0, hh.data(), sizeof(uint64_t) * count, hipMemcpyHostToDevice, stream));
+    // The seeds' hashes into w.h0 (which holds at least `count`), through a fixed host buffer.
+    int seeds_to_device(const DfScratch& w, int count, const int64_t* seeds) {
+        uint64_t hh[256];
+        for (int s0 = 0; s0 < count; s0 += 256) {
+            const int n = std::min(256, count - s0);
+            for (int s = 0; s < n; s++) hh[s] = dfd::seed_hash(seeds ? seeds[s0 + s] : seed);
+            HIP_TRY(hipMemcpyAsync(w.h0 + s0, hh, sizeof(uint64_t) * n, hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));  // hh is reused
+        }
         return NF_OK;
     }
 
     template <int SITES, int OUT>
-    int launch_sample(bool cached, int sites, int seeds, int s0, const uint64_t* h0c, const int4* c,
-                      const double* r_d, const double* z_d, const double* w_d, const dfd::Grid& g,
-                      double* out_d) {
+    int launch_sample(const DfScratch& ws, bool cached, int sites, int seeds, int s0, const uint64_t* h0c,
+                      const int4* c, const double* r_d, const double* z_d, const double* w_d,
+                      const dfd::Grid& g, double* out_d) {
         const int64_t total = (int64_t)sites * seeds;
         const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
         if (blocks == 0) return NF_OK;
         if (cached)
             hipLaunchKernelGGL((dfd::k_df_sample<SITES, OUT, true>), dim3(blocks), dim3(256), 0, stream, p,
-                               sites, seeds, s0, h0c, c, r_d, z_d, w_d, grid, g.i0, g.j0, g.ni, g.nj,
-                               out_d, status);
+                               sites, seeds, s0, h0c, c, r_d, z_d, w_d, ws.grid, g.i0, g.j0, g.ni, g.nj,
+                               out_d, ws.status);
         else
             hipLaunchKernelGGL((dfd::k_df_sample<SITES, OUT, false>), dim3(blocks), dim3(256), 0, stream,
                                p, sites, seeds, s0, h0c, c, r_d, z_d, w_d, (const double*)nullptr, 0, 0,
-                               0, 0, out_d, status);
+                               0, 0, out_d, ws.status);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
 
     // The shared body of sampleOnto and the smoothed point functions, on device arrays.
+    // The handle's own scratch, grown to hold `seedCount` hashes (the grid grows in run).
+    int own_scratch(int seedCount, DfScratch& w) {
+        TRY(grow(&h0, h0Cap, (size_t)std::max(1, seedCount)));
+        w = DfScratch{box, boxPartial, status, h0, h0Cap, grid, gridCap};
+        return NF_OK;
+    }
+
+    // fixed: the scratch is fixed (the no-allocation call); the grid then holds as many
+    // seeds as fit in w.gridCap, and a box too large for one seed hashes every tap, which
+    // gives the same values. Otherwise the handle's own grid grows to the chunk.
     template <int SITES, int OUT>
-    int run(int seedCount, const int64_t* seeds, int nodes, const double* r_d, const double* z_d,
-            const double* w_d, int sites, const int4* c, bool uncached, double* out_d) {
-        TRY(seeds_to_device(seedCount, seeds));
-        HIP_TRY(hipMemsetAsync(status, 0, sizeof(int), stream));
+    int run(DfScratch w, bool fixed, int seedCount, const int64_t* seeds, int nodes, const double* r_d,
+            const double* z_d, const double* w_d, int sites, const int4* c, bool uncached, double* out_d) {
+        TRY(seeds_to_device(w, seedCount, seeds));
+        hipLaunchKernelGGL(dfd::k_df_zero_int, dim3(1), dim3(64), 0, stream, w.status);
+        HIP_TRY(hipGetLastError());
         dfd::Grid g;
         bool cached;
-        TRY(box_for(r_d, z_d, nodes, uncached, g, cached));
+        TRY(box_for(w, r_d, z_d, nodes, uncached, g, cached));
         int chunk = seedCount;
         if (cached) {
             const int64_t per = (int64_t)g.ni * g.nj;
-            chunk = (int)std::max<int64_t>(1, std::min<int64_t>(seedCount, dfd::GRID_BUDGET / per));
-            TRY(grow(&grid, gridCap, (size_t)(per * chunk)));
+            const int64_t room = fixed ? (int64_t)w.gridCap : dfd::GRID_BUDGET;
+            if (per > room) {
+                cached = false;  // only reachable with fixed scratch: GRID_BUDGET bounds per already
+            } else {
+                chunk = (int)std::max<int64_t>(1, std::min<int64_t>(seedCount, room / per));
+                if (!fixed) {
+                    TRY(grow(&grid, gridCap, (size_t)(per * chunk)));
+                    w.grid = grid;
+                    w.gridCap = gridCap;
+                }
+            }
         }
         float noiseMs = 0.0f, sampleMs = 0.0f;
         for (int s0 = 0; s0 < seedCount; s0 += chunk) {
@@ -545,12 +589,12 @@ struct nf_defect {
             if (cached) {
                 const int64_t total = (int64_t)g.ni * g.nj * n;
                 const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
-                hipLaunchKernelGGL(dfd::k_df_noise, dim3(blocks), dim3(256), 0, stream, (const uint64_t*)(h0 + s0),
-                                   n, g.i0, g.j0, g.ni, g.nj, grid);
+                hipLaunchKernelGGL(dfd::k_df_noise, dim3(blocks), dim3(256), 0, stream, (const uint64_t*)(w.h0 + s0),
+                                   n, g.i0, g.j0, g.ni, g.nj, w.grid);
                 HIP_TRY(hipGetLastError());
             }
             HIP_TRY(hipEventRecord(ev[3], stream));
-            TRY((launch_sample<SITES, OUT>(cached, sites, n, s0, h0 + s0, c, r_d, z_d, w_d, g, out_d)));
+            TRY((launch_sample<SITES, OUT>(w, cached, sites, n, s0, w.h0 + s0, c, r_d, z_d, w_d, g, out_d)));
             HIP_TRY(hipEventRecord(ev[4], stream));
             HIP_TRY(hipEventSynchronize(ev[4]));
             float a = 0.0f, b = 0.0f;
@@ -562,7 +606,7 @@ struct nf_defect {
         float boxMs = 0.0f;
         HIP_TRY(hipEventElapsedTime(&boxMs, ev[0], ev[1]));
         int code = 0;
-        HIP_TRY(hipMemcpyAsync(&code, status, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(&code, w.status, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         lastMs[0] = boxMs;
         lastMs[1] = noiseMs;
@@ -589,28 +633,39 @@ __global__ void k_df_check_conn(const int4* __restrict__ conn, int elements, int
 
 namespace {
 
-int defect_sample(nf_defect* f, int32_t seed_count, const int64_t* seeds, int32_t node_count,
-                  const double* r_d, const double* z_d, int32_t element_count, const int4* conn_d,
-                  int32_t rule, double depth, int32_t flags, double* out_d) {
+// sampleOnto on device arrays. fixed: null for the handle's own (grown) scratch, else the
+// no-allocation call's fixed scratch.
+int defect_sample(nf_defect* f, const DfScratch* fixed, int32_t seed_count, const int64_t* seeds,
+                  int32_t node_count, const double* r_d, const double* z_d, int32_t element_count,
+                  const int4* conn_d, int32_t rule, double depth, int32_t flags, double* out_d) {
     if (rule != NF_DEFECT_RING && rule != NF_DEFECT_SLAB) return fail(NF_E_ARGUMENT, "unknown volume rule");
     if (rule == NF_DEFECT_SLAB && !(depth > 0.0)) return fail(NF_E_ARGUMENT, "depth must be positive");
     if (element_count == 0 || seed_count == 0) return NF_OK;
+    DfScratch w;
+    if (fixed) {
+        if ((size_t)seed_count > fixed->h0Cap)
+            return fail(NF_E_ARGUMENT, "seed count " + std::to_string(seed_count) + " exceeds the " +
+                                           std::to_string(fixed->h0Cap) + " this call's fixed scratch holds");
+        w = *fixed;
+    } else {
+        TRY(f->own_scratch(seed_count, w));
+    }
     // connectivity in range, before any kernel dereferences it
-    HIP_TRY(hipMemsetAsync(f->status, 0, sizeof(int), f->stream));
+    hipLaunchKernelGGL(dfd::k_df_zero_int, dim3(1), dim3(64), 0, f->stream, w.status);
     hipLaunchKernelGGL(dfd::k_df_check_conn, dim3(blocks_for(element_count)), dim3(BLOCK), 0, f->stream,
-                       conn_d, element_count, node_count, f->status);
+                       conn_d, element_count, node_count, w.status);
     HIP_TRY(hipGetLastError());
     int bad = 0;
-    HIP_TRY(hipMemcpyAsync(&bad, f->status, sizeof(int), hipMemcpyDeviceToHost, f->stream));
+    HIP_TRY(hipMemcpyAsync(&bad, w.status, sizeof(int), hipMemcpyDeviceToHost, f->stream));
     HIP_TRY(hipStreamSynchronize(f->stream));
     if (bad) return fail(NF_E_ARGUMENT, "connectivity names a node outside the mesh");
     f->p.depth = rule == NF_DEFECT_SLAB ? depth : 0.0;
     const bool uncached = (flags & NF_DEFECT_UNCACHED) != 0;
     if (rule == NF_DEFECT_RING)
-        return f->run<dfd::SITES_MESH, dfd::OUT_RING>(seed_count, seeds, node_count, r_d, z_d, nullptr,
-                                                      element_count, conn_d, uncached, out_d);
-    return f->run<dfd::SITES_MESH, dfd::OUT_SLAB>(seed_count, seeds, node_count, r_d, z_d, nullptr,
-                                                  element_count, conn_d, uncached, out_d);
+        return f->run<dfd::SITES_MESH, dfd::OUT_RING>(w, fixed != nullptr, seed_count, seeds, node_count, r_d, z_d,
+                                                      nullptr, element_count, conn_d, uncached, out_d);
+    return f->run<dfd::SITES_MESH, dfd::OUT_SLAB>(w, fixed != nullptr, seed_count, seeds, node_count, r_d, z_d,
+                                                  nullptr, element_count, conn_d, uncached, out_d);
 }
 
 }  // namespace
@@ -689,8 +744,8 @@ int nf_defect_sample_onto(nf_defect* f, int32_t seed_count, const int64_t* seeds
     HIP_TRY(hipMemcpyAsync(f->r, r, sizeof(double) * node_count, hipMemcpyHostToDevice, f->stream));
     HIP_TRY(hipMemcpyAsync(f->z, z, sizeof(double) * node_count, hipMemcpyHostToDevice, f->stream));
     HIP_TRY(hipMemcpyAsync(f->conn, conn, sizeof(int4) * element_count, hipMemcpyHostToDevice, f->stream));
-    TRY(defect_sample(f, seed_count, seeds, node_count, f->r, f->z, element_count, f->conn, volume_rule,
-                      depth, flags, f->out));
+    TRY(defect_sample(f, nullptr, seed_count, seeds, node_count, f->r, f->z, element_count, f->conn,
+                      volume_rule, depth, flags, f->out));
     HIP_TRY(hipMemcpyAsync(out, f->out, sizeof(double) * total, hipMemcpyDeviceToHost, f->stream));
     HIP_TRY(hipStreamSynchronize(f->stream));
     return NF_OK;
@@ -706,7 +761,89 @@ int nf_defect_sample_onto_device(nf_defect* f, int32_t seed_count, const int64_t
     if ((element_count && (!conn || !r || !z)) || (seed_count && element_count && !out))
         return fail(NF_E_ARGUMENT, "null array");
     if ((uintptr_t)conn % alignof(int4)) return fail(NF_E_ARGUMENT, "connectivity must be 16-byte aligned");
-    TRY(defect_sample(f, seed_count, seeds, node_count, r, z, element_count, (const int4*)conn, volume_rule,
+    TRY(defect_sample(f, nullptr, seed_count, seeds, node_count, r, z, element_count, (const int4*)conn,
+                      volume_rule, depth, flags, out));
+    HIP_TRY(hipStreamSynchronize(f->stream));
+    return NF_OK;
+}
+
+}  // extern "C"
+
+// ---- sampleOnto with no allocation: the library's static scratch ----------------------------
+//
+// Callers that may not allocate device memory (inside a captured stream, a serving loop, a
+// harness that forbids it) sample through nf_defect_sample_onto_static, which works in the
+// file-scope array below instead of the handle's grown buffers. Every GPU the library runs
+// on holds its own copy, 16 MiB, until the process ends. Its layout, in doubles: the
+// bounding box (4), the box partials (4 per bounding-box workgroup), the status word (one
+// slot), the seeds' hashes (DF_STATIC_SEEDS), and the white-noise grid (the rest). A box
+// whose grid does not fit even for one seed hashes every tap instead (the same values).
+namespace dfd {
+constexpr size_t STATIC_DOUBLES = size_t(1) << 21;  // 16 MiB
+constexpr size_t STATIC_SEEDS = 16384;
+constexpr size_t STATIC_GRID_AT = 4 + 4 * (size_t)BOX_BLOCKS + 4 + STATIC_SEEDS;  // a multiple of 4
+}  // namespace dfd
+
+static __device__ __align__(256) double scratch[dfd::STATIC_DOUBLES];
+
+namespace {
+
+// The device address of `scratch` on each GPU, looked up once per device with that device
+// current (the host-side name `scratch` is not a device address and must never reach a
+// kernel or a copy).
+std::mutex g_scratchLock;
+double* g_scratchOn[64] = {};
+
+int static_scratch(hipStream_t stream, DfScratch& w) {
+    int dev = 0;
+    HIP_TRY(hipStreamGetDevice(stream, &dev));
+    if (dev < 0 || dev >= 64) return fail(NF_E_DEVICE, "device index out of range for the static scratch table");
+    double* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratchLock);
+        if (!g_scratchOn[dev]) {
+            int current = 0;
+            HIP_TRY(hipGetDevice(&current));
+            if (current != dev) HIP_TRY(hipSetDevice(dev));
+            void* ptr = nullptr;
+            const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (current != dev) HIP_TRY(hipSetDevice(current));
+            if (err != hipSuccess || !ptr)
+                return fail(NF_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+            g_scratchOn[dev] = (double*)ptr;
+        }
+        base = g_scratchOn[dev];
+    }
+    w.box = base;
+    w.boxPartial = base + 4;
+    w.status = (int*)(base + 4 + 4 * (size_t)dfd::BOX_BLOCKS);
+    w.h0 = (uint64_t*)(base + 4 + 4 * (size_t)dfd::BOX_BLOCKS + 4);
+    w.h0Cap = dfd::STATIC_SEEDS;
+    w.grid = base + dfd::STATIC_GRID_AT;
+    w.gridCap = dfd::STATIC_DOUBLES - dfd::STATIC_GRID_AT;
+    return NF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nf_defect_sample_onto_static(nf_defect* f, int32_t seed_count, const int64_t* seeds, int32_t node_count,
+                                 const double* r, const double* z, int32_t element_count, const int32_t* conn,
+                                 int32_t volume_rule, double depth, int32_t flags, double* out) {
+    g_error.clear();
+    if (!f) return fail(NF_E_ARGUMENT, "null handle");
+    if (!seeds) seed_count = 1;
+    if (seed_count < 0 || node_count < 0 || element_count < 0) return fail(NF_E_ARGUMENT, "negative count");
+    if ((size_t)seed_count > dfd::STATIC_SEEDS)
+        return fail(NF_E_ARGUMENT, "seed count " + std::to_string(seed_count) + " exceeds the static scratch's " +
+                                       std::to_string(dfd::STATIC_SEEDS));
+    if ((element_count && (!conn || !r || !z)) || (seed_count && element_count && !out))
+        return fail(NF_E_ARGUMENT, "null array");
+    if ((uintptr_t)conn % alignof(int4)) return fail(NF_E_ARGUMENT, "connectivity must be 16-byte aligned");
+    DfScratch w;
+    TRY(static_scratch(f->stream, w));
+    TRY(defect_sample(f, &w, seed_count, seeds, node_count, r, z, element_count, (const int4*)conn, volume_rule,
                       depth, flags, out));
     HIP_TRY(hipStreamSynchronize(f->stream));
     return NF_OK;
@@ -736,15 +873,17 @@ int nf_defect_points(nf_defect* f, int32_t function, int32_t n, const double* r,
     if (needW) HIP_TRY(hipMemcpyAsync(f->w, w, sizeof(double) * n, hipMemcpyHostToDevice, f->stream));
     if (function <= NF_DEFECT_STRAIN_VOLUME) {
         // the smoothed functions, over a grid on the points' own bounding box
+        DfScratch ws;
+        TRY(f->own_scratch(1, ws));
         if (function == NF_DEFECT_GAUSSIAN)
-            TRY((f->run<dfd::SITES_POINTS, dfd::OUT_GAUSSIAN>(1, nullptr, n, f->r, f->z, nullptr, n, nullptr,
-                                                               false, f->out)));
+            TRY((f->run<dfd::SITES_POINTS, dfd::OUT_GAUSSIAN>(ws, false, 1, nullptr, n, f->r, f->z, nullptr, n,
+                                                               nullptr, false, f->out)));
         else if (function == NF_DEFECT_STRAIN_AREA)
-            TRY((f->run<dfd::SITES_POINTS, dfd::OUT_RING>(1, nullptr, n, f->r, f->z, f->w, n, nullptr, false,
-                                                           f->out)));
+            TRY((f->run<dfd::SITES_POINTS, dfd::OUT_RING>(ws, false, 1, nullptr, n, f->r, f->z, f->w, n, nullptr,
+                                                           false, f->out)));
         else
-            TRY((f->run<dfd::SITES_POINTS, dfd::OUT_VOLUME>(1, nullptr, n, f->r, f->z, f->w, n, nullptr, false,
-                                                             f->out)));
+            TRY((f->run<dfd::SITES_POINTS, dfd::OUT_VOLUME>(ws, false, 1, nullptr, n, f->r, f->z, f->w, n,
+                                                             nullptr, false, f->out)));
     } else {
         HIP_TRY(hipMemsetAsync(f->status, 0, sizeof(int), f->stream));
         hipLaunchKernelGGL(dfd::k_df_pointwise, dim3(blocks_for(n)), dim3(BLOCK), 0, f->stream, f->p, function,
