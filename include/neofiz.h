@@ -243,6 +243,24 @@ int nf_enable_sensitivity(nf_solver* s);
 int nf_get_node_sensitivity(nf_solver* s, int32_t field, int32_t parameter, double* out);
 int nf_get_step_sensitivity(nf_solver* s, double* out);
 
+/* Discrete adjoint (no counterpart in the reference): the reverse-mode gradient of
+ * objectives of the final nodal state with respect to {anvil friction, Johnson-Cook A, B, n,
+ * C, m}, for the same configurations as the sensitivities above. nf_adjoint_enable (before
+ * the first step) makes every later run call record a checkpoint of the whole state every
+ * checkpoint_interval steps (1 .. 4096); the trajectory is the plain one, bit for bit.
+ * nf_adjoint_gradient then runs the reverse sweep over every recorded step: for each
+ * checkpoint, from the last, it reruns its steps recording what each step read, and runs
+ * them backwards. seeds holds, per objective, its derivatives with respect to the final
+ * UR, UZ, VR and VZ ([objective_count][4][node_count]); gradient receives [objective_count][6].
+ * At most two objectives per sweep. The solver is left in its final state. advance() is
+ * refused, and contact, damage and erosion cannot be set, while the adjoint is enabled.
+ * nf_adjoint_info writes {recorded steps, checkpoints, element passes differentiated through
+ * a Newton solve stopped on its iteration cap in the last sweep, checkpoint interval}. */
+int nf_adjoint_enable(nf_solver* s, int32_t checkpoint_interval);
+int nf_adjoint_gradient(nf_solver* s, int32_t objective_count, const double* seeds,
+                        double* gradient);
+int nf_adjoint_info(nf_solver* s, int64_t* out);
+
 /* Batched ensemble (no single counterpart in the reference, which runs an ensemble as one
  * solver per thread: M0Cylinder.java:870-907, sweep/Sweep.java:77-106). A batch is made from
  * solvers already set up through the calls above (mesh, material, wall, velocities ...) and

@@ -103,6 +103,9 @@ _PROTOTYPES = {
     "enable_sensitivity": (C.c_int, [_P]),
     "get_node_sensitivity": (C.c_int, [_P, C.c_int32, C.c_int32, c_double_p]),
     "get_step_sensitivity": (C.c_int, [_P, c_double_p]),
+    "adjoint_enable": (C.c_int, [_P, C.c_int32]),
+    "adjoint_gradient": (C.c_int, [_P, C.c_int32, c_double_p, c_double_p]),
+    "adjoint_info": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
     "batch_destroy": (C.c_int, [_P]),
     "batch_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32]),

@@ -68,10 +68,12 @@ __device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t, const n
 // yield check on the same temperature (J2.java:221 and :334 compute the same value twice).
 // UNIT_RATE: the reference strain rate is 1/s, where rate / eps0 is rate itself; the loop is
 // instantiated for it so that no iteration evaluates a division whose result is discarded.
-template <bool UNIT_RATE>
+// REC (the adjoint's recording pass only): *status becomes 2 when the solve stops on its
+// iteration cap rather than its tolerance. The plain instantiation has no such code.
+template <bool UNIT_RATE, bool REC = false>
 __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, double plasticStrain,
                                                 double thermal, double dt, const Mat& j,
-                                                const nfm::Tables& T) {
+                                                const nfm::Tables& T, int* status = nullptr) {
     const double tolerance = 1.0e-10 * j.a;
     double lo = 0.0;
     double hi = norm / stiffness;
@@ -97,14 +99,16 @@ __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, d
         const double next = dGamma - g / slope;
         dGamma = (next > lo && next < hi) ? next : 0.5 * (lo + hi);
     }
+    if (REC) *status = 2;
     return dGamma;
 }
 
+template <bool REC = false>
 __device__ __forceinline__ double j2_solve(double norm, double stiffness, double plasticStrain,
                                            double thermal, double dt, const Mat& j,
-                                           const nfm::Tables& T) {
-    return j.eps0 == 1.0 ? j2_solve_loop<true>(norm, stiffness, plasticStrain, thermal, dt, j, T)
-                         : j2_solve_loop<false>(norm, stiffness, plasticStrain, thermal, dt, j, T);
+                                           const nfm::Tables& T, int* status = nullptr) {
+    return j.eps0 == 1.0 ? j2_solve_loop<true, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status)
+                         : j2_solve_loop<false, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status);
 }
 
 // Damage state of one Gauss point (solver/Damage.java:156-192).
@@ -164,10 +168,13 @@ struct PointState {
 // when no material has kinematic hardening: the back stress is then +0 forever (its
 // co-rotation and its update only ever add signed zeros to +0), so it is neither loaded nor
 // stored, and subtracting it or adding alpha : n changes no bit of any result.
-template <bool DAMAGE, bool KIN>
+// REC (the adjoint's recording pass only): *status receives 0 for an elastic step, 1 for a
+// plastic one whose multiplier met the solve's tolerance, 2 where the solve hit its cap.
+template <bool DAMAGE, bool KIN, bool REC = false>
 __device__ __forceinline__ double j2_update(PointState& q, double dRR, double dZZ, double dTT,
                                             double dGamRZ, const Mat& fl, double strainDt,
-                                            DamagePoint* dmg, const nfm::Tables& T) {
+                                            DamagePoint* dmg, const nfm::Tables& T,
+                                            int* status = nullptr) {
     const double lambda = fl.lambda, mu = fl.mu;
     const double lt = lambda * (dRR + dZZ + dTT);
     const double tRR = q.s0 + lt + 2.0 * mu * dRR;
@@ -210,8 +217,10 @@ __device__ __forceinline__ double j2_update(PointState& q, double dRR, double dZ
         q.s1 = tZZ;
         q.s2 = tTT;
         q.s3 = tRZ;
+        if (REC) *status = 0;
         return 0.0;
     }
+    if (REC) *status = 1;
 
     const double stiffness = 2.0 * mu + NFD_TWO_THIRDS * fl.hKin;
     double dGamma;
@@ -221,7 +230,7 @@ __device__ __forceinline__ double j2_update(PointState& q, double dRR, double dZ
             dGamma = norm / stiffness;
         }
     } else {
-        dGamma = j2_solve(norm, stiffness, q.epsP, thermal, strainDt, fl, T);
+        dGamma = j2_solve<REC>(norm, stiffness, q.epsP, thermal, strainDt, fl, T, status);
     }
     const double inv = 1.0 / norm;
     const double nRR = xRR * inv, nZZ = xZZ * inv, nTT = xTT * inv, nRZ = xRZ * inv;

@@ -565,9 +565,12 @@ __device__ __forceinline__ void write_zero_slots(const Dev& d, int e) {
 }
 
 // One element under reduced integration, solver/ExplicitSolver.java:1373-1504.
-template <bool DAMAGE, bool KIN>
+// REC (the adjoint's recording pass only): *dGammaOut and *status receive the return map's
+// multiplier and outcome (j2_update).
+template <bool DAMAGE, bool KIN, bool REC = false>
 __device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, double strainDt,
-                                       const nfm::Tables& T) {
+                                       const nfm::Tables& T, double* dGammaOut = nullptr,
+                                       int* status = nullptr) {
     const Mat fl = load_mat(d, e);
     const double mu = fl.mu;
     const double mid = -0.5 * strainDt;
@@ -625,7 +628,8 @@ __device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, d
     }
     nfd::DamagePoint dp;
     if (DAMAGE) load_damage(d, e, dp);
-    nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp, T);
+    const double dGamma = nfd::j2_update<DAMAGE, KIN, REC>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp, T, status);
+    if (REC) *dGammaOut = dGamma;
     store_point<KIN>(d, e, q);
     if (DAMAGE) {
         d.dStrength[e] = dp.strength;
@@ -1192,6 +1196,7 @@ int check_material(const nf_material* m) {
 // ------------------------------------------------------------------ the handle
 
 #include "tangent.h"
+#include "adjoint.h"
 
 struct nf_solver {
     hipStream_t stream = nullptr;
@@ -1212,6 +1217,19 @@ struct nf_solver {
     bool anyJC = false;
     bool hasKin = false;  // some material has kinematic hardening
     bool tangent = false;  // parameter sensitivities are carried (nf_enable_sensitivity)
+    // Discrete adjoint (nf_adjoint_enable): checkpoints of the whole state every adjK steps
+    // of the forward run, then the reverse sweep of nf_adjoint_gradient.
+    bool adjoint = false;
+    int adjK = 64;
+    struct Checkpoint {
+        long long steps = 0;
+        int q = 0;
+        char* mem = nullptr;
+    };
+    std::vector<Checkpoint> checkpoints;  // after 0, K, 2K, ... steps
+    Checkpoint finalState;                // after the last recorded step
+    ADev ad{};
+    bool adjAllocated = false;
     int cFCapAlloc = 0;    // capture-grid entries allocated (nf_contact_set_limits may use fewer)
     TDev td{};
     double wallPlane = 0.0, wallFriction = 0.0;
@@ -1235,6 +1253,9 @@ struct nf_solver {
         for (hipEvent_t e : statusEvent)
             if (e) (void)hipEventDestroy(e);
         if (statusHost) (void)hipHostFree(statusHost);
+        for (Checkpoint& c : checkpoints)
+            if (c.mem) (void)hipFree(c.mem);
+        if (finalState.mem) (void)hipFree(finalState.mem);
         for (void* p : owned) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1465,15 +1486,251 @@ struct nf_solver {
     // zero-interval assembly is followed by the first nodal update (PHASE_START with
     // gatherFirst); advance() ends with a bare gather.
     int assembleAdvance() {
-        if (tangent) return fail(NF_E_STATE, "advance() is not differentiated; sensitivities follow step() only");
+        if (tangent || adjoint) return fail(NF_E_STATE, "advance() is not differentiated; sensitivities follow step() only");
         TRY(launchElement(MODE_ADVANCE, q));
         TRY(launchGN(PHASE_ASSEMBLE_END, q, 1));
         q ^= 1;
         return NF_OK;
     }
 
+    // ---------------------------------------------------------------- discrete adjoint
+    // A checkpoint holds every array the step loop reads or writes across steps, and Ctl.
+    size_t checkpointBytes() const {
+        return (size_t)N * (3 * sizeof(double2) + 1) + (size_t)E * 9 * sizeof(double) + sizeof(Ctl) + 64;
+    }
+
+    int checkpointCopy(Checkpoint& c, bool save) {
+        char* p = c.mem;
+        auto move = [&](void* live, size_t bytes) -> int {
+            HIP_TRY(hipMemcpyAsync(save ? (void*)p : live, save ? live : (void*)p, bytes, hipMemcpyDeviceToDevice, stream));
+            p += (bytes + 15) & ~(size_t)15;
+            return NF_OK;
+        };
+        TRY(move(ctl, sizeof(Ctl)));
+        TRY(move(d.u, sizeof(double2) * N));
+        TRY(move(d.v, sizeof(double2) * N));
+        TRY(move(d.f, sizeof(double2) * N));
+        TRY(move(d.sig0, sizeof(double) * E));
+        TRY(move(d.sig1, sizeof(double) * E));
+        TRY(move(d.sig2, sizeof(double) * E));
+        TRY(move(d.sig3, sizeof(double) * E));
+        TRY(move(d.epsP, sizeof(double) * E));
+        TRY(move(d.work, sizeof(double) * E));
+        TRY(move(d.hgR, sizeof(double) * E));
+        TRY(move(d.hgZ, sizeof(double) * E));
+        TRY(move(d.hgK, sizeof(double) * E));
+        TRY(move(d.flags, N));
+        return NF_OK;
+    }
+
+    int saveCheckpoint(Checkpoint& c, long long steps) {
+        if (!c.mem) HIP_TRY(hipMalloc((void**)&c.mem, checkpointBytes() + 16 * 16));
+        c.steps = steps;
+        c.q = q;
+        TRY(checkpointCopy(c, true));
+        return sync();
+    }
+
+    int restoreCheckpoint(Checkpoint& c) {
+        TRY(checkpointCopy(c, false));
+        q = c.q;
+        return sync();
+    }
+
+    // The forward run with checkpoints: the plain steps in chunks that end on multiples of
+    // adjK, each chunk preceded by a checkpoint of the state it starts from.
+    int adjointRun(double endTime, long long maxSteps, long long* taken) {
+        Step h;
+        TRY(readStep(h));
+        const long long start = h.steps;
+        long long total = 0;
+        const long long limit = maxSteps > 0 ? maxSteps : 0x7FFFFFFFFFFFFFFFLL;
+        adjoint = false;  // the chunks run the plain path
+        int rc = NF_OK;
+        while (rc == NF_OK) {
+            const long long cur = start + total;
+            if (cur % adjK == 0 && (checkpoints.empty() || checkpoints.back().steps < cur)) {
+                checkpoints.emplace_back();
+                rc = saveCheckpoint(checkpoints.back(), cur);
+                if (rc) break;
+            }
+            const long long chunk = std::min<long long>(adjK - cur % adjK, limit - total);
+            if (chunk <= 0) break;
+            long long t = 0;
+            rc = runSteps(endTime, chunk, &t);
+            total += t;
+            if (t < chunk) break;
+        }
+        adjoint = true;
+        if (rc) return rc;
+        if (taken) *taken = total;
+        Step e;
+        TRY(readStep(e));
+        return saveCheckpoint(finalState, e.steps);
+    }
+
+    int adjointAllocate() {
+        if (adjAllocated) return NF_OK;
+        ARec& a = ad.rec;
+        const size_t K = adjK;
+        a.K = adjK;
+        TRY(alloc(&a.u, (K + 1) * N));
+        TRY(alloc(&a.v, (K + 1) * N));
+        TRY(alloc(&a.f, (K + 1) * N));
+        TRY(alloc(&a.flags, (K + 1) * N));
+        TRY(alloc(&a.sig, K * 4 * E));
+        TRY(alloc(&a.ep, K * E));
+        TRY(alloc(&a.work, K * E));
+        TRY(alloc(&a.hg, K * 2 * E));
+        TRY(alloc(&a.dGamma, K * E));
+        TRY(alloc(&a.status, K * E));
+        TRY(alloc(&a.capJac, K * CAP_DIRS * E));
+        TRY(alloc(&a.edge, (size_t)E));
+        TRY(alloc(&a.stepDt, K + 1));
+        TRY(alloc(&a.stepDtPrev, K + 1));
+        TRY(alloc(&a.minEdgeBits, K + 1));
+        TRY(alloc(&a.argmin, K + 1));
+        TRY(alloc(&a.capList, K * E));
+        TRY(alloc(&a.capCount, 1));
+        const size_t W = AW_MAX;
+        TRY(alloc(&ad.uBar, W * N));
+        TRY(alloc(&ad.vBar, W * N));
+        TRY(alloc(&ad.fBar, W * N));
+        TRY(alloc(&ad.cornerBar, W * 4 * E));
+        TRY(alloc(&ad.sigBar, W * 4 * E));
+        TRY(alloc(&ad.epBar, W * E));
+        TRY(alloc(&ad.workBar, W * E));
+        TRY(alloc(&ad.hgBar, W * 2 * E));
+        // block counts for the widest layout (one objective per half workgroup)
+        ad.elemBlocks = (E + BLOCK / 2 - 1) / (BLOCK / 2);
+        ad.nodeBlocks = (N + BLOCK / 2 - 1) / (BLOCK / 2);
+        TRY(alloc(&ad.elemPartial, W * ad.elemBlocks * A_EP));
+        TRY(alloc(&ad.nodePartial, W * ad.nodeBlocks * A_NP));
+        TRY(alloc(&ad.ctl, 1));
+        adjAllocated = true;
+        return NF_OK;
+    }
+
+    // Recomputes the steps after checkpoint c (count of them) with every step recorded; the
+    // per-step dt, dtPrev and shortest edge come back in hDt, hDtPrev, hMin (index j).
+    int adjointRecord(Checkpoint& c, int count, std::vector<double>& hDt, std::vector<double>& hDtPrev,
+                      std::vector<double>& hMin) {
+        TRY(restoreCheckpoint(c));
+        ARec& a = ad.rec;
+        Step h;
+        TRY(readStep(h));
+        h.endTime = HUGE_VAL;
+        h.stepLimit = c.steps + count;
+        TRY(writeStep(h));
+        HIP_TRY(hipMemsetAsync(a.capCount, 0, sizeof(int), stream));
+        const int gSave = blocks_for(std::max(N, E));
+        hipLaunchKernelGGL(nfa::k_adj_save, dim3(gSave), dim3(BLOCK), 0, stream, d, a, 0, q);
+        if (c.steps == 0) {
+            TRY(launchElement(MODE_INITIAL, q));
+            TRY(launchGN(PHASE_START, q, 1));
+        } else {
+            TRY(launchGN(PHASE_START, q, 0));
+        }
+        q ^= 1;
+        for (int j = 1; j <= count; j++) {
+            hipLaunchKernelGGL(nfa::k_adj_save, dim3(gSave), dim3(BLOCK), 0, stream, d, a, j, q);
+            hipLaunchKernelGGL(nfa::k_element_rec, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, a, j, q);
+            hipLaunchKernelGGL(nfa::k_adj_argmin, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, a, j, q);
+            TRY(launchGN(PHASE_STEP, q, 0));
+            q ^= 1;
+        }
+        HIP_TRY(hipGetLastError());
+        hDt.assign(count + 1, 0.0);
+        hDtPrev.assign(count + 1, 0.0);
+        std::vector<unsigned long long> bits(count + 1);
+        int caps = 0;
+        HIP_TRY(hipMemcpyAsync(hDt.data(), a.stepDt, sizeof(double) * (count + 1), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(hDtPrev.data(), a.stepDtPrev, sizeof(double) * (count + 1), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(bits.data(), a.minEdgeBits, sizeof(unsigned long long) * (count + 1), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(&caps, a.capCount, sizeof(int), hipMemcpyDeviceToHost, stream));
+        TRY(sync());
+        hMin.assign(count + 1, 0.0);
+        for (int j = 1; j <= count; j++) hMin[j] = nfm::from_bits(bits[j]);
+        if (caps > 0) {
+            const long long threads = (long long)caps * CAP_DIRS;
+            hipLaunchKernelGGL(nfa::k_adj_cap, dim3((unsigned)((threads + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream,
+                               d, a, caps);
+            HIP_TRY(hipGetLastError());
+        }
+        adjCaps += caps;
+        return NF_OK;
+    }
+    long long adjCaps = 0;  // element passes differentiated through the capped solve's Jacobian
+
+    // The reverse sweep: seeds [W][4][N] are the objectives' adjoints of (ur, uz, vr, vz) at
+    // the final state; grad [W][6] receives their gradients.
+    int adjointSweep(int W, const double* seeds, double* grad) {
+        TRY(adjointAllocate());
+        const long long Nsteps = finalState.steps;
+        ad.W = W;
+        ad.elemBlocks = (E + BLOCK / W - 1) / (BLOCK / W);
+        ad.nodeBlocks = (N + BLOCK / W - 1) / (BLOCK / W);
+        // zero every adjoint, then the seeds
+        HIP_TRY(hipMemsetAsync(ad.vBar, 0, sizeof(double2) * W * N, stream));
+        HIP_TRY(hipMemsetAsync(ad.fBar, 0, sizeof(double2) * W * N, stream));
+        HIP_TRY(hipMemsetAsync(ad.sigBar, 0, sizeof(double) * W * 4 * E, stream));
+        HIP_TRY(hipMemsetAsync(ad.epBar, 0, sizeof(double) * W * E, stream));
+        HIP_TRY(hipMemsetAsync(ad.workBar, 0, sizeof(double) * W * E, stream));
+        HIP_TRY(hipMemsetAsync(ad.hgBar, 0, sizeof(double) * W * 2 * E, stream));
+        HIP_TRY(hipMemsetAsync(ad.elemPartial, 0, sizeof(double) * W * ad.elemBlocks * A_EP, stream));
+        HIP_TRY(hipMemsetAsync(ad.nodePartial, 0, sizeof(double) * W * ad.nodeBlocks * A_NP, stream));
+        HIP_TRY(hipMemsetAsync(ad.ctl, 0, sizeof(AdjCtl), stream));
+        {
+            std::vector<double2> u((size_t)W * N), v((size_t)W * N);
+            for (int w = 0; w < W; w++)
+                for (int i = 0; i < N; i++) {
+                    const double* sd = seeds + (size_t)w * 4 * N;
+                    u[(size_t)w * N + i] = make_double2(sd[i], sd[N + i]);
+                    v[(size_t)w * N + i] = make_double2(sd[2 * N + i], sd[3 * N + i]);
+                }
+            HIP_TRY(hipMemcpyAsync(ad.uBar, u.data(), sizeof(double2) * W * N, hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(ad.vBar, v.data(), sizeof(double2) * W * N, hipMemcpyHostToDevice, stream));
+            TRY(sync());
+        }
+        adjCaps = 0;
+        bool first = true;
+        double nextDt = 0.0, nextDtPrev = 0.0;
+        std::vector<double> hDt, hDtPrev, hMin;
+        const int eb = ad.elemBlocks, nbk = ad.nodeBlocks;
+        for (int c = (int)checkpoints.size() - 1; c >= 0; c--) {
+            Checkpoint& ck = checkpoints[c];
+            const long long end = c + 1 < (int)checkpoints.size() ? checkpoints[c + 1].steps : Nsteps;
+            const int count = (int)(end - ck.steps);
+            if (count <= 0) continue;
+            TRY(adjointRecord(ck, count, hDt, hDtPrev, hMin));
+            for (int j = count; j >= 1; j--) {
+                hipLaunchKernelGGL(nfa::k_adj_reduce, dim3(1), dim3(BLOCK), 0, stream, d, ad, nextDt, nextDtPrev,
+                                   first ? 0 : 1, hMin[j], 1);
+                hipLaunchKernelGGL(nfa::k_adj_element, dim3(eb), dim3(BLOCK), 0, stream, d, ad, j);
+                if (hasWall) hipLaunchKernelGGL(nfa::k_adj_node<true>, dim3(nbk), dim3(BLOCK), 0, stream, d, ad, j);
+                else hipLaunchKernelGGL(nfa::k_adj_node<false>, dim3(nbk), dim3(BLOCK), 0, stream, d, ad, j);
+                nextDt = hDt[j];
+                nextDtPrev = hDtPrev[j];
+                first = false;
+            }
+            HIP_TRY(hipGetLastError());
+        }
+        // fold the first step's partials into the gradient
+        hipLaunchKernelGGL(nfa::k_adj_reduce, dim3(1), dim3(BLOCK), 0, stream, d, ad, nextDt, nextDtPrev,
+                           first ? 0 : 1, 0.0, 0);
+        HIP_TRY(hipGetLastError());
+        AdjCtl h;
+        HIP_TRY(hipMemcpyAsync(&h, ad.ctl, sizeof h, hipMemcpyDeviceToHost, stream));
+        TRY(sync());
+        for (int w = 0; w < W; w++)
+            for (int k = 0; k < NP; k++) grad[w * NP + k] = h.grad[w][k];
+        // leave the solver as the forward run left it
+        return restoreCheckpoint(finalState);
+    }
+
     // run steps while time < endTime and fewer than maxSteps have been taken.
     int runSteps(double endTime, long long maxSteps, long long* taken) {
+        if (adjoint) return adjointRun(endTime, maxSteps, taken);
         if (tangent && (hasContact || hasDamage || hasErosion))
             return fail(NF_E_STATE, "the sensitivity kernels do not carry contact, damage or erosion");
         Step h;
@@ -1861,7 +2118,7 @@ int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double fric
 // damage and erosion have no derivative kernels, so switching them on afterwards is refused
 // (switching contact off is allowed: it is already off).
 static int refuse_if_sensitive(const nf_solver* s, const char* what) {
-    if (!s->tangent) return NF_OK;
+    if (!s->tangent && !s->adjoint) return NF_OK;
     return fail(NF_E_STATE, std::string(what) + " cannot be set on a solver carrying sensitivities; "
                 "they cover reduced integration of one Johnson-Cook material without damage, erosion or contact");
 }
@@ -2801,6 +3058,7 @@ int nf_enable_sensitivity(nf_solver* s) {
     Step h;
     TRY(s->readStep(h));
     if (h.steps != 0) return fail(NF_E_STATE, "sensitivities start from the initial state; enable them before the first step");
+    if (s->adjoint) return fail(NF_E_STATE, "a solver carries forward sensitivities or the adjoint, not both");
     if (s->integration != NF_REDUCED || s->hasDamage || s->hasErosion || s->hasContact || s->edges ||
         !s->materialOf.empty() || !s->anyJC || s->hasKin)
         return fail(NF_E_STATE, "sensitivities cover reduced integration of one Johnson-Cook material without "
@@ -2850,6 +3108,47 @@ int nf_get_step_sensitivity(nf_solver* s, double* out) {
     if (!s->tangent) return fail(NF_E_STATE, "sensitivities are not enabled");
     TRY(s->sync());
     HIP_TRY(hipMemcpy(out, s->td.tstep + (size_t)s->q * 2 * NP, sizeof(double) * 2 * NP, hipMemcpyDeviceToHost));
+    return NF_OK;
+}
+
+int nf_adjoint_enable(nf_solver* s, int32_t checkpoint_interval) {
+    g_error.clear();
+    if (s->adjoint) return NF_OK;
+    if (checkpoint_interval < 1 || checkpoint_interval > 4096)
+        return fail(NF_E_ARGUMENT, "the checkpoint interval lies in [1, 4096] steps");
+    Step h;
+    TRY(s->readStep(h));
+    if (h.steps != 0) return fail(NF_E_STATE, "the adjoint records from the initial state; enable it before the first step");
+    if (s->tangent) return fail(NF_E_STATE, "a solver carries forward sensitivities or the adjoint, not both");
+    if (s->integration != NF_REDUCED || s->hasDamage || s->hasErosion || s->hasContact || s->edges ||
+        !s->materialOf.empty() || !s->anyJC || s->hasKin)
+        return fail(NF_E_STATE, "the adjoint covers reduced integration of one Johnson-Cook material without "
+                                "kinematic hardening, damage, erosion, contact or pressure edges");
+    s->adjK = checkpoint_interval;
+    s->adjoint = true;
+    s->graphDirty = true;
+    return NF_OK;
+}
+
+int nf_adjoint_gradient(nf_solver* s, int32_t objective_count, const double* seeds, double* gradient) {
+    g_error.clear();
+    if (!s->adjoint) return fail(NF_E_STATE, "the adjoint is not enabled");
+    if (objective_count < 1 || objective_count > AW_MAX)
+        return fail(NF_E_ARGUMENT, "one or two objectives per reverse sweep");
+    if (!seeds || !gradient) return fail(NF_E_ARGUMENT, "null array");
+    if (s->checkpoints.empty() || s->finalState.steps == 0) return fail(NF_E_STATE, "no steps were recorded");
+    Step h;
+    TRY(s->readStep(h));
+    if (h.steps != s->finalState.steps) return fail(NF_E_STATE, "the solver moved past the recorded run");
+    return s->adjointSweep(objective_count, seeds, gradient);
+}
+
+int nf_adjoint_info(nf_solver* s, int64_t* out) {
+    g_error.clear();
+    out[0] = s->finalState.steps;
+    out[1] = (int64_t)s->checkpoints.size();
+    out[2] = s->adjCaps;
+    out[3] = s->adjK;
     return NF_OK;
 }
 

@@ -99,6 +99,36 @@ def gradients(point, end_time=taylor.ROD_DURATION, max_steps=0):
     return obj, grad, steps, seconds
 
 
+def objective_seeds(mesh, extremes):
+    """d(L/L0, mushroom diameter)/d(UR, UZ, VR, VZ) at the final state, [2][4][nodes], for
+    the extreme nodes (i_zmax, i_zmin, i_rmax) that set them."""
+    i_hi, i_lo, i_r = extremes
+    seeds = np.zeros((2, 4, mesh.nodeCount))
+    seeds[0, 1, i_hi] += 1.0 / taylor.ROD_LENGTH
+    seeds[0, 1, i_lo] -= 1.0 / taylor.ROD_LENGTH
+    seeds[1, 0, i_r] = 2.0
+    return seeds
+
+
+def adjoint_gradients(point, end_time=taylor.ROD_DURATION, max_steps=0, interval=64):
+    """The rod run with the discrete adjoint: a forward run recording checkpoints every
+    `interval` steps, then one reverse sweep carrying both objectives. Returns (objectives,
+    gradient [2][6], steps, forward seconds, reverse seconds, adjoint info)."""
+    mesh, solver = build(point)
+    solver.enableAdjoint(interval)
+    t0 = time.perf_counter()
+    solver.runUntil(end_time, max_steps)
+    fwd = time.perf_counter() - t0
+    obj, extremes = objectives(mesh, solver)
+    t0 = time.perf_counter()
+    grad = solver.adjointGradient(objective_seeds(mesh, extremes))
+    rev = time.perf_counter() - t0
+    steps = solver.steps()
+    info = solver.adjointInfo()
+    solver.shutdown()
+    return obj, grad, steps, fwd, rev, info
+
+
 def forward(point, end_time=taylor.ROD_DURATION, max_steps=0):
     """The plain forward run of the same rod: (objectives, steps, seconds)."""
     mesh, solver = build(point)

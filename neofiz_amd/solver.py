@@ -768,6 +768,32 @@ class ExplicitSolver:
         self._call("get_node_sensitivity", _abi.NODE_FIELDS[field], parameter, _dptr(out))
         return out
 
+    def enableAdjoint(self, checkpointInterval=64):
+        """Records checkpoints every `checkpointInterval` steps of the runs that follow, for
+        adjointGradient (nf_adjoint_enable)."""
+        self._call("adjoint_enable", int(checkpointInterval))
+
+    def adjointGradient(self, seeds):
+        """Reverse-mode gradients of up to two objectives of the final state: `seeds` is
+        [objectives][4][nodes] (d objective / d UR, UZ, VR, VZ); returns [objectives][6] over
+        SENSITIVITY_PARAMETERS."""
+        sd = np.ascontiguousarray(seeds, dtype=np.float64)
+        if sd.ndim == 2:
+            sd = sd[None]
+        count = sd.shape[0]
+        if sd.shape[1:] != (4, self.mesh.nodeCount):
+            raise IllegalArgumentException("seeds are [objectives][4][nodes]")
+        out = np.zeros((count, len(self.SENSITIVITY_PARAMETERS)))
+        self._call("adjoint_gradient", count, _dptr(sd), _dptr(out))
+        return out
+
+    def adjointInfo(self):
+        """(recorded steps, checkpoints, capped-solve element passes of the last sweep,
+        checkpoint interval)."""
+        out = (C.c_int64 * 4)()
+        self._call("adjoint_info", out)
+        return tuple(out)
+
     def centroidStress(self, e):
         """Mean stress over an element's Gauss points {rr, zz, tt, rz} (:1886-1894)."""
         ppe = self.state().points_per_element

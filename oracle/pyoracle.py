@@ -17,7 +17,12 @@ _binding = None
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    """Brings the oracle up to date; concurrent test workers take turns (a file lock), so none
+    loads the library while another is relinking it."""
+    import fcntl
+    with open(os.path.join(HERE, ".build.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
 def binding():

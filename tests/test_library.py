@@ -44,7 +44,8 @@ def test_hip_library_targets_gfx950():
 # forward solver instead).
 GPU_ONLY = ("nf_profile", "nf_profile_read", "nf_copy_bandwidth", "nf_enable_sensitivity",
             "nf_get_node_sensitivity", "nf_get_step_sensitivity", "nf_defect_sample_onto_device",
-            "nf_contact_set_limits", "nf_defect_sample_onto_static")
+            "nf_contact_set_limits", "nf_defect_sample_onto_static", "nf_adjoint_enable",
+            "nf_adjoint_gradient", "nf_adjoint_info")
 
 
 def test_oracle_exports_the_same_surface(oracle):
