@@ -218,22 +218,34 @@ def rod_run(nr, nz, johnson_cook, cap):
 
 
 def sensitivity_leg(nr, nz):
-    """Cost of the step loop's derivative: the Johnson-Cook Taylor rod on an nr x nz mesh to
-    80 us, once plain and once carrying its six parameter sensitivities (the same steps,
-    bit-identical trajectory), both timed on the GPU."""
+    """Cost of the step loop's derivatives on the Johnson-Cook Taylor rod (nr x nz, to 80 us):
+    the plain forward run; the discrete adjoint (the forward run recording a checkpoint every
+    64 steps, then one reverse sweep carrying both objectives, recomputing each checkpoint's
+    steps); the forward-mode tangent in six directions. The adjoint's cost ratio is
+    (recording run + reverse sweep) / plain run; its twelve gradients are compared with the
+    tangent's."""
+    import numpy as np
     from neofiz_amd import sensitivity as S, taylor
     jc = taylor.ROD_JC.johnsonCook
     pt = S.Point(nr, nz, taylor.ROD_SPEED, taylor.ROD_JC_FRICTION, jc.a, jc.b, jc.n, jc.c, jc.m)
-    print(f"# sensitivities: {nr}x{nz} rod to 80 us, forward then with derivatives", file=sys.stderr, flush=True)
+    print(f"# sensitivities: {nr}x{nz} rod to 80 us: plain, adjoint, tangent", file=sys.stderr, flush=True)
     obj, fwd_steps, fwd_s = S.forward(pt)
-    obj2, grad, steps, sens_s = S.gradients(pt)
-    return {"workload": f"taylor-rod-jc-{nr}x{nz}, 80 us", "steps": steps,
-            "forward_seconds": fwd_s, "with_derivative_seconds": sens_s,
-            "cost_ratio": sens_s / fwd_s, "same_trajectory": fwd_steps == steps and obj == obj2,
+    obj_a, grad_a, steps_a, rec_s, rev_s, info = S.adjoint_gradients(pt, interval=64)
+    obj_t, grad_t, steps_t, tan_s = S.gradients(pt)
+    agree = float(np.max(np.abs(grad_a - grad_t) / np.abs(grad_t)))
+    return {"workload": f"taylor-rod-jc-{nr}x{nz}, 80 us", "steps": fwd_steps,
+            "forward_seconds": fwd_s,
+            "adjoint": {"recording_forward_seconds": rec_s, "reverse_seconds": rev_s,
+                        "cost_ratio": (rec_s + rev_s) / fwd_s, "checkpoint_interval": info[3],
+                        "checkpoints": info[1], "capped_solve_element_passes": info[2]},
+            "tangent": {"seconds": tan_s, "cost_ratio": tan_s / fwd_s},
+            "cost_ratio": (rec_s + rev_s) / fwd_s,
+            "adjoint_vs_tangent_max_rel": agree,
+            "same_trajectory": fwd_steps == steps_a == steps_t and obj == obj_a == obj_t,
             "objectives": {"L/L0": obj[0], "mushroom_diameter": obj[1]},
-            "gradient": {name: {"L/L0": grad[0, k], "mushroom_diameter": grad[1, k]}
+            "gradient": {name: {"L/L0": grad_a[0, k], "mushroom_diameter": grad_a[1, k]}
                          for k, name in enumerate(S.PARAMETERS)},
-            "mode": "forward (tangent) derivative in six directions"}
+            "mode": "reverse (discrete adjoint, checkpoints every 64 steps); tangent beside it"}
 
 
 def perforation_leg(elements, warmup, steps):

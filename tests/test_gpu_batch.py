@@ -58,3 +58,32 @@ def test_batch_refuses_contact_and_damage():
     s.setDamage(np.full(mesh.elementCount, 1e4), np.full(mesh.elementCount, 0.5))
     with pytest.raises(IllegalStateException):
         Batch([s])
+
+
+def test_batch_with_the_large_meshes_bitwise():
+    """A batch holding a 32 x 320 and a 64 x 640 rod of the ensemble's draw (with 8 x 80 and
+    16 x 160 beside them), 6000 steps each: every instance equals its solo oracle run."""
+    import os
+    from oracle.pyoracle import OracleSolver
+    picks = {}
+    for inst in ensemble.draw(64, 5):
+        picks.setdefault((inst.nr, inst.nz), inst)
+    insts = [picks[m] for m in ((8, 80), (16, 160), (32, 320), (64, 640))]
+    built = [ensemble.build(i) for i in insts]
+    batch = Batch([s for _, s in built])
+    print("batch of 4 (8x80 .. 64x640), 6000 steps", flush=True)
+    batch.runUntil(80e-6, 6000)
+    batch.scatter()
+    for k, inst in enumerate(insts):
+        mesh, ref = ensemble.build(inst, OracleSolver)
+        ref.setThreads(min(16, os.cpu_count() or 1))
+        ref.runUntil(80e-6, 6000)
+        gpu = built[k][1]
+        assert gpu.steps() == ref.steps() == 6000, inst
+        assert gpu.time() == ref.time() and gpu.timestep() == ref.timestep()
+        for f in FIELDS:
+            assert first_difference(gpu.nodeField(f), ref.nodeField(f)) is None, f"{inst.nr}x{inst.nz}: {f}"
+        for f in ("stress", "plastic_strain", "plastic_work"):
+            assert first_difference(gpu.pointField(f), ref.pointField(f)) is None, f"{inst.nr}x{inst.nz}: {f}"
+        ref.shutdown()
+    batch.close()

@@ -125,6 +125,50 @@ def test_rod_jc_8x80_full_run_bitwise():
     assert gs.wall_stuck_count == rs.wall_stuck_count
 
 
+def threaded_oracle_builder(material, nr, nz, friction):
+    """The rod, with the oracle threaded on the host's share of cores (the answer does not
+    depend on the thread count)."""
+    import os
+    inner = rod_builder(material, nr, nz, friction)
+
+    def build(cls):
+        s = inner(cls)
+        if cls is not ExplicitSolver:
+            s.setThreads(min(16, os.cpu_count() or 1))
+        return s
+    return build
+
+
+@pytest.mark.parametrize("nr,nz", [(16, 160), (32, 320)])
+def test_rod_jc_ladder_full_run_bitwise(nr, nz):
+    """The Johnson-Cook rods of the refinement ladder (mu 0.09) from t = 0 to 80 us: the same
+    step count, every state array bit-identical, and so the same L/L0 and mushroom diameter
+    (TaylorImpactCase's measurements)."""
+    from oracle.pyoracle import OracleSolver
+    out = []
+    for cls in (ExplicitSolver, OracleSolver):
+        mesh = None
+        s = threaded_oracle_builder(taylor.ROD_JC, nr, nz, 0.09)(cls)
+        a0 = s.audits()
+        steps = s.runUntil(taylor.ROD_DURATION)
+        m = taylor.measure(s.mesh, s, taylor.ROD_LENGTH, a0.axial_momentum, a0.kinetic_energy)
+        print(f"{cls.__name__} {nr}x{nz}: {steps} steps, L/L0 {m['lengthRatio']!r}, "
+              f"mushroom {m['mushroomDiameter']!r}", flush=True)
+        out.append((snapshot(s), s.state(), m))
+        s.shutdown()
+    (g, gs, gm), (r, rs, rm) = out
+    assert gs.steps == rs.steps and gs.time == rs.time
+    assert_parity(g, r, f"JC {nr}x{nz} to 80 us")
+    assert gm["lengthRatio"] == rm["lengthRatio"] and gm["mushroomDiameter"] == rm["mushroomDiameter"]
+    assert gm["maxPlasticStrain"] == rm["maxPlasticStrain"]
+
+
+def test_rod_jc_64x640_5000_steps_bitwise():
+    (g, _, gs), (r, _, rs) = run_pair(threaded_oracle_builder(taylor.ROD_JC, 64, 640, 0.09), steps=5000)
+    assert gs.steps == rs.steps == 5000
+    assert_parity(g, r, "JC 64x640 5000 steps")
+
+
 def test_rod_jc_16x160_prefix_bitwise():
     (g, _, gs), (r, _, rs) = run_pair(rod_builder(taylor.ROD_JC, 16, 160, 0.09), steps=3000)
     assert_parity(g, r, "JC 16x160 3000 steps")

@@ -31,6 +31,8 @@ def run(cls, field_binding, threads=None):
     out["eroded"] = s.elementField("eroded")
     out["state"] = (st.steps, st.time, st.eroded_elements, st.contact_pairs, st.contact_escaped_nodes)
     out["energy"] = st.contact_energy
+    out["pairs"] = s.contactPairs()
+    out["surface"] = s.contactSurface()
     print(f"{cls.__name__}: {mesh.elementCount} elements, {STEPS} steps in {time.perf_counter() - t0:.1f} s, "
           f"{st.eroded_elements} eroded, {st.contact_pairs} pairs", flush=True)
     s.shutdown()
@@ -48,3 +50,6 @@ def test_full_size_perforation_is_bitwise():
     for k in ("ur", "uz", "vr", "vz"):
         assert same_bits(gpu[k], ref[k]), (k, first_difference(gpu[k], ref[k]))
     assert abs(gpu["energy"] - ref["energy"]) <= 1e-12 * abs(ref["energy"])
+    # the pair set itself (slave node, master segment ends) and the rebuilt surface
+    assert np.array_equal(gpu["pairs"], ref["pairs"]) and len(ref["pairs"]) == ref["state"][3]
+    assert np.array_equal(gpu["surface"], ref["surface"]) and len(ref["surface"]) > 1000
