@@ -169,64 +169,82 @@ def finite_differences(point, end_time=taylor.ROD_DURATION, max_steps=0, rel=1e-
     return grad, steps
 
 
+def central_differences(point, steps, extremes, rel=1e-6):
+    """Central differences of the forward solver at a fixed step count (the derivative the
+    adjoint and the tangent compute is that of the state after `steps` steps; a perturbed run
+    stopped by the end time instead may take a step more or less, a jump of the map). All
+    twelve perturbed rods run as one batch. Returns (fd [2][6], objectives of the + and - runs
+    [2][2][6])."""
+    ih, il, ir = extremes
+    pts = []
+    for q, v in enumerate(point.values()):
+        h = rel * abs(v) if v != 0.0 else rel
+        pts += [point.with_value(q, v + h), point.with_value(q, v - h)]
+    built = [build(p) for p in pts]
+    batch = Batch([x for _, x in built])
+    batch.runUntil(1.0, steps)  # the step count binds, not the time
+    batch.scatter()
+    batch.close()
+    fd = np.zeros((2, len(PARAMETERS)))
+    obj = np.zeros((2, 2, len(PARAMETERS)))
+    for q in range(len(PARAMETERS)):
+        dp = pts[2 * q].values()[q] - pts[2 * q + 1].values()[q]
+        for side in range(2):
+            mesh, sv = built[2 * q + side]
+            assert sv.steps() == steps
+            z = mesh.z + sv.axialDisplacement()
+            r = mesh.r + sv.radialDisplacement()
+            obj[side, 0, q] = (z[ih] - z[il]) / taylor.ROD_LENGTH
+            obj[side, 1, q] = 2.0 * r[ir]
+        fd[:, q] = (obj[0, :, q] - obj[1, :, q]) / dp
+    for _, x in built:
+        x.shutdown()
+    return fd, obj
+
+
 def study(points=16, seed=4, nr=32, nz=320, end_time=taylor.ROD_DURATION, max_steps=20000, rel=1e-6):
-    """The derivative against central differences on seeded parameter points drawn as the
-    ensemble draws them (mesh fixed at nr x nz), with the objectives' extreme nodes held at
-    the unperturbed run's. For each point: the worst and median relative disagreement over the
-    twelve derivatives, the cost ratio, and the asymmetry of the one-sided differences of the
-    nodal positions, max|(u+ - u0) + (u- - u0)| / max|u+ - u-| over the nodes: of order h on a
-    smooth branch, of order one where a perturbed run crossed a discrete event (a node landing on the anvil a step
-    earlier or later zeroes its velocity, a jump of the discrete map), across which a central
-    difference is not a derivative."""
+    """The discrete adjoint against central differences on seeded parameter points drawn as
+    the ensemble draws them (mesh fixed at nr x nz), the differences at the adjoint run's step
+    count with the objectives' extreme nodes held at the unperturbed run's. For each point and
+    each of the twelve derivatives: the relative disagreement and the asymmetry of the two
+    one-sided differences of the objective, |J+ + J- - 2 J0| / |J+ - J-|, which is of order h
+    where the map is smooth on the scale of the step and of order one where a perturbed run
+    crossed a discrete event (a node landing, leaving the anvil or switching between stick and
+    slip a step earlier or later; a point yielding or not); and the difference's resolution,
+    |FD(h) - FD(h/2)| / |FD(h)|, how far the central difference itself moves when its step is
+    halved (of order h^2 on a smooth map, the map's roughness at that scale otherwise)."""
     from . import ensemble
     rows = []
     for k, inst in enumerate(ensemble.draw(points, seed=seed)):
         p0 = from_instance(inst)
         pt = Point(nr, nz, p0.speed, p0.friction, p0.a, p0.b, p0.n, p0.c, p0.m)
         mesh, s = build(pt)
-        s.enableSensitivity()
+        s.enableAdjoint(64)
         t0 = time.perf_counter()
         s.runUntil(end_time, max_steps)
-        sec = time.perf_counter() - t0
-        _, (ih, il, ir) = objectives(mesh, s)
-        tang = np.array([[(s.nodeSensitivity("uz", q)[ih] - s.nodeSensitivity("uz", q)[il]) / taylor.ROD_LENGTH
-                          for q in range(len(PARAMETERS))],
-                         [2.0 * s.nodeSensitivity("ur", q)[ir] for q in range(len(PARAMETERS))]])
-        z0 = mesh.z + s.axialDisplacement()
-        r0 = mesh.r + s.radialDisplacement()
+        rec = time.perf_counter() - t0
+        obj0, ext = objectives(mesh, s)
+        t0 = time.perf_counter()
+        grad = s.adjointGradient(objective_seeds(mesh, ext))
+        rev = time.perf_counter() - t0
         steps = s.steps()
         s.shutdown()
         _, _, fsec = forward(pt, end_time, max_steps)
-        pts = []
-        for q, v in enumerate(pt.values()):
-            h = rel * abs(v) if v != 0.0 else rel
-            pts += [pt.with_value(q, v + h), pt.with_value(q, v - h)]
-        built = [build(p) for p in pts]
-        batch = Batch([x for _, x in built])
-        batch.runUntil(end_time, max_steps)
-        batch.scatter()
-        batch.close()
-        fd = np.zeros_like(tang)
-        asym = 0.0
-        for q in range(len(PARAMETERS)):
-            (mp, sp), (mm, sm) = built[2 * q], built[2 * q + 1]
-            dp = pts[2 * q].values()[q] - pts[2 * q + 1].values()[q]
-            zp, zm = mesh.z + sp.axialDisplacement(), mesh.z + sm.axialDisplacement()
-            rp, rm = mesh.r + sp.radialDisplacement(), mesh.r + sm.radialDisplacement()
-            fd[0, q] = ((zp[ih] - zp[il]) - (zm[ih] - zm[il])) / taylor.ROD_LENGTH / dp
-            fd[1, q] = 2.0 * (rp[ir] - rm[ir]) / dp
-            # Symmetry of the two one-sided differences over every node: second order in h
-            # on a smooth branch, of the order of the difference itself across a jump.
-            for up, um, u0 in ((zp, zm, z0), (rp, rm, r0)):
-                spread = float(np.abs(up - um).max())
-                if spread > 0.0:
-                    asym = max(asym, float(np.abs((up - u0) + (um - u0)).max()) / spread)
-        for _, x in built:
-            x.shutdown()
-        err = np.abs(tang - fd) / np.maximum(np.abs(fd), 1e-300)
+        j0 = np.array(obj0)[:, None]
+        diffs = []
+        for h in (rel, 0.5 * rel):
+            fd, obj = central_differences(pt, steps, ext, h)
+            spread = np.abs(obj[0] - obj[1])
+            diffs.append((fd, np.abs(obj[0] + obj[1] - 2.0 * j0) / np.maximum(spread, 1e-300)))
+        (fd, asym), (fd2, asym2) = diffs
+        err = np.abs(grad - fd) / np.maximum(np.abs(fd), 1e-300)
+        # the difference's own resolution: how far it moves when the step is halved
+        resolution = np.abs(fd - fd2) / np.maximum(np.abs(fd), 1e-300)
         rows.append({"point": k, "speed": pt.speed, "friction": pt.friction, "steps": steps,
-                     "max_rel": float(err.max()), "median_rel": float(np.median(err)),
-                     "asymmetry": asym, "cost_ratio": sec / fsec})
+                     "grad": grad.tolist(), "fd": fd.tolist(), "fd_half_step": fd2.tolist(),
+                     "rel_err": err.tolist(), "resolution": resolution.tolist(),
+                     "asymmetry": np.maximum(asym, asym2).tolist(),
+                     "cost_ratio": (rec + rev) / fsec})
     return rows
 
 
