@@ -7,7 +7,10 @@
  *   int32 N, E, A, hasJC, steps
  *   double r[N], z[N]; int32 conn[4E]; int32 axis[A]
  *   double material[6] (E, nu, rho, yield, Hiso, Hkin), jc[10], speed, friction
- * OUT: double ur[N], uz[N], vr[N], vz[N], then the solver time (nf_state.time).
+ * OUT: double ur[N], uz[N], vr[N], vz[N], then the solver time (nf_state.time), then the
+ * queries temperature(0), plasticFrontRadius(), mostDamagedElement(), borePressure(),
+ * meshZ(N - 1), the kinetic energy audit, and the discrete adjoint's gradient (6 values) of
+ * the final axial displacement of node N - 1 (the run records checkpoints every 50 steps).
  *
  * The setup is the Taylor harness's (validate/TaylorImpactCase.java:225-249): axisymmetric,
  * reduced integration, finite strain, thickness 1, CFL 0.5, axis nodes pinned radially, a
@@ -59,6 +62,7 @@ int main(int argc, char** argv) {
         if ((rc = nf_fix_radial(s, axis[k]))) return fail("nf_fix_radial", rc);
     if ((rc = nf_set_rigid_wall(s, 1, 0.0, friction))) return fail("nf_set_rigid_wall", rc);
     if ((rc = nf_set_uniform_velocity(s, 0.0, -speed))) return fail("nf_set_uniform_velocity", rc);
+    if ((rc = nf_adjoint_enable(s, 50))) return fail("nf_adjoint_enable", rc);
     if ((rc = nf_run(s, steps))) return fail("nf_run", rc);
 
     double* buf = malloc(sizeof(double) * N);
@@ -72,6 +76,25 @@ int main(int argc, char** argv) {
     nf_state st;
     if ((rc = nf_get_state(s, &st))) return fail("nf_get_state", rc);
     if (fwrite(&st.time, sizeof(double), 1, out) != 1) return fail("write output", -1);
+    double q[6];
+    int32_t worst = 0;
+    nf_audits au;
+    if ((rc = nf_temperature(s, 0, &q[0]))) return fail("nf_temperature", rc);
+    if ((rc = nf_plastic_front_radius(s, &q[1]))) return fail("nf_plastic_front_radius", rc);
+    if ((rc = nf_most_damaged_element(s, &worst))) return fail("nf_most_damaged_element", rc);
+    q[2] = (double)worst;
+    if ((rc = nf_bore_pressure(s, &q[3]))) return fail("nf_bore_pressure", rc);
+    if ((rc = nf_mesh_z(s, N - 1, &q[4]))) return fail("nf_mesh_z", rc);
+    if ((rc = nf_get_audits(s, &au))) return fail("nf_get_audits", rc);
+    q[5] = au.kinetic_energy;
+    if (fwrite(q, sizeof(double), 6, out) != 6) return fail("write output", -1);
+    double* seed = calloc(4 * (size_t)N, sizeof(double));
+    double grad[6];
+    if (!seed) return fail("allocate", -1);
+    seed[N + (N - 1)] = 1.0;  /* d / d uz of the last node */
+    if ((rc = nf_adjoint_gradient(s, 1, seed, grad))) return fail("nf_adjoint_gradient", rc);
+    if (fwrite(grad, sizeof(double), 6, out) != 6) return fail("write output", -1);
+    free(seed);
     fclose(out);
     if ((rc = nf_solver_destroy(s))) return fail("nf_solver_destroy", rc);
     free(r);

@@ -1,7 +1,8 @@
 """The library driven from plain C (tests/c_client/step_loop.c, built by
 __graft_entry__.build()): the 16 x 160 Johnson-Cook rod set up and stepped through
 include/neofiz.h alone ends bit for bit where the same run through the Python host mirror
-ends. The C client stands in for the Java FFM binding, which needs a JDK this image lacks."""
+ends, and its queries and its discrete-adjoint gradient are the mirror's. The C client
+stands in for the Java FFM binding, which needs a JDK this image lacks."""
 import os
 import struct
 import subprocess
@@ -24,9 +25,15 @@ def test_c_client_matches_the_python_mirror(tmp_path):
         pytest.fail(f"{CLIENT} is not built; run __graft_entry__.build() first")
     mesh, solver, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER, taylor.ROD_SPEED,
                                    NR, NZ, friction=taylor.ROD_JC_FRICTION, solver_cls=ExplicitSolver)
+    solver.enableAdjoint(50)
     solver.run(STEPS)
     want = [solver.nodeField(k) for k in ("ur", "uz", "vr", "vz")]
     want_time = solver.time()
+    want_q = [solver.temperature(0), solver.plasticFrontRadius(), float(solver.mostDamagedElement()),
+              solver.borePressure(), solver.meshZ(mesh.nodeCount - 1), solver.kineticEnergy()]
+    seed = np.zeros((1, 4, mesh.nodeCount))
+    seed[0, 1, mesh.nodeCount - 1] = 1.0
+    want_grad = solver.adjointGradient(seed)[0]
     solver.shutdown()
 
     m = taylor.ROD_JC.to_c()
@@ -49,7 +56,10 @@ def test_c_client_matches_the_python_mirror(tmp_path):
     assert done.returncode == 0, done.stderr
     got = np.fromfile(path_out, dtype="<f8")
     n = mesh.nodeCount
-    assert got.size == 4 * n + 1
+    assert got.size == 4 * n + 1 + 6 + 6
     for k in range(4):
         assert np.array_equal(got[k * n:(k + 1) * n].view(np.uint64), want[k].view(np.uint64)), k
-    assert got[-1] == want_time
+    assert got[4 * n] == want_time
+    assert list(got[4 * n + 1:4 * n + 7]) == want_q
+    assert np.array_equal(got[4 * n + 7:].view(np.uint64), want_grad.view(np.uint64))
+    assert want_q[1] > 0.0 and want_q[2] == -1.0 and np.all(want_grad != 0.0)
