@@ -70,10 +70,13 @@ __device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t, const n
 // instantiated for it so that no iteration evaluates a division whose result is discarded.
 // REC (the adjoint's recording pass only): *status becomes 2 when the solve stops on its
 // iteration cap rather than its tolerance. The plain instantiation has no such code.
+// `iterations` (optional) is increased by the number of iterations the solve ran; the element
+// pass schedules its waves by it.
 template <bool UNIT_RATE, bool REC = false>
 __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, double plasticStrain,
                                                 double thermal, double dt, const Mat& j,
-                                                const nfm::Tables& T, int* status = nullptr) {
+                                                const nfm::Tables& T, int* status = nullptr,
+                                                int* iterations = nullptr) {
     const double tolerance = 1.0e-10 * j.a;
     double lo = 0.0;
     double hi = norm / stiffness;
@@ -93,22 +96,27 @@ __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, d
         const double g = norm - stiffness * dGamma - NFD_SQRT_2_3 * strain * rateFactor * thermal;
         lo = g > 0.0 ? dGamma : lo;
         hi = g > 0.0 ? hi : dGamma;
-        if (jabs(g) <= tolerance) return dGamma;
+        if (jabs(g) <= tolerance) {
+            if (iterations) *iterations += iteration + 1;
+            return dGamma;
+        }
         const double slope0 = -stiffness - NFD_TWO_THIRDS * (j.n * (strain - j.a) / eps) * rateFactor * thermal;
         const double slope = clamped ? slope0 : slope0 - NFD_SQRT_2_3 * strain * j.c * thermal / dGamma;
         const double next = dGamma - g / slope;
         dGamma = (next > lo && next < hi) ? next : 0.5 * (lo + hi);
     }
     if (REC) *status = 2;
+    if (iterations) *iterations += 25;
     return dGamma;
 }
 
 template <bool REC = false>
 __device__ __forceinline__ double j2_solve(double norm, double stiffness, double plasticStrain,
                                            double thermal, double dt, const Mat& j,
-                                           const nfm::Tables& T, int* status = nullptr) {
-    return j.eps0 == 1.0 ? j2_solve_loop<true, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status)
-                         : j2_solve_loop<false, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status);
+                                           const nfm::Tables& T, int* status = nullptr,
+                                           int* iterations = nullptr) {
+    return j.eps0 == 1.0 ? j2_solve_loop<true, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status, iterations)
+                         : j2_solve_loop<false, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status, iterations);
 }
 
 // Damage state of one Gauss point (solver/Damage.java:156-192).
@@ -170,11 +178,12 @@ struct PointState {
 // stored, and subtracting it or adding alpha : n changes no bit of any result.
 // REC (the adjoint's recording pass only): *status receives 0 for an elastic step, 1 for a
 // plastic one whose multiplier met the solve's tolerance, 2 where the solve hit its cap.
+// `iterations` (optional) accumulates the Johnson-Cook solve's iteration count.
 template <bool DAMAGE, bool KIN, bool REC = false>
 __device__ __forceinline__ double j2_update(PointState& q, double dRR, double dZZ, double dTT,
                                             double dGamRZ, const Mat& fl, double strainDt,
                                             DamagePoint* dmg, const nfm::Tables& T,
-                                            int* status = nullptr) {
+                                            int* status = nullptr, int* iterations = nullptr) {
     const double lambda = fl.lambda, mu = fl.mu;
     const double lt = lambda * (dRR + dZZ + dTT);
     const double tRR = q.s0 + lt + 2.0 * mu * dRR;
@@ -230,7 +239,7 @@ __device__ __forceinline__ double j2_update(PointState& q, double dRR, double dZ
             dGamma = norm / stiffness;
         }
     } else {
-        dGamma = j2_solve<REC>(norm, stiffness, q.epsP, thermal, strainDt, fl, T, status);
+        dGamma = j2_solve<REC>(norm, stiffness, q.epsP, thermal, strainDt, fl, T, status, iterations);
     }
     const double inv = 1.0 / norm;
     const double nRR = xRR * inv, nZZ = xZZ * inv, nTT = xTT * inv, nRZ = xRZ * inv;

@@ -31,6 +31,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -106,6 +107,13 @@ struct Dev {
     const double* mass;
     uint8_t* flags;
     const int4* conn;
+    // The element pass's schedule (null order: thread t takes element t). Elements are grouped
+    // into patches of 64 neighbours, one wave each; wave slot s takes patch patchOf[s] (-1: no
+    // patch), lane l element order[64 p + l] (-1 past the last element).
+    const int* order;     // [64 patches]
+    int* patchOf;         // [4 patchBlocks]
+    uint8_t* patchCost;   // [patches] Newton iterations of the patch's slowest lane, last step
+    int patches, patchBlocks, units;  // units: the device's compute units
     const int* matIndex;  // null when uniform
     const Mat* mats;
     Mat mat0;             // the uniform material, by value (kernel arguments sit in SGPRs)
@@ -309,9 +317,77 @@ constexpr int PART_ALL = 0, PART_GATHER = 1, PART_REST = 2;
 //   PHASE_START  the first nodal update of a run call (gathering first after the initial
 //                zero-interval assembly when gatherFirst is set);
 //   PHASE_ASSEMBLE_END  gather only, after advance().
+// Wave slot of the patch ranked r (0 = costliest) in a grid of patchBlocks workgroups of
+// four waves: ranks fill workgroups four at a time, and the workgroups are dealt to the compute
+// units in rounds that alternate direction, so that a unit holding one of the costliest
+// workgroups of the first round holds one of the cheapest of the second.
+__host__ __device__ __forceinline__ int patch_slot(int r, int patchBlocks, int units) {
+    const int b = r >> 2, round = b / units, k = b - round * units;
+    const int inRound = patchBlocks - round * units < units ? patchBlocks - round * units : units;
+    const int pos = round * units + ((round & 1) ? inRound - 1 - k : k);
+    return 4 * pos + (r & 3);
+}
+
+// The element pass's schedule for its next launch, from each patch's cost in the last one:
+// a counting sort of the patches by cost, costliest first, into patch_slot order. One
+// workgroup does it, so the cost array it reads is the one the finished element launch wrote.
+// Patches of equal cost are ranked in the order the LDS atomics give: the schedule decides
+// only which wave takes a patch, never a result.
+__device__ void schedule_patches(const Dev& d) {
+    constexpr int CHUNK = 4096;  // costs staged in LDS per pass, read 16 at a time
+    __shared__ int start[256];
+    __shared__ uint4 staged[CHUNK / 16];
+    const uint8_t* cost = (const uint8_t*)staged;
+    const int c = threadIdx.x;  // BLOCK = 256 cost bins
+    const int P = d.patches;
+    const bool whole = P <= CHUNK;  // then one staging serves both passes
+    auto stage = [&](int from) {
+        const int n = min(CHUNK, P - from);
+        __syncthreads();
+        for (int k = threadIdx.x; k < (n + 15) / 16; k += BLOCK) {
+            if (16 * k + 16 <= n && ((from & 15) == 0)) {
+                staged[k] = *(const uint4*)(d.patchCost + from + 16 * k);
+            } else {
+                for (int j = 0; j < 16; j++)
+                    ((uint8_t*)staged)[16 * k + j] = 16 * k + j < n ? d.patchCost[from + 16 * k + j] : 0;
+            }
+        }
+        __syncthreads();
+        return n;
+    };
+    start[c] = 0;
+    for (int from = 0; from < P; from += CHUNK) {
+        const int n = stage(from);
+        for (int p = threadIdx.x; p < n; p += BLOCK) atomicAdd(&start[cost[p]], 1);
+    }
+    __syncthreads();
+    // suffix sums over the bins (Hillis-Steele), then the exclusive start of each bin
+    const int count = start[c];
+    for (int off = 1; off < 256; off <<= 1) {
+        const int add = c + off < 256 ? start[c + off] : 0;
+        __syncthreads();
+        start[c] += add;
+        __syncthreads();
+    }
+    const int first = start[c] - count;
+    __syncthreads();
+    start[c] = first;
+    for (int from = 0; from < P; from += CHUNK) {
+        const int n = whole ? (__syncthreads(), P) : stage(from);
+        for (int p = threadIdx.x; p < n; p += BLOCK) {
+            const int r = atomicAdd(&start[cost[p]], 1);
+            d.patchOf[patch_slot(r, d.patchBlocks, d.units)] = from + p;
+        }
+    }
+}
+
 // Reads control buffer q; block 0 writes buffer q ^ 1.
 template <bool WALL>
 __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part) {
+    if (d.order && !d.gone && part != PART_REST && blockIdx.x == gridDim.x - 1) {
+        schedule_patches(d);  // the workgroup past the others (launchGNT)
+        return;
+    }
     Ctl* c = d.ctl;
     const Step& in = c->s[q];
     Step& out = c->s[q ^ 1];
@@ -570,7 +646,7 @@ __device__ __forceinline__ void write_zero_slots(const Dev& d, int e) {
 template <bool DAMAGE, bool KIN, bool REC = false>
 __device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, double strainDt,
                                        const nfm::Tables& T, double* dGammaOut = nullptr,
-                                       int* status = nullptr) {
+                                       int* status = nullptr, int* iterations = nullptr) {
     const Mat fl = load_mat(d, e);
     const double mu = fl.mu;
     const double mid = -0.5 * strainDt;
@@ -628,7 +704,7 @@ __device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, d
     }
     nfd::DamagePoint dp;
     if (DAMAGE) load_damage(d, e, dp);
-    const double dGamma = nfd::j2_update<DAMAGE, KIN, REC>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp, T, status);
+    const double dGamma = nfd::j2_update<DAMAGE, KIN, REC>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp, T, status, iterations);
     if (REC) *dGammaOut = dGamma;
     store_point<KIN>(d, e, q);
     if (DAMAGE) {
@@ -684,7 +760,7 @@ __device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, d
 // One element under full 2x2 integration, solver/ExplicitSolver.java:1219-1339.
 template <bool DAMAGE, bool KIN>
 __device__ __forceinline__ double full_element(const Dev& d, int e, int4 n, double strainDt,
-                                       const nfm::Tables& T) {
+                                       const nfm::Tables& T, int* iterations = nullptr) {
     const Mat fl = load_mat(d, e);
     const double mid = -0.5 * strainDt;
     const double2 p0 = d.x0[n.x], p1 = d.x0[n.y], p2 = d.x0[n.z], p3 = d.x0[n.w];
@@ -753,7 +829,7 @@ __device__ __forceinline__ double full_element(const Dev& d, int e, int4 n, doub
         }
         nfd::DamagePoint dp;
         if (DAMAGE) load_damage(d, p, dp);
-        nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp, T);
+        nfd::j2_update<DAMAGE, KIN>(q, dEpsR, dEpsZ, dEpsT, dGamRZ, fl, strainDt, &dp, T, nullptr, iterations);
         store_point<KIN>(d, p, q);
         if (DAMAGE) {
             d.dStrength[p] = dp.strength;
@@ -834,13 +910,29 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
     __shared__ double sExp[64][2];
     const nfm::Tables T = nfd::stage_tables(sLog, sExp);
     const double strainDt = mode == MODE_INITIAL ? 0.0 : st.dt;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    // Each element's update reads and writes only that element's own state, so which thread
+    // takes which element changes no result. With a schedule, the first 256 patchBlocks
+    // threads take patches and the pressure edges follow; erosion's per-block energy sums
+    // are grouped by thread, so with erosion on the threads keep index order.
+    const bool scheduled = d.order && !d.gone;
+    int e = t, patch = -1, edgeIndex = t - d.E;
+    if (scheduled) {
+        const int elementThreads = BLOCK * d.patchBlocks;
+        edgeIndex = t - elementThreads;
+        e = -1;
+        if (t < elementThreads) {
+            patch = d.patchOf[t >> 6];
+            if (patch >= 0) e = d.order[64 * patch + (t & 63)];
+        }
+    }
     double edge = NFD_MAX_VALUE;
     bool sawNaN = false;
     double eEnergy = 0.0;
     int eCount = 0, iCount = 0;
     bool stale = false;
-    if (e < d.E) {
+    int iterations = 0;
+    if (e >= 0 && e < d.E) {
         const int4 n = d.conn[e];
         bool gone = false;
         if (d.gone) {
@@ -849,11 +941,21 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
         if (gone) {
             write_zero_slots(d, e);
         } else {
-            edge = FULL ? full_element<DAMAGE, KIN>(d, e, n, strainDt, T) : reduced_element<DAMAGE, KIN>(d, e, n, strainDt, T);
+            edge = FULL ? full_element<DAMAGE, KIN>(d, e, n, strainDt, T, &iterations)
+                        : reduced_element<DAMAGE, KIN>(d, e, n, strainDt, T, nullptr, nullptr, &iterations);
             sawNaN = edge != edge;
         }
-    } else if (e < d.E + d.edges) {
-        pressure_edge(d, e - d.E, pressureAt(d, st.time));
+    } else if (edgeIndex >= 0 && edgeIndex < d.edges) {
+        pressure_edge(d, edgeIndex, pressureAt(d, st.time));
+    }
+    if (patch >= 0) {
+        // The patch's cost for the next schedule: its slowest lane's Newton iterations.
+        int m = iterations;
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(m, off, 64);
+            m = o > m ? o : m;
+        }
+        if ((threadIdx.x & 63) == 0) d.patchCost[patch] = (uint8_t)(m < 255 ? m : 255);
     }
 
     // Shortest edge: exact minimum over non-negative doubles by their bit patterns.
@@ -1108,6 +1210,65 @@ __global__ void k_math_eval(int fn, const double* x, const double* y, double* ou
 }
 
 inline int blocks_for(int n) { return (n + BLOCK - 1) / BLOCK; }
+
+#ifndef NF_ELEMENT_ORDER
+#define NF_ELEMENT_ORDER 2
+#endif
+
+// The element pass's thread order. A wave runs as long as its slowest lane, and in a plastic
+// zone the Johnson-Cook Newton solve's iteration count varies smoothly in space, so a wave of
+// 64 neighbouring elements in a compact patch wastes fewer lane-iterations than 64 elements in
+// a line. Elements are sorted by the cell of their centroid in a square grid of side h (the
+// mesh's shortest edge, so a lattice puts one element in each cell): 8 x 8 cells to a wave,
+// four such patches side by side (8 x 32 cells) to a workgroup, workgroups row by row. Ties
+// (several centroids in one cell of an irregular mesh) keep index order.
+static std::vector<int> element_order(int E, const int* conn, const double* r, const double* z, double h) {
+    std::vector<int> order(E);
+    for (int e = 0; e < E; e++) order[e] = e;
+    if (NF_ELEMENT_ORDER == 0 || E == 0 || !(h > 0.0)) return order;
+    std::vector<double> rc(E), zc(E);
+    double rmin = HUGE_VAL, zmin = HUGE_VAL, rmax = -HUGE_VAL;
+    for (int e = 0; e < E; e++) {
+        const int* c = conn + 4 * (size_t)e;
+        rc[e] = 0.25 * (r[c[0]] + r[c[1]] + r[c[2]] + r[c[3]]);
+        zc[e] = 0.25 * (z[c[0]] + z[c[1]] + z[c[2]] + z[c[3]]);
+        rmin = std::min(rmin, rc[e]);
+        rmax = std::max(rmax, rc[e]);
+        zmin = std::min(zmin, zc[e]);
+    }
+    auto cell = [&](double x) { return (int64_t)std::min(std::floor(x / h), 2147483647.0); };
+    const int64_t cols = cell(rmax - rmin) + 1;
+    std::vector<uint64_t> key(E);
+    for (int e = 0; e < E; e++) {
+        const int64_t I = cell(rc[e] - rmin), J = cell(zc[e] - zmin);
+        uint64_t k;
+        if (NF_ELEMENT_ORDER == 1) {  // Morton (Z-order) of the cell
+            k = 0;
+            for (int b = 0; b < 32; b++) k |= (uint64_t)((I >> b) & 1) << (2 * b) | (uint64_t)((J >> b) & 1) << (2 * b + 1);
+        } else if (NF_ELEMENT_ORDER == 3) {  // 16 x 16 workgroups of 2 x 2 patches
+            const int64_t bc = (cols + 15) >> 4;
+            k = ((((uint64_t)(J >> 4) * bc + (I >> 4)) * 4 + ((J >> 3) & 1) * 2 + ((I >> 3) & 1)) * 64) + (J & 7) * 8 + (I & 7);
+        } else {  // 8 x 32 workgroups of four patches side by side
+            const int64_t bc = (cols + 31) >> 5;
+            k = ((((uint64_t)(J >> 3) * bc + (I >> 5)) * 4 + ((I >> 3) & 3)) * 64) + (J & 7) * 8 + (I & 7);
+        }
+        key[e] = k;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    if (NF_ELEMENT_ORDER == 4) {  // the 8 x 32 workgroups, dealt out with a stride
+        const int nb = blocks_for(E);
+        std::vector<int> tmp(order);
+        int stride = 97;
+        while (std::gcd(stride, nb) != 1) stride++;
+        int w = 0;
+        for (int k = 0; k < nb; k++) {
+            const int src = (int)(((int64_t)k * stride) % nb);
+            for (int t = src * BLOCK; t < std::min(E, (src + 1) * BLOCK); t++) tmp[w++] = order[t];
+        }
+        order = tmp;
+    }
+    return order;
+}
 // Workgroups of a launch that gives each thread `chunk` items.
 inline int blocks_for_chunks(int n, int chunk) { return std::max(1, (n + BLOCK * chunk - 1) / (BLOCK * chunk)); }
 
@@ -1347,7 +1508,8 @@ struct nf_solver {
     // around it.
     template <bool FULL, bool DAMAGE, bool KIN>
     int launchElementT(int mode, int parity, hipEvent_t a, hipEvent_t b) {
-        const int grid = blocks_for(E + edges);
+        // scheduled: patch workgroups, then the pressure edges (k_element)
+        const int grid = (d.order && !d.gone) ? d.patchBlocks + blocks_for(edges) : blocks_for(E + edges);
         if (a) hipExtLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, mode);
         else hipLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
         HIP_TRY(hipGetLastError());
@@ -1370,7 +1532,9 @@ struct nf_solver {
     template <bool WALL>
     int launchGNT(int phase, int parity, int gatherFirst, int part, hipEvent_t a, hipEvent_t b) {
         // the gather before the contact launches carries the contact's segment rebuild
-        const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0);
+        // one workgroup more for the element pass's schedule (k_gn)
+        const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0) +
+                         (d.order && !d.gone && part != PART_REST ? 1 : 0);
         if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part);
         else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part);
         HIP_TRY(hipGetLastError());
@@ -2027,6 +2191,34 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         h.s[k].minEdgeBits = MAX_BITS;
     }
     s->dt0 = h.s[0].dt;
+    if (E > 0) {
+        // The element pass's patches (64 neighbours each, in the locality order) and its
+        // first schedule, patch s in wave slot s.
+        const std::vector<int> ord = element_order(E, conn, r, z, minEdge);
+        const int P = (E + 63) / 64, nb = (P + 3) / 4;
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+            return bail(fail(NF_E_DEVICE, "device query failed"));
+        const int units = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
+        std::vector<int> padded((size_t)64 * P, -1), first((size_t)4 * nb, -1);
+        std::copy(ord.begin(), ord.end(), padded.begin());
+        for (int k = 0; k < P; k++) first[patch_slot(k, nb, units)] = k;
+        int* o = nullptr;
+        int* po = nullptr;
+        uint8_t* pc = nullptr;
+        if ((rc = s->alloc(&o, padded.size())) || (rc = s->alloc(&po, first.size())) || (rc = s->alloc(&pc, P)))
+            return bail(rc);
+        if (hipMemcpy(o, padded.data(), sizeof(int) * padded.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(po, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(NF_E_DEVICE, "upload failed"));
+        s->d.order = o;
+        s->d.patchOf = po;
+        s->d.patchCost = pc;
+        s->d.patches = P;
+        s->d.patchBlocks = nb;
+        s->d.units = units;
+    }
     s->d.nodalBlocks = blocks_for(N);
     s->d.elementBlocks = blocks_for(E + pressure_edge_count);
     if ((rc = s->writeCtl(h))) return bail(rc);
