@@ -432,7 +432,9 @@ int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_
 /* Device self-check: evaluates the GPU's log (0), exp (1), pow (2), atan2 (3), sin (4),
  * cos (5), hypot (6), sqrt (7), division (8), log1p (9), or the Newton loop's pow (10) and
  * log (11) with special inputs out of line, and the derivative pass's pow (12) with the log
- * it shares (13) (the same results as 2, 0, 2 and 0) on n inputs. These are the
+ * it shares (13) (the same results as 2, 0, 2 and 0), the Newton loop's pow(x, y) (14) and
+ * log(y) (15) evaluated side by side, and the yield check's pow(x, y) (16) and pow(y, x)
+ * (17) evaluated side by side (the same results as pow and log) on n inputs. These are the
  * functions the kernels use in place of java.lang.Math's (core/JohnsonCook.java:139, :182,
  * :196; solver/Damage.java:315-322; core/Weibull.java:118, :130; core/DefectField.java:331),
  * evaluated where the kernels run. */

@@ -62,6 +62,18 @@ __device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t, const n
     if (homologous >= 1.0) return 0.0;
     return 1.0 - nfm::pow_ordinary(homologous, j.m, T);
 }
+// strainTerm(e) and thermalTerm(t) together, their two pows side by side
+// (nfm::pow_pair_ordinary). A term that takes no pow gives its pow a stand-in input of 0.5,
+// whose result is not used; each result is the single-term function's own.
+__device__ __forceinline__ void jc_terms(const Mat& j, double e, double t, const nfm::Tables& T,
+                                         double& strain, double& thermal) {
+    const double homologous = (t - j.troom) / (j.tmelt - j.troom);
+    const bool strainPow = !(e <= 0.0), thermalPow = !(homologous <= 0.0) && !(homologous >= 1.0);
+    double ps, pt;
+    nfm::pow_pair_ordinary(strainPow ? e : 0.5, j.n, thermalPow ? homologous : 0.5, j.m, T, ps, pt);
+    strain = e <= 0.0 ? j.a : j.a + j.b * ps;
+    thermal = homologous <= 0.0 ? 1.0 : (homologous >= 1.0 ? 0.0 : 1.0 - pt);
+}
 
 // solver/J2.java:332-375, the bracketed Newton solve for the Johnson-Cook multiplier.
 // `thermal` is thermalTerm(temperature), which the caller has already evaluated for the
@@ -88,10 +100,10 @@ __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, d
         const double dEps = NFD_SQRT_2_3 * dGamma;
         const double eps = plasticStrain + dEps;
         const double rate = dt > 0.0 ? dEps / dt : 0.0;
-        const double pw = nfm::pow_ordinary(eps, j.n, T);
+        double pw, lg;
+        nfm::pow_and_log_ordinary(eps, j.n, UNIT_RATE ? rate : rate / j.eps0, T, pw, lg);
         const double strain = eps <= 0.0 ? j.a : j.a + j.b * pw;  // strainTerm
         const bool clamped = rate <= j.eps0;
-        const double lg = nfm::log_ordinary(UNIT_RATE ? rate : rate / j.eps0, T);
         const double rateFactor = clamped ? 1.0 : 1.0 + j.c * lg;  // rateTerm
         const double g = norm - stiffness * dGamma - NFD_SQRT_2_3 * strain * rateFactor * thermal;
         lo = g > 0.0 ? dGamma : lo;
@@ -216,8 +228,9 @@ __device__ __forceinline__ double j2_update(PointState& q, double dRR, double dZ
         flowStress = fl.sy0 + fl.hIso * q.epsP;
         hardening = fl.hIso;
     } else {
-        thermal = jc_thermalTerm(fl, temperature, T);
-        flowStress = jc_strainTerm(fl, q.epsP, T) * thermal;  // quasiStaticFlowStress
+        double strain;
+        jc_terms(fl, q.epsP, temperature, T, strain, thermal);
+        flowStress = strain * thermal;  // quasiStaticFlowStress
     }
     const double radius = NFD_SQRT_2_3 * flowStress;
 

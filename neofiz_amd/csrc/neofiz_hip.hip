@@ -111,6 +111,7 @@ struct Dev {
     // into patches of 64 neighbours, one wave each; wave slot s takes patch patchOf[s] (-1: no
     // patch), lane l element order[64 p + l] (-1 past the last element).
     const int* order;     // [64 patches]
+    const int4* orderConn;  // [64 patches] the corners of order[k]
     int* patchOf;         // [4 patchBlocks]
     uint8_t* patchCost;   // [patches] Newton iterations of the patch's slowest lane, last step
     int patches, patchBlocks, units;  // units: the device's compute units
@@ -908,24 +909,31 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
     if (mode == MODE_STEP && !st.go) return;
     __shared__ double sLog[97][3];
     __shared__ double sExp[64][2];
-    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
-    const double strainDt = mode == MODE_INITIAL ? 0.0 : st.dt;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     // Each element's update reads and writes only that element's own state, so which thread
     // takes which element changes no result. With a schedule, the first 256 patchBlocks
     // threads take patches and the pressure edges follow; erosion's per-block energy sums
-    // are grouped by thread, so with erosion on the threads keep index order.
+    // are grouped by thread, so with erosion on the threads keep index order. The element and
+    // its corners are read before the table staging's barrier, so that the two overlap.
     const bool scheduled = d.order && !d.gone;
     int e = t, patch = -1, edgeIndex = t - d.E;
+    int4 n = make_int4(0, 0, 0, 0);
     if (scheduled) {
         const int elementThreads = BLOCK * d.patchBlocks;
         edgeIndex = t - elementThreads;
         e = -1;
         if (t < elementThreads) {
-            patch = d.patchOf[t >> 6];
-            if (patch >= 0) e = d.order[64 * patch + (t & 63)];
+            patch = d.patchOf[__builtin_amdgcn_readfirstlane(t >> 6)];  // one scalar load a wave
+            if (patch >= 0) {
+                e = d.order[64 * patch + (t & 63)];
+                n = d.orderConn[64 * patch + (t & 63)];
+            }
         }
+    } else if (t < d.E) {
+        n = d.conn[t];
     }
+    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
+    const double strainDt = mode == MODE_INITIAL ? 0.0 : st.dt;
     double edge = NFD_MAX_VALUE;
     bool sawNaN = false;
     double eEnergy = 0.0;
@@ -933,7 +941,6 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
     bool stale = false;
     int iterations = 0;
     if (e >= 0 && e < d.E) {
-        const int4 n = d.conn[e];
         bool gone = false;
         if (d.gone) {
             gone = mode == MODE_STEP ? erode_element(d, e, n, eEnergy, eCount, iCount, stale) : d.gone[e] != 0;
@@ -1202,6 +1209,20 @@ __global__ void k_math_eval(int fn, const double* x, const double* y, double* ou
             double lx;
             const double pw = nfm::pow_log_ordinary(x[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, lx);
             r = fn == 12 ? pw : lx;
+            break;
+        }
+        case 14:
+        case 15: {  // pow_ordinary(x, y) and log_ordinary(y) side by side
+            double pw, lg;
+            nfm::pow_and_log_ordinary(x[i], y[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, pw, lg);
+            r = fn == 14 ? pw : lg;
+            break;
+        }
+        case 16:
+        case 17: {  // pow_ordinary(x, y) and pow_ordinary(y, x) side by side
+            double p1, p2;
+            nfm::pow_pair_ordinary(x[i], y[i], y[i], x[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, p1, p2);
+            r = fn == 16 ? p1 : p2;
             break;
         }
         default: r = 0.0;
@@ -2204,15 +2225,24 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         std::vector<int> padded((size_t)64 * P, -1), first((size_t)4 * nb, -1);
         std::copy(ord.begin(), ord.end(), padded.begin());
         for (int k = 0; k < P; k++) first[patch_slot(k, nb, units)] = k;
+        std::vector<int4> corners(padded.size(), make_int4(0, 0, 0, 0));
+        for (size_t k = 0; k < ord.size(); k++) {
+            const int* c = conn + 4 * (size_t)ord[k];
+            corners[k] = make_int4(c[0], c[1], c[2], c[3]);
+        }
         int* o = nullptr;
+        int4* oc = nullptr;
         int* po = nullptr;
         uint8_t* pc = nullptr;
-        if ((rc = s->alloc(&o, padded.size())) || (rc = s->alloc(&po, first.size())) || (rc = s->alloc(&pc, P)))
+        if ((rc = s->alloc(&o, padded.size())) || (rc = s->alloc(&oc, corners.size())) ||
+            (rc = s->alloc(&po, first.size())) || (rc = s->alloc(&pc, P)))
             return bail(rc);
         if (hipMemcpy(o, padded.data(), sizeof(int) * padded.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(oc, corners.data(), sizeof(int4) * corners.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(po, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(NF_E_DEVICE, "upload failed"));
         s->d.order = o;
+        s->d.orderConn = oc;
         s->d.patchOf = po;
         s->d.patchCost = pc;
         s->d.patches = P;
@@ -3224,7 +3254,7 @@ int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_
 int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n) {
     g_error.clear();
     if (n <= 0) return NF_OK;
-    if (fn < 0 || fn > 13) return fail(NF_E_ARGUMENT, "unknown function");
+    if (fn < 0 || fn > 17) return fail(NF_E_ARGUMENT, "unknown function");
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;
     TRY(dev_alloc(&dx, n));
     TRY(dev_alloc(&dy, n));

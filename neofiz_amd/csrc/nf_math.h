@@ -297,6 +297,55 @@ NF_HD double log_ordinary(double x, const Tables& T) {
     return log(x, T);
 }
 
+// Two of the ordinary-input functions side by side, for a dependency chain that waits on
+// both (the Johnson-Cook Newton loop: the strain term's pow and the rate term's log; the yield
+// check: the strain and thermal terms' pows). When every input is ordinary, both main paths
+// run as one straight-line block, so that the two chains interleave; exp's out-of-range
+// results come from exp_dd's selects rather than a branch (inside exp's range exp_dd is
+// exp_dd_main exactly). Otherwise each function runs on its own. Either way the results are
+// pow_ordinary's and log_ordinary's own bits, hence pow's and log's.
+NF_HD bool pow_input_ordinary(double x, double y) {
+    return x >= 0x1p-1022 && x < inf() && x != 1.0 && !is_nan(y) && y != 0.0 && y != 1.0;
+}
+
+NF_HD double pow_from_log(double l_hi, double l_lo, double y, const Tables& T) {
+    double p_hi, p_lo;
+    two_prod(y, l_hi, p_hi, p_lo);
+    p_lo = p_lo + y * l_lo;
+    double t_hi, t_lo;
+    fast_two_sum(p_hi, p_lo, t_hi, t_lo);
+    return exp_dd(t_hi, t_lo, T.exp);
+}
+
+// pw = pow_ordinary(x, y), lg = log_ordinary(z)
+NF_HD void pow_and_log_ordinary(double x, double y, double z, const Tables& T, double& pw, double& lg) {
+    if (pow_input_ordinary(x, y) && z >= 0x1p-1022 && z < inf()) {
+        double l_hi, l_lo, m_hi, m_lo;
+        log_dd_impl<false>(x, l_hi, l_lo, T.log);
+        log_dd_impl<false>(z, m_hi, m_lo, T.log);
+        lg = m_hi + m_lo;
+        pw = pow_from_log(l_hi, l_lo, y, T);
+    } else {
+        pw = pow_ordinary(x, y, T);
+        lg = log_ordinary(z, T);
+    }
+}
+
+// p1 = pow_ordinary(x1, y1), p2 = pow_ordinary(x2, y2)
+NF_HD void pow_pair_ordinary(double x1, double y1, double x2, double y2, const Tables& T, double& p1,
+                             double& p2) {
+    if (pow_input_ordinary(x1, y1) && pow_input_ordinary(x2, y2)) {
+        double a_hi, a_lo, b_hi, b_lo;
+        log_dd_impl<false>(x1, a_hi, a_lo, T.log);
+        log_dd_impl<false>(x2, b_hi, b_lo, T.log);
+        p1 = pow_from_log(a_hi, a_lo, y1, T);
+        p2 = pow_from_log(b_hi, b_lo, y2, T);
+    } else {
+        p1 = pow_ordinary(x1, y1, T);
+        p2 = pow_ordinary(x2, y2, T);
+    }
+}
+
 #define NFM_PI 3.141592653589793
 #define NFM_PI_2_HI 0x1.921fb54442d18p0    // pi/2 rounded
 #define NFM_PI_2_LO 0x1.1a62633145c07p-54  // pi/2 - NFM_PI_2_HI

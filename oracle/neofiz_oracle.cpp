@@ -2141,6 +2141,20 @@ extern "C" int nfo_math_eval(int32_t fn, const double* x, const double* y, doubl
                 r = fn == 12 ? pw : lx;
                 break;
             }
+            case 14:
+            case 15: {  // pow_ordinary(x, y) and log_ordinary(y) side by side
+                double pw, lg;
+                nfm::pow_and_log_ordinary(x[i], y[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, pw, lg);
+                r = fn == 14 ? pw : lg;
+                break;
+            }
+            case 16:
+            case 17: {  // pow_ordinary(x, y) and pow_ordinary(y, x) side by side
+                double p1, p2;
+                nfm::pow_pair_ordinary(x[i], y[i], y[i], x[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, p1, p2);
+                r = fn == 16 ? p1 : p2;
+                break;
+            }
             default: return fail(NF_E_ARGUMENT, "unknown function");
         }
         out[i] = r;

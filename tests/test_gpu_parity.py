@@ -60,12 +60,16 @@ def rel(a, b):
     (11, -40.0, 40.0, None),    # the Newton loop's log of exp(U)
     (12, -30.0, 3.0, (0.05, 1.5)),  # the derivative pass's pow of exp(U)
     (13, -30.0, 3.0, (0.05, 1.5)),  # and the log it shares
+    (14, -30.0, 3.0, (0.05, 1.5)),  # the Newton loop's pow and log side by side: the pow
+    (15, -30.0, 3.0, (1e-3, 1e8)),  # and the log
+    (16, -30.0, 3.0, (0.05, 1.5)),  # the yield check's two pows side by side
+    (17, -30.0, 3.0, (0.05, 1.5)),
 ])
 def test_device_math_matches_host(hip, oracle, fn, lo, hi, second):
     rng = np.random.default_rng(100 + fn)
     n = 200_000
     x = rng.uniform(lo, hi, n)
-    if fn in (0, 2, 7, 10, 11, 12, 13):
+    if fn in (0, 2, 7, 10, 11, 12, 13, 14, 15, 16, 17):
         x = np.exp(x)
     y = rng.uniform(*second, n) if second else np.zeros(n)
     import ctypes as C

@@ -100,3 +100,9 @@ def test_ordinary_input_variants_are_pow_and_log(oracle):
     # the derivative pass's pow with the logarithm it shares (functions 12, 13)
     assert np.array_equal(_bits(evaluate(b, 12, x, y)), _bits(evaluate(b, 2, x, y)))
     assert np.array_equal(_bits(evaluate(b, 13, x, y)), _bits(evaluate(b, 0, x, None)))
+    # the Newton loop's pow(x, y) and log(y) side by side (functions 14, 15), and the yield
+    # check's pow(x, y) and pow(y, x) side by side (16, 17)
+    assert np.array_equal(_bits(evaluate(b, 14, x, y)), _bits(evaluate(b, 2, x, y)))
+    assert np.array_equal(_bits(evaluate(b, 15, x, y)), _bits(evaluate(b, 0, y, None)))
+    assert np.array_equal(_bits(evaluate(b, 16, x, y)), _bits(evaluate(b, 2, x, y)))
+    assert np.array_equal(_bits(evaluate(b, 17, x, y)), _bits(evaluate(b, 2, y, x)))
