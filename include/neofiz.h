@@ -434,7 +434,13 @@ int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_
  * log (11) with special inputs out of line, and the derivative pass's pow (12) with the log
  * it shares (13) (the same results as 2, 0, 2 and 0), the Newton loop's pow(x, y) (14) and
  * log(y) (15) evaluated side by side, and the yield check's pow(x, y) (16) and pow(y, x)
- * (17) evaluated side by side (the same results as pow and log) on n inputs. These are the
+ * (17) evaluated side by side (the same results as pow and log) on n inputs, the hardware's
+ * single-precision log2 (18), exp2 (19) and reciprocal (20) of the inputs rounded to single
+ * precision, and the Johnson-Cook multiplier solve (solver/J2.java:332-375) with its leading
+ * bisections proven rather than evaluated (21), with every iteration evaluated (22; the same
+ * results as 21), and the number of bisections 21 proved (23); for 21-23, x holds five
+ * values per solve (norm, stiffness, plastic strain, thermal factor, dt) and y the five
+ * material values A, B, n, C and the reference strain rate. These are the
  * functions the kernels use in place of java.lang.Math's (core/JohnsonCook.java:139, :182,
  * :196; solver/Damage.java:315-322; core/Weibull.java:118, :130; core/DefectField.java:331),
  * evaluated where the kernels run. */

@@ -75,6 +75,79 @@ __device__ __forceinline__ void jc_terms(const Mat& j, double e, double t, const
     thermal = homologous <= 0.0 ? 1.0 : (homologous >= 1.0 ? 0.0 : 1.0 - pt);
 }
 
+// ---- The solve's leading bisection steps, proven instead of evaluated.
+//
+// The solve starts at the top of its bracket, dGamma = hi = norm / stiffness, which for a
+// point that barely yields lies far above the root. Its first iterations are then all alike:
+// g <= 0 (so hi = dGamma), |g| > tolerance, and a Newton step that leaves the bracket through
+// zero (so dGamma = 0.5 (lo + hi) with lo still +0). The state after such an iteration,
+// (lo, hi, dGamma) = (0, d, 0.5 (0 + d)), depends on its decisions alone, not on the values
+// of g and the slope. An iteration whose decisions are proven can therefore be skipped, and
+// every later value is bit-identical to the reference's.
+//
+// Proof for the iterate d: g < -tolerance (hence g <= 0 and |g| > tolerance) and
+// |g| >= d |slope| (so that g / slope >= d in real arithmetic, its rounded quotient Q >= d
+// because rounding is monotone and d is a double, and next = d - Q <= 0 is outside the
+// bracket (0, d) -- or the slope is not negative and next >= d or NaN, outside it as well).
+// g and d |slope| are estimated with single-precision hardware log2, exp2 and reciprocal for
+// the strain term's power, the rate term's logarithm and 1/eps (relative errors near 1e-6 at
+// most for the arguments allowed: eps > 1e-12, rate / eps0 < 1e15; the device self-check
+// measures them), and the exact double-precision evaluation is bounded by the
+// estimate plus FF_MARGIN = 2^-14 times the sum of the magnitudes of the terms. All terms are
+// nonnegative (A > 0, B >= 0, n in (0, 1], C >= 0 by the reference's JohnsonCook constructor,
+// core/JohnsonCook.java:74-77; thermal factor in [0, 1]; stiffness > 0), so relative errors
+// add up. (strain - A), which the exact slope forms by a rounded subtraction, is bounded by
+// B pw plus 2^-50 of the strain term. A non-finite value anywhere fails the comparisons, and
+// the iteration is then evaluated exactly.
+#define NFD_FF_MARGIN 0x1p-14
+
+struct FFSolve {
+    double plasticStrain, norm, stiffness, thermal, tolerance;
+    double invRateScale;  // 1 / (dt eps0), or 0 when dt <= 0 (rate 0, on its clamp)
+    double a, b, n, c;
+    float nf;
+};
+
+__device__ __forceinline__ bool ff_bisects(double d, const FFSolve& f) {
+    const double dEps = NFD_SQRT_2_3 * d;
+    const double eps = f.plasticStrain + dEps;  // the iteration's own eps, bit for bit
+    const float ef = (float)eps;
+    const double pw = (double)__builtin_amdgcn_exp2f(f.nf * __builtin_amdgcn_logf(ef));
+    const double bpw = f.b * pw;
+    const double strain = f.a + bpw;
+    const double arg = dEps * f.invRateScale;  // rate / eps0
+    const double lr = arg > 1.0 ? (double)__builtin_amdgcn_logf((float)arg) * 0x1.62e42fefa39efp-1 : 0.0;
+    const double rateFactor = 1.0 + f.c * lr;
+    const double flow = NFD_SQRT_2_3 * strain * rateFactor * f.thermal;
+    const double sd = f.stiffness * d;
+    const double gUp = f.norm - sd - flow + NFD_FF_MARGIN * (f.norm + sd + flow);
+    const double dOverEps = d * (double)__builtin_amdgcn_rcpf(ef);
+    const double dSlope = sd + (NFD_TWO_THIRDS * f.n * (bpw + 0x1p-50 * strain) * dOverEps * rateFactor * f.thermal +
+                                NFD_SQRT_2_3 * strain * f.c * f.thermal) * (1.0 + NFD_FF_MARGIN);
+    return eps > 1e-12 && arg < 1e15 && gUp < -f.tolerance && -gUp >= dSlope * (1.0 + 0x1p-40);
+}
+
+// The number of leading iterations proven to bisect toward zero, at most 25, with the state
+// they leave: hi and dGamma updated as the reference's loop would, lo still +0. Candidates are
+// tested two at a time (the second is the first's bisection), so that their chains overlap.
+__device__ __forceinline__ int ff_leading_bisections(const FFSolve& f, double lo, double& hi, double& dGamma) {
+    int first = 0;
+    while (first < 25) {
+        const double d0 = dGamma;
+        const double d1 = 0.5 * (lo + d0);
+        const bool p0 = ff_bisects(d0, f), p1 = ff_bisects(d1, f);
+        if (!p0) break;
+        hi = d0;
+        dGamma = d1;
+        first++;
+        if (!p1 || first == 25) break;
+        hi = d1;
+        dGamma = 0.5 * (lo + d1);
+        first++;
+    }
+    return first;
+}
+
 // solver/J2.java:332-375, the bracketed Newton solve for the Johnson-Cook multiplier.
 // `thermal` is thermalTerm(temperature), which the caller has already evaluated for the
 // yield check on the same temperature (J2.java:221 and :334 compute the same value twice).
@@ -82,18 +155,36 @@ __device__ __forceinline__ void jc_terms(const Mat& j, double e, double t, const
 // instantiated for it so that no iteration evaluates a division whose result is discarded.
 // REC (the adjoint's recording pass only): *status becomes 2 when the solve stops on its
 // iteration cap rather than its tolerance. The plain instantiation has no such code.
-// `iterations` (optional) is increased by the number of iterations the solve ran; the element
+// `iterations` (optional) is increased by the solve's cost: the iterations it evaluated, plus
+// a quarter of one for each leading bisection it proved (ff_leading_bisections); the element
 // pass schedules its waves by it.
-template <bool UNIT_RATE, bool REC = false>
+// FF = false evaluates every iteration (the device self-check compares the two forms).
+template <bool UNIT_RATE, bool REC = false, bool FF = true>
 __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, double plasticStrain,
                                                 double thermal, double dt, const Mat& j,
                                                 const nfm::Tables& T, int* status = nullptr,
-                                                int* iterations = nullptr) {
+                                                int* iterations = nullptr, int* proven = nullptr) {
     const double tolerance = 1.0e-10 * j.a;
     double lo = 0.0;
     double hi = norm / stiffness;
     double dGamma = hi;
-    for (int iteration = 0; iteration < 25; iteration++) {
+    FFSolve f;
+    f.plasticStrain = plasticStrain;
+    f.norm = norm;
+    f.stiffness = stiffness;
+    f.thermal = thermal;
+    f.tolerance = tolerance;
+    f.invRateScale = dt > 0.0 ? 1.0 / (dt * j.eps0) : 0.0;
+    f.a = j.a;
+    f.b = j.b;
+    f.n = j.n;
+    f.c = j.c;
+    f.nf = (float)j.n;
+    const int first = FF ? ff_leading_bisections(f, lo, hi, dGamma) : 0;
+    if (proven) *proven = first;
+    // The scheduling cost: exact iterations, plus a quarter of one per proven bisection.
+    if (iterations) *iterations += (first + 2) / 4 - first;
+    for (int iteration = first; iteration < 25; iteration++) {
         // The body is written with selects rather than branches (the values are the
         // reference's): the strain term's pow and the rate term's log are independent, and
         // without branches between them the compiler interleaves the two chains.

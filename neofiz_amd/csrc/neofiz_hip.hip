@@ -1225,6 +1225,29 @@ __global__ void k_math_eval(int fn, const double* x, const double* y, double* ou
             r = fn == 16 ? p1 : p2;
             break;
         }
+        // the hardware's single-precision log2, exp2 and reciprocal, whose error the Newton
+        // solve's proven bisections (nfd::ff_bisects) bound
+        case 18: r = (double)__builtin_amdgcn_logf((float)x[i]); break;
+        case 19: r = (double)__builtin_amdgcn_exp2f((float)x[i]); break;
+        case 20: r = (double)__builtin_amdgcn_rcpf((float)x[i]); break;
+        case 21:
+        case 22:
+        case 23: {  // the Johnson-Cook solve: x[5i..5i+4] = norm, stiffness, plastic strain,
+                    // thermal factor, dt; y[0..4] = A, B, n, C, eps0 (the unit-rate form when eps0 = 1)
+            const double* a = x + 5 * (size_t)i;
+            nfd::Mat m{};
+            m.a = y[0]; m.b = y[1]; m.n = y[2]; m.c = y[3]; m.eps0 = y[4];
+            const nfm::Tables T{NFM_LOG_TBL, NFM_EXP_TBL};
+            int proven = 0;
+            if (fn == 22)
+                r = m.eps0 == 1.0 ? nfd::j2_solve_loop<true, false, false>(a[0], a[1], a[2], a[3], a[4], m, T)
+                                  : nfd::j2_solve_loop<false, false, false>(a[0], a[1], a[2], a[3], a[4], m, T);
+            else
+                r = m.eps0 == 1.0 ? nfd::j2_solve_loop<true>(a[0], a[1], a[2], a[3], a[4], m, T, nullptr, nullptr, &proven)
+                                  : nfd::j2_solve_loop<false>(a[0], a[1], a[2], a[3], a[4], m, T, nullptr, nullptr, &proven);
+            if (fn == 23) r = proven;
+            break;
+        }
         default: r = 0.0;
     }
     out[i] = r;
@@ -3250,17 +3273,21 @@ int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_
 // Evaluates the device implementations of the shared transcendental functions (and the
 // device's division and square root) on n inputs: a self-check that the GPU computes exactly
 // what the host computes. fn: 0 log, 1 exp, 2 pow, 3 atan2(x, y), 4 sin, 5 cos, 6 hypot,
-// 7 sqrt, 8 x / y, 9 log1p.
+// 7 sqrt, 8 x / y, 9 log1p, ... (include/neofiz.h lists them all). For the solve (21-23) x
+// holds 5 n values and y 5.
 int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n) {
     g_error.clear();
     if (n <= 0) return NF_OK;
-    if (fn < 0 || fn > 17) return fail(NF_E_ARGUMENT, "unknown function");
+    if (fn < 0 || fn > 23) return fail(NF_E_ARGUMENT, "unknown function");
+    const bool solve = fn >= 21;
+    if (solve && !y) return fail(NF_E_ARGUMENT, "the solve needs the material values");
+    const size_t nx = solve ? 5 * (size_t)n : (size_t)n, ny = solve ? 5 : (size_t)n;
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;
-    TRY(dev_alloc(&dx, n));
-    TRY(dev_alloc(&dy, n));
+    TRY(dev_alloc(&dx, nx));
+    TRY(dev_alloc(&dy, ny));
     TRY(dev_alloc(&dout, n));
-    HIP_TRY(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
-    if (y) HIP_TRY(hipMemcpy(dy, y, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dx, x, sizeof(double) * nx, hipMemcpyHostToDevice));
+    if (y) HIP_TRY(hipMemcpy(dy, y, sizeof(double) * ny, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_math_eval, dim3(blocks_for(n)), dim3(BLOCK), 0, 0, fn, (const double*)dx,
                        (const double*)dy, dout, n);
     hipError_t err = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
