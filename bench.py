@@ -13,11 +13,13 @@ times exactly K steps with the device synchronised on both sides, and prints one
 Inputs are resident in HBM before the timed region starts.
 
 Extra objects on the line:
-  roofline      the element kernel's algorithmic bytes per launch over its mean launch
+  roofline      the dominant kernel's algorithmic bytes per launch over its mean launch
                 duration on the timed steps (replayed on an identical twin of the rod, whose
                 trajectory is bit-identical, with start/stop events recorded by each dispatch
                 on the library's stream), against the 8 TB/s HBM peak; traffic (HBM bytes per
-                launch) from the committed rocprofv3 PMC summary.
+                launch) from the committed rocprofv3 PMC summary. The dominant kernel is the
+                fused step kernel k_step (the node update and the element pass in one launch,
+                every step of a run call after its first), so its bytes are the whole step's.
   full_run      the same rod from t = 0 to 80 us: steps, element-updates/s, HBM fraction,
                 L/L0 and mushroom diameter, so the figure does not rest on one window.
   ladder        the rods of the refinement ladder to 80 us: 8x80 linear hardening without
@@ -44,6 +46,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "element_kernel_traffic.txt")
+STEP_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_traffic.txt")
 
 
 def element_kernel_bytes(E, N, kinematic=False):
@@ -63,6 +66,13 @@ def step_bytes(E, N):
     """Algorithmic bytes of one whole step (element + gather + nodal):
     B = 352 + 160 N/E per element-step for reduced integration with a rigid wall."""
     return 352.0 * E + 160.0 * N
+
+
+def step_kernel_bytes(E, N, kinematic=False):
+    """Algorithmic bytes one launch of the fused step kernel must move: the whole step's
+    (step_bytes), less the back stress read and written (64 per element) when no material
+    hardens kinematically and the kernel neither loads nor stores it."""
+    return step_bytes(E, N) - (0.0 if kinematic else 64.0 * E)
 
 
 def cpu_baseline(nr, nz, seconds, threads, bursts=3):
@@ -263,15 +273,15 @@ def perforation_leg(elements, warmup, steps):
     elapsed = time.perf_counter() - t0
     st = s.state()
     b = s._b
-    ms = (C.c_double * 3)()
-    counts = (C.c_int64 * 3)()
+    ms = (C.c_double * 4)()
+    counts = (C.c_int64 * 4)()
     b.profile(s._h, 1)
     b.profile_read(s._h, ms, counts)
     s.run(200)
     b.profile_read(s._h, ms, counts)
     b.profile(s._h, 0)
     s.shutdown()
-    per = [ms[k] / max(1, counts[k]) for k in range(3)]
+    per = [ms[k] / max(1, counts[k]) for k in range(4)]
     return {"workload": f"perforation-{mesh.elementCount}", "elements": mesh.elementCount,
             "nodes": mesh.nodeCount, "cell_m": cell, "steps_from_t0": [warmup, warmup + steps],
             "value": mesh.elementCount * steps / elapsed, "unit": "element-updates/s",
@@ -407,23 +417,30 @@ def main():
     _, twin, _ = rod()
     twin.run(args.warmup)
     b = twin._b
-    ms = (C.c_double * 3)()
-    counts = (C.c_int64 * 3)()
+    ms = (C.c_double * 4)()
+    counts = (C.c_int64 * 4)()
     b.profile(twin._h, 1)
     b.profile_read(twin._h, ms, counts)
     twin.run(args.steps)
     b.profile_read(twin._h, ms, counts)
     b.profile(twin._h, 0)
-    kernel_ms = [ms[k] / max(1, counts[k]) for k in range(3)]
+    kernel_ms = [ms[k] / max(1, counts[k]) for k in range(4)]
+    launches = [int(counts[k]) for k in range(4)]
     same_window = bool(np.array_equal(twin.axialDisplacement().view(np.uint64), final_u.view(np.uint64)))
     twin.shutdown()
 
-    elem_bytes = element_kernel_bytes(E, N, kinematic=False)
-    achieved = elem_bytes / (kernel_ms[1] * 1e-3) / 1e9
+    # The dominant kernel: the fused step kernel (node update + element pass, one launch per
+    # step after a run call's first), or the element kernel where the step runs as two launches.
+    fused = launches[3] > 0
+    if fused:
+        dom_name, dom_ms, dom_bytes, dom_file = "k_step", kernel_ms[3], step_kernel_bytes(E, N), STEP_TRAFFIC_FILE
+    else:
+        dom_name, dom_ms, dom_bytes, dom_file = "k_element", kernel_ms[1], element_kernel_bytes(E, N), TRAFFIC_FILE
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if os.path.exists(dom_file):
         t = {}
-        with open(TRAFFIC_FILE) as f:
+        with open(dom_file) as f:
             for line in f:
                 if ":" in line and not line.startswith("#"):
                     k, v = line.split(":", 1)
@@ -453,12 +470,14 @@ def main():
                    "steps_from_t0": [args.warmup, args.warmup + args.steps],
                    "parallelism": "single-gpu"},
         "hbm_fraction_step": value * (step_bytes(E, N) / E) / (HBM_PEAK_GBPS * 1e9),
-        "kernel_ms": {"gather_nodal": kernel_ms[0], "element": kernel_ms[1],
+        "kernel_ms": {"step_kernel": kernel_ms[3] if fused else None, "element": kernel_ms[1],
+                      "node_pass": kernel_ms[0],
+                      "launches": {"step_kernel": launches[3], "element": launches[1], "node_pass": launches[0]},
                       "window": "the timed steps, replayed on an identical twin with events on",
                       "twin_trajectory_identical": same_window},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "bytes_per_launch": elem_bytes},
+                     "bytes_per_launch": dom_bytes},
     }
     if not args.no_copy:
         try:

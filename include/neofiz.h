@@ -408,8 +408,8 @@ int nf_normal_eval(int32_t function, int32_t n, const double* x, double* out);
 /* Per-kernel timing of the step loop (no counterpart in the reference, which times whole
  * runs with System.nanoTime, validate/TaylorImpactCase.java:254-260). When enabled, each
  * launch is bracketed by HIP events on the handle's stream; nf_profile_read returns summed
- * milliseconds and launch counts for {node pass, element pass, contact launches} and resets
- * them. */
+ * milliseconds and launch counts for {node pass, element pass, contact launches, fused step
+ * kernel} (arrays of four) and resets them. */
 int nf_profile(nf_solver* s, int32_t enable);
 int nf_profile_read(nf_solver* s, double* ms, int64_t* counts);
 

@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_element_rec(Dev d, ARec a, int j, 
         if (m != MAX_BITS) atomicMin(&st.minEdgeBits, m);
         if (ldsNaN) atomicOr(&st.nanEdge, 1);
     }
-    if (d.wall && blockIdx.x == 0) reduce_wall_partials(d);
+    if (d.wall && blockIdx.x == 0) reduce_wall_partials(d, d.wallPartial, d.wallCountPartial, d.nodalBlocks);
 }
 
 // The step's shortest squared edge and the first element holding it (the CFL step's

@@ -59,6 +59,8 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr int BLOCK = 256;
+// Nodes of one patch the fused step kernel holds in LDS (a lattice patch of 8 x 8 elements has 81).
+constexpr int STEP_PATCH_NODES = 128;
 constexpr uint8_t FIXED_R = 1, FIXED_Z = 2, ON_WALL = 4;
 constexpr int MODE_STEP = 0, MODE_INITIAL = 1, MODE_ADVANCE = 2;
 
@@ -71,6 +73,11 @@ struct Step {
     int go;       // the element pass that reads this buffer executes
     int nanEdge;  // some element's shortest edge was NaN
     unsigned long long minEdgeBits;  // exact minimum of the squared shortest edge, as bits
+    // The fused step kernel (k_step): which of the two node-state and slot buffers holds the
+    // current state (0: the canonical arrays), and whether the launch that wrote this buffer
+    // left wall audit partials (in wallPartialF[this buffer's index]) for the next to add.
+    int buf;
+    int pendingWall;
 };
 
 struct Ctl {
@@ -91,9 +98,24 @@ struct Ctl {
     double cEnergy, cDissipation, cMaxPen;
     long long cEscaped;
     int cFCount, cFOverflow;  // entries of the capture grid this assembly; it overflowed
+    // The fused step kernel's shortest-edge minima, rotated over three slots: launch k reads
+    // slot k % 3, atomically lowers slot (k + 1) % 3 and resets slot (k + 2) % 3, so that no
+    // slot is reset while another workgroup of the same launch reads or lowers it.
+    unsigned long long fMin[3];
+    int fNaN[3];
 };
 
 constexpr unsigned long long MAX_BITS = 0x7FEFFFFFFFFFFFFFULL;  // Double.MAX_VALUE
+
+// One node of a patch in the fused step kernel's node lists: the node's static data in one
+// 64-byte record, read with coalesced loads (info: bit 0 the patch owns the node, bit 1 the node
+// has more than four force slots, the rest on the CSR).
+struct alignas(16) PatchNode {
+    double2 x0;
+    double z0, pad;
+    int4 slot4;
+    int node, info, pad0, pad1;
+};
 
 // Everything a kernel needs, passed by value.
 struct Dev {
@@ -112,9 +134,26 @@ struct Dev {
     // patch), lane l element order[64 p + l] (-1 past the last element).
     const int* order;     // [64 patches]
     const int4* orderConn;  // [64 patches] the corners of order[k]
-    int* patchOf;         // [4 patchBlocks]
-    uint8_t* patchCost;   // [patches] Newton iterations of the patch's slowest lane, last step
+    // Both double-buffered by the element pass's control-buffer index q: the launch that reads
+    // Step q takes its schedule from patchOf + q * 4 patchBlocks and writes its costs to
+    // patchCost + q * patches.
+    int* patchOf;         // [2][4 patchBlocks]
+    uint8_t* patchCost;   // [2][patches] Newton cost of the patch's slowest lane, last step
     int patches, patchBlocks, units;  // units: the device's compute units
+    // The fused step kernel (k_step). Patch p's nodes are pn[pnStart[p] .. pnStart[p+1])
+    // (ascending node numbers; p owns a node when it holds the node's lowest-numbered element);
+    // orderLocal[64 p + l] are the corners of element order[64 p + l] as indices into that
+    // list. Node state and force slots are double-buffered: buffer 0 is u, v, flags, slot;
+    // buffer 1 is uB, vB, flagsB, slotB.
+    const int* pnStart;
+    const PatchNode* pn;
+    const uchar4* orderLocal;
+    double2* uB;
+    double2* vB;
+    uint8_t* flagsB;
+    double2* slotB;
+    double* wallPartialF;     // [2][patchBlocks * 3]
+    int* wallCountPartialF;   // [2][patchBlocks * 2]
     const int* matIndex;  // null when uniform
     const Mat* mats;
     Mat mat0;             // the uniform material, by value (kernel arguments sit in SGPRs)
@@ -218,15 +257,16 @@ __device__ void block_sum(double (&vals)[K], double* lds) {
 // One node of the central-difference update, solver/ExplicitSolver.java:998-1062, with the
 // rigid wall of solver/RigidWall.java:163-279. acc receives this node's contribution to the
 // wall's impulse, arrival energy loss and friction dissipation.
+// The central-difference update of one node (solver/ExplicitSolver.java:998-1062, with the
+// rigid wall of solver/RigidWall.java:163-279) on values in registers: mass m, reference
+// axial coordinate z0, flags fl, velocity vi, displacement ui, gathered force fi. The caller
+// loads and stores them (nodal_node below, or the fused step kernel's node phase).
 template <bool WALL>
-__device__ __forceinline__ void nodal_node(const Dev& d, int i, double2 fi, double dt,
-                                           double dtAvg, double decay, double gain,
-                                           double (&acc)[3], int& contacts, int& stuck) {
-    const double m = d.mass[i];
+__device__ __forceinline__ void nodal_core(const Dev& d, double m, double z0, uint8_t& fl, double2& vi,
+                                           double2& ui, double2 fi, double dt, double dtAvg,
+                                           double decay, double gain, double (&acc)[3], int& contacts,
+                                           int& stuck) {
     const double invM = 1.0 / m;
-    uint8_t fl = d.flags[i];
-    double2 vi = d.v[i];
-    double2 ui = d.u[i];
     const bool resting = WALL && (fl & ON_WALL) && fi.y <= 0.0;
     if (fl & FIXED_R) {
         vi.x = 0.0;
@@ -257,7 +297,7 @@ __device__ __forceinline__ void nodal_node(const Dev& d, int i, double2 fi, doub
         contacts++;
     } else {
         const double trial = decay * vi.y + gain * fi.y * invM;
-        const double zc = d.z0[i] + ui.y;
+        const double zc = z0 + ui.y;
         if (WALL && zc + trial * dt < d.plane) {
             acc[0] += -m * trial;  // RigidWall.arrive :201-205
             acc[1] += 0.5 * m * trial * trial;
@@ -271,6 +311,16 @@ __device__ __forceinline__ void nodal_node(const Dev& d, int i, double2 fi, doub
             if (WALL) fl &= (uint8_t)~ON_WALL;
         }
     }
+}
+
+template <bool WALL>
+__device__ __forceinline__ void nodal_node(const Dev& d, int i, double2 fi, double dt,
+                                           double dtAvg, double decay, double gain,
+                                           double (&acc)[3], int& contacts, int& stuck) {
+    uint8_t fl = d.flags[i];
+    double2 vi = d.v[i];
+    double2 ui = d.u[i];
+    nodal_core<WALL>(d, d.mass[i], WALL ? d.z0[i] : 0.0, fl, vi, ui, fi, dt, dtAvg, decay, gain, acc, contacts, stuck);
     d.v[i] = vi;
     d.u[i] = ui;
     if (WALL) d.flags[i] = fl;
@@ -281,13 +331,14 @@ __device__ __forceinline__ void nodal_node(const Dev& d, int i, double2 fi, doub
 // A node's first four slots come from one 16-byte load, so their four loads issue together
 // instead of each waiting on an index load; a node with more sources (rare on a quad mesh)
 // continues through the CSR. The sum runs in the same order either way.
-__device__ __forceinline__ double2 gather_node(const Dev& d, int n) {
+// slot: the force slots to gather (d.slot, or the fused step's other slot buffer).
+__device__ __forceinline__ double2 gather_slots(const Dev& d, const double2* __restrict__ slot, int n) {
     const int4 k4 = d.slot4[n];
     const double2 z = make_double2(0.0, 0.0);
-    const double2 s0 = k4.x >= 0 ? d.slot[k4.x] : z;
-    const double2 s1 = k4.y >= 0 ? d.slot[k4.y] : z;
-    const double2 s2 = k4.z >= 0 ? d.slot[k4.z] : z;
-    const double2 s3 = k4.w >= 0 ? d.slot[k4.w] : z;
+    const double2 s0 = k4.x >= 0 ? slot[k4.x] : z;
+    const double2 s1 = k4.y >= 0 ? slot[k4.y] : z;
+    const double2 s2 = k4.z >= 0 ? slot[k4.z] : z;
+    const double2 s3 = k4.w >= 0 ? slot[k4.w] : z;
     double sr = 0.0, sz = 0.0;
     if (k4.x >= 0) { sr += s0.x; sz += s0.y; }
     if (k4.y >= 0) { sr += s1.x; sz += s1.y; }
@@ -295,13 +346,14 @@ __device__ __forceinline__ double2 gather_node(const Dev& d, int n) {
     if (k4.w >= 0) { sr += s3.x; sz += s3.y; }
     const int start = d.slotStart[n] + 4, end = d.slotStart[n + 1];
     for (int k = start; k < end; k++) {
-        const double2 s = d.slot[d.slotOf[k]];
+        const double2 s = slot[d.slotOf[k]];
         sr += s.x;
         sz += s.y;
     }
     const double m = d.mass[n];
     return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
 }
+__device__ __forceinline__ double2 gather_node(const Dev& d, int n) { return gather_slots(d, d.slot, n); }
 
 #include "device_contact.h"
 
@@ -329,12 +381,11 @@ __host__ __device__ __forceinline__ int patch_slot(int r, int patchBlocks, int u
     return 4 * pos + (r & 3);
 }
 
-// The element pass's schedule for its next launch, from each patch's cost in the last one:
-// a counting sort of the patches by cost, costliest first, into patch_slot order. One
-// workgroup does it, so the cost array it reads is the one the finished element launch wrote.
-// Patches of equal cost are ranked in the order the LDS atomics give: the schedule decides
-// only which wave takes a patch, never a result.
-__device__ void schedule_patches(const Dev& d) {
+// The element pass's schedule for a later launch, from each patch's cost in a finished one
+// (costs in, schedule out): a counting sort of the patches by cost, costliest first, into
+// patch_slot order. One workgroup does it. Patches of equal cost are ranked in the order the
+// LDS atomics give: the schedule decides only which wave takes a patch, never a result.
+__device__ void schedule_patches(const Dev& d, const uint8_t* __restrict__ costIn, int* __restrict__ patchOfOut) {
     constexpr int CHUNK = 4096;  // costs staged in LDS per pass, read 16 at a time
     __shared__ int start[256];
     __shared__ uint4 staged[CHUNK / 16];
@@ -347,10 +398,10 @@ __device__ void schedule_patches(const Dev& d) {
         __syncthreads();
         for (int k = threadIdx.x; k < (n + 15) / 16; k += BLOCK) {
             if (16 * k + 16 <= n && ((from & 15) == 0)) {
-                staged[k] = *(const uint4*)(d.patchCost + from + 16 * k);
+                staged[k] = *(const uint4*)(costIn + from + 16 * k);
             } else {
                 for (int j = 0; j < 16; j++)
-                    ((uint8_t*)staged)[16 * k + j] = 16 * k + j < n ? d.patchCost[from + 16 * k + j] : 0;
+                    ((uint8_t*)staged)[16 * k + j] = 16 * k + j < n ? costIn[from + 16 * k + j] : 0;
             }
         }
         __syncthreads();
@@ -377,16 +428,20 @@ __device__ void schedule_patches(const Dev& d) {
         const int n = whole ? (__syncthreads(), P) : stage(from);
         for (int p = threadIdx.x; p < n; p += BLOCK) {
             const int r = atomicAdd(&start[cost[p]], 1);
-            d.patchOf[patch_slot(r, d.patchBlocks, d.units)] = from + p;
+            patchOfOut[patch_slot(r, d.patchBlocks, d.units)] = from + p;
         }
     }
 }
 
 // Reads control buffer q; block 0 writes buffer q ^ 1.
+// minSlot >= 0: the preceding element pass was the fused step kernel's, whose shortest edge is
+// in Ctl::fMin[minSlot] (and the node state has been brought back to buffer 0, k_canon).
 template <bool WALL>
-__global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part) {
+__global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part, int minSlot) {
     if (d.order && !d.gone && part != PART_REST && blockIdx.x == gridDim.x - 1) {
-        schedule_patches(d);  // the workgroup past the others (launchGNT)
+        // the workgroup past the others (launchGNT): the schedule of the element launch that
+        // reads Step q ^ 1, from the costs of the one that read Step q
+        schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)(q ^ 1) * 4 * d.patchBlocks);
         return;
     }
     Ctl* c = d.ctl;
@@ -394,7 +449,12 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
     Step& out = c->s[q ^ 1];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (phase == PHASE_STEP && !in.go) {  // nothing ran: carry the state over
-        if (lead && part != PART_GATHER) out = in;
+        if (lead && part != PART_GATHER) {
+            Step o = in;
+            o.buf = 0;
+            o.pendingWall = 0;
+            out = o;
+        }
         return;
     }
     const bool gathered = phase == PHASE_STEP || gatherFirst;
@@ -422,7 +482,9 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
     bool next = true;
     if (phase == PHASE_STEP) {
         if (d.finite) {
-            const double minEdge = in.nanEdge ? NFD_MAX_VALUE : nfm::from_bits(in.minEdgeBits);
+            const bool nanEdge = minSlot >= 0 ? c->fNaN[minSlot] != 0 : in.nanEdge != 0;
+            const unsigned long long bits = minSlot >= 0 ? c->fMin[minSlot] : in.minEdgeBits;
+            const double minEdge = nanEdge ? NFD_MAX_VALUE : nfm::from_bits(bits);
             dtPrev = in.dt;
             dt = d.cfl * __builtin_sqrt(minEdge * d.massScale) / d.fastestWave;
         }
@@ -441,6 +503,8 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
         }
         o.nanEdge = 0;
         o.minEdgeBits = MAX_BITS;
+        o.buf = 0;
+        o.pendingWall = 0;
         out = o;
     }
 
@@ -491,17 +555,18 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
 
 // Adds the wall partials of the preceding nodal update to the running totals, in block
 // order through a fixed tree (run by block 0 of the element launch that follows it).
-__device__ void reduce_wall_partials(const Dev& d) {
+__device__ void reduce_wall_partials(const Dev& d, const double* __restrict__ partial,
+                                     const int* __restrict__ countPartial, int blocks) {
     __shared__ double lds[4 * 3];
     __shared__ int ldsi[2 * 4];
     double tot[3] = {0.0, 0.0, 0.0};
     int cc = 0, cs = 0;
-    for (int b = threadIdx.x; b < d.nodalBlocks; b += BLOCK) {
-        tot[0] += d.wallPartial[3 * b];
-        tot[1] += d.wallPartial[3 * b + 1];
-        tot[2] += d.wallPartial[3 * b + 2];
-        cc += d.wallCountPartial[2 * b];
-        cs += d.wallCountPartial[2 * b + 1];
+    for (int b = threadIdx.x; b < blocks; b += BLOCK) {
+        tot[0] += partial[3 * b];
+        tot[1] += partial[3 * b + 1];
+        tot[2] += partial[3 * b + 2];
+        cc += countPartial[2 * b];
+        cs += countPartial[2 * b + 1];
     }
     block_sum<3>(tot, lds);
     for (int off = 32; off > 0; off >>= 1) {
@@ -644,16 +709,27 @@ __device__ __forceinline__ void write_zero_slots(const Dev& d, int e) {
 // One element under reduced integration, solver/ExplicitSolver.java:1373-1504.
 // REC (the adjoint's recording pass only): *dGammaOut and *status receive the return map's
 // multiplier and outcome (j2_update).
+// The reference positions, displacements and velocities of an element's four corners.
+struct Corners {
+    double2 p0, p1, p2, p3, u0, u1, u2, u3, v0, v1, v2, v3;
+};
+__device__ __forceinline__ Corners load_corners(const Dev& d, int4 n) {
+    return Corners{d.x0[n.x], d.x0[n.y], d.x0[n.z], d.x0[n.w], d.u[n.x], d.u[n.y], d.u[n.z], d.u[n.w],
+                   d.v[n.x], d.v[n.y], d.v[n.z], d.v[n.w]};
+}
+
+// The element on corner values already in registers (c), writing its four force slots to
+// slotOut (d.slot, or the fused step kernel's other slot buffer).
 template <bool DAMAGE, bool KIN, bool REC = false>
-__device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, double strainDt,
-                                       const nfm::Tables& T, double* dGammaOut = nullptr,
+__device__ __forceinline__ double reduced_element_at(const Dev& d, int e, const Corners& c, double2* slotOut,
+                                       double strainDt, const nfm::Tables& T, double* dGammaOut = nullptr,
                                        int* status = nullptr, int* iterations = nullptr) {
     const Mat fl = load_mat(d, e);
     const double mu = fl.mu;
     const double mid = -0.5 * strainDt;
-    const double2 p0 = d.x0[n.x], p1 = d.x0[n.y], p2 = d.x0[n.z], p3 = d.x0[n.w];
-    const double2 u0 = d.u[n.x], u1 = d.u[n.y], u2 = d.u[n.z], u3 = d.u[n.w];
-    const double2 v0 = d.v[n.x], v1 = d.v[n.y], v2 = d.v[n.z], v3 = d.v[n.w];
+    const double2 p0 = c.p0, p1 = c.p1, p2 = c.p2, p3 = c.p3;
+    const double2 u0 = c.u0, u1 = c.u1, u2 = c.u2, u3 = c.u3;
+    const double2 v0 = c.v0, v1 = c.v1, v2 = c.v2, v3 = c.v3;
     double r0, r1, r2, r3, z0, z1, z2, z3;
     if (d.finite) {  // cr / cz :1176-1182
         r0 = p0.x + u0.x + mid * v0.x; r1 = p1.x + u1.x + mid * v1.x;
@@ -751,22 +827,31 @@ __device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, d
         g0z -= y0 * qz; g1z -= y1 * qz; g2z -= y2 * qz; g3z -= y3 * qz;
         d.hgK[e] = k;
     }
-    d.slot[4 * e] = make_double2(g0r, g0z);
-    d.slot[4 * e + 1] = make_double2(g1r, g1z);
-    d.slot[4 * e + 2] = make_double2(g2r, g2z);
-    d.slot[4 * e + 3] = make_double2(g3r, g3z);
+    slotOut[4 * e] = make_double2(g0r, g0z);
+    slotOut[4 * e + 1] = make_double2(g1r, g1z);
+    slotOut[4 * e + 2] = make_double2(g2r, g2z);
+    slotOut[4 * e + 3] = make_double2(g3r, g3z);
     return minEdge;
 }
 
-// One element under full 2x2 integration, solver/ExplicitSolver.java:1219-1339.
+template <bool DAMAGE, bool KIN, bool REC = false>
+__device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, double strainDt,
+                                       const nfm::Tables& T, double* dGammaOut = nullptr,
+                                       int* status = nullptr, int* iterations = nullptr) {
+    return reduced_element_at<DAMAGE, KIN, REC>(d, e, load_corners(d, n), d.slot, strainDt, T, dGammaOut,
+                                                status, iterations);
+}
+
+// One element under full 2x2 integration, solver/ExplicitSolver.java:1219-1339, on corner
+// values in registers.
 template <bool DAMAGE, bool KIN>
-__device__ __forceinline__ double full_element(const Dev& d, int e, int4 n, double strainDt,
-                                       const nfm::Tables& T, int* iterations = nullptr) {
+__device__ __forceinline__ double full_element_at(const Dev& d, int e, const Corners& c, double2* slotOut,
+                                       double strainDt, const nfm::Tables& T, int* iterations = nullptr) {
     const Mat fl = load_mat(d, e);
     const double mid = -0.5 * strainDt;
-    const double2 p0 = d.x0[n.x], p1 = d.x0[n.y], p2 = d.x0[n.z], p3 = d.x0[n.w];
-    const double2 u0 = d.u[n.x], u1 = d.u[n.y], u2 = d.u[n.z], u3 = d.u[n.w];
-    const double2 v0 = d.v[n.x], v1 = d.v[n.y], v2 = d.v[n.z], v3 = d.v[n.w];
+    const double2 p0 = c.p0, p1 = c.p1, p2 = c.p2, p3 = c.p3;
+    const double2 u0 = c.u0, u1 = c.u1, u2 = c.u2, u3 = c.u3;
+    const double2 v0 = c.v0, v1 = c.v1, v2 = c.v2, v3 = c.v3;
     double r0, r1, r2, r3, z0, z1, z2, z3;
     if (d.finite) {
         r0 = p0.x + u0.x + mid * v0.x; r1 = p1.x + u1.x + mid * v1.x;
@@ -856,11 +941,17 @@ __device__ __forceinline__ double full_element(const Dev& d, int e, int4 n, doub
         g2z -= (d2z * sZ + d2r * sRZ) * dV;
         g3z -= (d3z * sZ + d3r * sRZ) * dV;
     }
-    d.slot[4 * e] = make_double2(g0r, g0z);
-    d.slot[4 * e + 1] = make_double2(g1r, g1z);
-    d.slot[4 * e + 2] = make_double2(g2r, g2z);
-    d.slot[4 * e + 3] = make_double2(g3r, g3z);
+    slotOut[4 * e] = make_double2(g0r, g0z);
+    slotOut[4 * e + 1] = make_double2(g1r, g1z);
+    slotOut[4 * e + 2] = make_double2(g2r, g2z);
+    slotOut[4 * e + 3] = make_double2(g3r, g3z);
     return minEdge;
+}
+
+template <bool DAMAGE, bool KIN>
+__device__ __forceinline__ double full_element(const Dev& d, int e, int4 n, double strainDt,
+                                       const nfm::Tables& T, int* iterations = nullptr) {
+    return full_element_at<DAMAGE, KIN>(d, e, load_corners(d, n), d.slot, strainDt, T, iterations);
 }
 
 // Pressure edge k, solver/ExplicitSolver.java:1511-1556 (coordinates at the current, not the
@@ -923,7 +1014,7 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
         edgeIndex = t - elementThreads;
         e = -1;
         if (t < elementThreads) {
-            patch = d.patchOf[__builtin_amdgcn_readfirstlane(t >> 6)];  // one scalar load a wave
+            patch = d.patchOf[(size_t)q * 4 * d.patchBlocks + __builtin_amdgcn_readfirstlane(t >> 6)];  // one scalar load a wave
             if (patch >= 0) {
                 e = d.order[64 * patch + (t & 63)];
                 n = d.orderConn[64 * patch + (t & 63)];
@@ -962,7 +1053,7 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
             const int o = __shfl_xor(m, off, 64);
             m = o > m ? o : m;
         }
-        if ((threadIdx.x & 63) == 0) d.patchCost[patch] = (uint8_t)(m < 255 ? m : 255);
+        if ((threadIdx.x & 63) == 0) d.patchCost[(size_t)q * d.patches + patch] = (uint8_t)(m < 255 ? m : 255);
     }
 
     // Shortest edge: exact minimum over non-negative doubles by their bit patterns.
@@ -1008,7 +1099,281 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
         if (m != MAX_BITS) atomicMin(&st.minEdgeBits, m);
         if (ldsNaN) atomicOr(&st.nanEdge, 1);
     }
-    if (mode == MODE_STEP && d.wall && blockIdx.x == 0) reduce_wall_partials(d);
+    if (mode == MODE_STEP && d.wall && blockIdx.x == 0)
+        reduce_wall_partials(d, d.wallPartial, d.wallCountPartial, d.nodalBlocks);
+}
+
+// ------------------------------------------------------------------ the fused step
+
+// One launch per step where the step has no penalty contact, erosion or pressure edges: the
+// node launch that closes step n - 1 and opens step n (k_gn) and the element launch of step n
+// (k_element) in one kernel. The host enqueues it in place of the pair k_gn, k_element, so a
+// run is: k_element, k_step ... k_step, k_canon, k_gn.
+//
+// Each wave takes a patch of 64 elements from the element pass's schedule and first does the
+// node launch's work for every node of the patch: the fixed-order gather of step n - 1's force
+// slots plus gravity, the CFL step and end test (the same values in every workgroup), and the
+// central-difference update with the rigid wall that opens step n. A node on the patch's border
+// is also computed by the neighbouring patches, from the same inputs by the same operations,
+// so every copy has the same bits; only the owning patch writes its new state and gathered
+// force and counts its wall audits. The updated positions and velocities stay in LDS, and the
+// element pass of step n reads its corners there. Because a patch reads the state that the
+// owners of its border nodes overwrite in the same launch, node state and force slots are
+// double-buffered: the launch reads buffer Step::buf and writes the other.
+//
+// Control: the launch reads Step q and its lead writes Step q ^ 1 (as k_gn does); the element
+// phase belongs to Step q ^ 1, so it takes the schedule patchOf[q ^ 1] and writes
+// patchCost[q ^ 1]; the last workgroup remakes patchOf[q] from patchCost[q] (the previous
+// launch's costs) for the launch after next. The shortest edge goes through Ctl::fMin: the
+// launch with rotation index k3 reads slot k3 (or, first after a plain element launch, Step q),
+// lowers slot (k3 + 1) % 3 and resets slot (k3 + 2) % 3. Wall partials go to
+// wallPartialF[q ^ 1]; the next launch (or k_canon) adds them up when Step::pendingWall says so.
+template <bool FULL, bool DAMAGE, bool KIN, bool WALL>
+__global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d, int q, int k3, int minFromStep) {
+    if (blockIdx.x == gridDim.x - 1) {
+        schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)q * 4 * d.patchBlocks);
+        return;
+    }
+    Ctl* c = d.ctl;
+    const Step& in = c->s[q];
+    Step& out = c->s[q ^ 1];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int patch = d.patchOf[(size_t)(q ^ 1) * 4 * d.patchBlocks +
+                                __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / 64) + wave)];
+    const int next3 = k3 == 2 ? 0 : k3 + 1, last3 = k3 == 0 ? 2 : k3 - 1;
+    if (!in.go) {  // nothing ran: carry the control over
+        if (lead) {
+            out = in;
+            c->fMin[last3] = MAX_BITS;
+            c->fNaN[last3] = 0;
+        }
+        return;
+    }
+    double dt = in.dt, dtPrev = in.dtPrev;
+    if (d.finite) {
+        const bool nanEdge = minFromStep ? in.nanEdge != 0 : c->fNaN[k3] != 0;
+        const unsigned long long bits = minFromStep ? in.minEdgeBits : c->fMin[k3];
+        const double minEdge = nanEdge ? NFD_MAX_VALUE : nfm::from_bits(bits);
+        dtPrev = in.dt;
+        dt = d.cfl * __builtin_sqrt(minEdge * d.massScale) / d.fastestWave;
+    }
+    const bool next = in.time < in.endTime && in.steps < in.stepLimit;
+    const int b = in.buf;
+    if (lead) {
+        Step o = in;
+        o.dt = dt;
+        o.dtPrev = dtPrev;
+        o.go = next ? 1 : 0;
+        if (next) {
+            o.time = in.time + dt;
+            o.steps = in.steps + 1;
+        }
+        o.nanEdge = 0;
+        o.minEdgeBits = MAX_BITS;
+        o.buf = next ? b ^ 1 : b;
+        o.pendingWall = (WALL && next) ? 1 : 0;
+        out = o;
+        c->fMin[last3] = MAX_BITS;
+        c->fNaN[last3] = 0;
+    }
+    const double2* slotIn = b ? d.slotB : d.slot;
+    double2* slotOut = b ? d.slot : d.slotB;
+    const double2* uIn = b ? d.uB : d.u;
+    const double2* vIn = b ? d.vB : d.v;
+    const uint8_t* flagsIn = b ? d.flagsB : d.flags;
+    double2* uOut = b ? d.u : d.uB;
+    double2* vOut = b ? d.v : d.vB;
+    uint8_t* flagsOut = b ? d.flags : d.flagsB;
+
+    // The log/exp tables first: their loads overlap the node records' below, and their barrier
+    // is the only workgroup barrier before the element phase.
+    __shared__ double sLog[97][3];
+    __shared__ double sExp[64][2];
+    int first = 0, count = 0;
+    if (patch >= 0) {
+        first = d.pnStart[patch];
+        count = d.pnStart[patch + 1] - first;
+    }
+    const bool has0 = lane < count, has1 = lane + 64 < count;
+    PatchNode r0{}, r1{};
+    if (has0) r0 = d.pn[first + lane];
+    if (has1) r1 = d.pn[first + lane + 64];
+    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
+
+    // Node phase: the patch's nodes, at most two per lane, every load of both issued before
+    // either is used.
+    __shared__ double2 sX0[BLOCK / 64][STEP_PATCH_NODES];
+    __shared__ double2 sU[BLOCK / 64][STEP_PATCH_NODES];
+    __shared__ double2 sV[BLOCK / 64][STEP_PATCH_NODES];
+    const double dtAvg = 0.5 * (dtPrev + dt);
+    const double half = 0.5 * d.damping * dtAvg;
+    const double decay = (1.0 - half) / (1.0 + half);
+    const double gain = dtAvg / (1.0 + half);
+    double acc[3] = {0.0, 0.0, 0.0};
+    int contacts = 0, stuck = 0;
+    {
+        const double2 z2 = make_double2(0.0, 0.0);
+        const int k0 = has0 ? r0.node : 0, k1 = has1 ? r1.node : 0;
+        double2 a0[4], a1[4];
+        a0[0] = has0 && r0.slot4.x >= 0 ? slotIn[r0.slot4.x] : z2;
+        a0[1] = has0 && r0.slot4.y >= 0 ? slotIn[r0.slot4.y] : z2;
+        a0[2] = has0 && r0.slot4.z >= 0 ? slotIn[r0.slot4.z] : z2;
+        a0[3] = has0 && r0.slot4.w >= 0 ? slotIn[r0.slot4.w] : z2;
+        a1[0] = has1 && r1.slot4.x >= 0 ? slotIn[r1.slot4.x] : z2;
+        a1[1] = has1 && r1.slot4.y >= 0 ? slotIn[r1.slot4.y] : z2;
+        a1[2] = has1 && r1.slot4.z >= 0 ? slotIn[r1.slot4.z] : z2;
+        a1[3] = has1 && r1.slot4.w >= 0 ? slotIn[r1.slot4.w] : z2;
+        const double m0 = has0 ? d.mass[k0] : 1.0, m1 = has1 ? d.mass[k1] : 1.0;
+        double2 uu0 = has0 ? uIn[k0] : z2, uu1 = has1 ? uIn[k1] : z2;
+        double2 vv0 = has0 ? vIn[k0] : z2, vv1 = has1 ? vIn[k1] : z2;
+        uint8_t fl0 = has0 ? flagsIn[k0] : 0, fl1 = has1 ? flagsIn[k1] : 0;
+        // The gather in the reference's order (solver/ExplicitSolver.java:1147-1165): the first
+        // four slots, then any further ones on the CSR, then gravity (gather_slots' sum).
+        auto gathered = [&](const PatchNode& r, const double2 (&a)[4], int k, double m) {
+            double sr = 0.0, sz = 0.0;
+            if (r.slot4.x >= 0) { sr += a[0].x; sz += a[0].y; }
+            if (r.slot4.y >= 0) { sr += a[1].x; sz += a[1].y; }
+            if (r.slot4.z >= 0) { sr += a[2].x; sz += a[2].y; }
+            if (r.slot4.w >= 0) { sr += a[3].x; sz += a[3].y; }
+            if (r.info & 2) {
+                for (int j = d.slotStart[k] + 4, e = d.slotStart[k + 1]; j < e; j++) {
+                    const double2 x = slotIn[d.slotOf[j]];
+                    sr += x.x;
+                    sz += x.y;
+                }
+            }
+            return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
+        };
+        auto update = [&](const PatchNode& r, int j, int k, double m, double2 fk, double2 uk, double2 vk, uint8_t fl) {
+            const bool owned = r.info & 1;
+            if (owned) d.f[k] = fk;
+            if (!next) return;
+            double a3[3] = {0.0, 0.0, 0.0};
+            int cc = 0, cs = 0;
+            nodal_core<WALL>(d, m, r.z0, fl, vk, uk, fk, dt, dtAvg, decay, gain, a3, cc, cs);
+            if (owned) {
+                uOut[k] = uk;
+                vOut[k] = vk;
+                flagsOut[k] = fl;
+                acc[0] += a3[0];
+                acc[1] += a3[1];
+                acc[2] += a3[2];
+                contacts += cc;
+                stuck += cs;
+            }
+            sX0[wave][j] = r.x0;
+            sU[wave][j] = uk;
+            sV[wave][j] = vk;
+        };
+        if (has0) update(r0, lane, k0, m0, gathered(r0, a0, k0, m0), uu0, vv0, fl0);
+        if (has1) update(r1, lane + 64, k1, m1, gathered(r1, a1, k1, m1), uu1, vv1, fl1);
+    }
+    if (!next) {  // the gather that closes the last step; no step follows
+        if (WALL && in.pendingWall && blockIdx.x == 0)
+            reduce_wall_partials(d, d.wallPartialF + (size_t)q * 3 * d.patchBlocks,
+                                 d.wallCountPartialF + (size_t)q * 2 * d.patchBlocks, d.patchBlocks);
+        return;
+    }
+    // The element phase reads only its own wave's nodes: a wave-level fence and barrier publish
+    // the LDS writes to the other lanes.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    // Element phase: step n on the corners in LDS.
+    double edge = NFD_MAX_VALUE;
+    bool sawNaN = false;
+    int iterations = 0;
+    if (patch >= 0) {
+        const int e = d.order[64 * patch + lane];
+        if (e >= 0) {
+            const uchar4 li = d.orderLocal[64 * patch + lane];
+            const Corners cr{sX0[wave][li.x], sX0[wave][li.y], sX0[wave][li.z], sX0[wave][li.w],
+                             sU[wave][li.x], sU[wave][li.y], sU[wave][li.z], sU[wave][li.w],
+                             sV[wave][li.x], sV[wave][li.y], sV[wave][li.z], sV[wave][li.w]};
+            edge = FULL ? full_element_at<DAMAGE, KIN>(d, e, cr, slotOut, dt, T, &iterations)
+                        : reduced_element_at<DAMAGE, KIN>(d, e, cr, slotOut, dt, T, nullptr, nullptr, &iterations);
+            sawNaN = edge != edge;
+        }
+        int m = iterations;
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(m, off, 64);
+            m = o > m ? o : m;
+        }
+        if (lane == 0) d.patchCost[(size_t)(q ^ 1) * d.patches + patch] = (uint8_t)(m < 255 ? m : 255);
+    }
+    if (WALL) {
+        // this nodal update's wall audits, per workgroup (fixed-order block sum), for the next launch
+        __shared__ double ldsW[4 * 3];
+        __shared__ int ldsWi[2 * 4];
+        block_sum<3>(acc, ldsW);
+        int a = contacts, bb = stuck;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            bb += __shfl_xor(bb, off, 64);
+        }
+        if (lane == 0) {
+            ldsWi[wave * 2] = a;
+            ldsWi[wave * 2 + 1] = bb;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double* pw = d.wallPartialF + (size_t)(q ^ 1) * 3 * d.patchBlocks + 3 * blockIdx.x;
+            pw[0] = acc[0];
+            pw[1] = acc[1];
+            pw[2] = acc[2];
+            int* pc = d.wallCountPartialF + (size_t)(q ^ 1) * 2 * d.patchBlocks + 2 * blockIdx.x;
+            pc[0] = ldsWi[0] + ldsWi[2] + ldsWi[4] + ldsWi[6];
+            pc[1] = ldsWi[1] + ldsWi[3] + ldsWi[5] + ldsWi[7];
+        }
+    }
+
+    // Shortest edge: exact minimum over non-negative doubles by their bit patterns.
+    unsigned long long bits = sawNaN ? MAX_BITS : (unsigned long long)nfm::bits_of(edge);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o < bits ? o : bits;
+    }
+    const unsigned long long anyNaN = __ballot(sawNaN);
+    __shared__ unsigned long long ldsMin[BLOCK / 64];
+    __shared__ int ldsNaN;
+    if (threadIdx.x == 0) ldsNaN = 0;
+    __syncthreads();
+    if (lane == 0) {
+        ldsMin[wave] = bits;
+        if (anyNaN) ldsNaN = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = ldsMin[0];
+        for (int w = 1; w < BLOCK / 64; w++) m = ldsMin[w] < m ? ldsMin[w] : m;
+        if (m != MAX_BITS) atomicMin(&c->fMin[next3], m);
+        if (ldsNaN) atomicOr(&c->fNaN[next3], 1);
+    }
+    if (WALL && in.pendingWall && blockIdx.x == 0)
+        reduce_wall_partials(d, d.wallPartialF + (size_t)q * 3 * d.patchBlocks,
+                             d.wallCountPartialF + (size_t)q * 2 * d.patchBlocks, d.patchBlocks);
+}
+
+// Closes a run of fused steps: adds the last launch's wall partials to the totals and, if the
+// current state is in buffer 1, copies node state and force slots back to buffer 0, where
+// every other kernel and the host read them. Reads Step q; the k_gn that follows writes
+// Step q ^ 1 with buf = 0.
+template <bool WALL>
+__global__ void __launch_bounds__(BLOCK) k_canon(Dev d, int q) {
+    const Step& in = d.ctl->s[q];
+    if (WALL && in.pendingWall && blockIdx.x == 0)
+        reduce_wall_partials(d, d.wallPartialF + (size_t)q * 3 * d.patchBlocks,
+                             d.wallCountPartialF + (size_t)q * 2 * d.patchBlocks, d.patchBlocks);
+    if (!in.buf) return;
+    const size_t stride = (size_t)gridDim.x * BLOCK;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < (size_t)d.N; i += stride) {
+        d.u[i] = d.uB[i];
+        d.v[i] = d.vB[i];
+        d.flags[i] = d.flagsB[i];
+    }
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < 4 * (size_t)d.E; i += stride) d.slot[i] = d.slotB[i];
 }
 
 // ------------------------------------------------------------------ film frames
@@ -1442,8 +1807,8 @@ struct nf_solver {
     bool profiling = false;
     std::vector<hipEvent_t> evPool;
     std::vector<std::pair<int, std::pair<int, int>>> evPending;  // kernel id, (start, stop)
-    double kernelMs[3] = {0.0, 0.0, 0.0};
-    long long kernelCount[3] = {0, 0, 0};
+    double kernelMs[4] = {0.0, 0.0, 0.0, 0.0};
+    long long kernelCount[4] = {0, 0, 0, 0};
     // device buffers owned here
     std::vector<void*> owned;
     double* densityOfElement = nullptr;
@@ -1575,12 +1940,13 @@ struct nf_solver {
 
     template <bool WALL>
     int launchGNT(int phase, int parity, int gatherFirst, int part, hipEvent_t a, hipEvent_t b) {
+        const int minSlot = gnMinSlot;
         // the gather before the contact launches carries the contact's segment rebuild
         // one workgroup more for the element pass's schedule (k_gn)
         const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0) +
                          (d.order && !d.gone && part != PART_REST ? 1 : 0);
-        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part);
-        else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part);
+        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part, minSlot);
+        else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part, minSlot);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
@@ -1638,9 +2004,98 @@ struct nf_solver {
             q ^= 1;
             return NF_OK;
         }
+        if (fusedRun) {
+            // k_element, then k_step for each later step; closeFused adds k_canon and k_gn.
+            if (!openElement) {
+                TRY(timed(1, [&](hipEvent_t a, hipEvent_t b) { return launchElement(MODE_STEP, q, a, b); }));
+                openElement = true;
+                lastK3 = -1;
+                return NF_OK;
+            }
+            const int k3 = (int)(fusedCount % 3);
+            TRY(timed(3, [&](hipEvent_t a, hipEvent_t b) { return launchStep(q, k3, lastK3 < 0 ? 1 : 0, a, b); }));
+            lastK3 = k3;
+            fusedCount++;
+            q ^= 1;
+            return NF_OK;
+        }
         TRY(timed(1, [&](hipEvent_t a, hipEvent_t b) { return launchElement(MODE_STEP, q, a, b); }));
         TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_STEP, q, 0, a, b); }));
         q ^= 1;
+        return NF_OK;
+    }
+
+    // ---- the fused step (k_step): one launch per step without contact, erosion, pressure edges
+    bool fusedRun = false;      // this run call enqueues fused steps
+    bool openElement = false;   // an element pass is enqueued whose closing k_gn is not
+    int lastK3 = -1;            // rotation index of the last k_step since that element pass
+    long long fusedCount = 0;   // k_step launches so far (their fMin rotation)
+    int gnMinSlot = -1;         // k_gn's minSlot argument for the next node launch
+    bool fusedDisabled = false; // NEOFIZ_FUSED=0
+
+    bool fusedEligible() const {
+        return d.pnStart && !fusedDisabled && !useGraphs && !tangent && !adjoint && !hasContact &&
+               !d.gone && edges == 0;
+    }
+
+    int fusedBuffers() {
+        if (d.uB) return NF_OK;
+        TRY(alloc(&d.uB, N));
+        TRY(alloc(&d.vB, N));
+        TRY(alloc(&d.flagsB, N));
+        TRY(alloc(&d.slotB, 4 * (size_t)E));
+        TRY(alloc(&d.wallPartialF, 2 * 3 * (size_t)d.patchBlocks));
+        TRY(alloc(&d.wallCountPartialF, 2 * 2 * (size_t)d.patchBlocks));
+        return NF_OK;
+    }
+
+    template <bool FULL, bool DAMAGE, bool KIN, bool WALL>
+    int launchStepT(int parity, int k3, int minFromStep, hipEvent_t a, hipEvent_t b) {
+        const int grid = d.patchBlocks + 1;  // the last workgroup remakes the schedule
+        if (a) hipExtLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, k3, minFromStep);
+        else hipLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, k3, minFromStep);
+        HIP_TRY(hipGetLastError());
+        return NF_OK;
+    }
+
+    int launchStep(int parity, int k3, int minFromStep, hipEvent_t a, hipEvent_t b) {
+        switch ((integration == NF_FULL ? 8 : 0) | (hasDamage ? 4 : 0) | (hasKin ? 2 : 0) | (hasWall ? 1 : 0)) {
+            case 0: return launchStepT<false, false, false, false>(parity, k3, minFromStep, a, b);
+            case 1: return launchStepT<false, false, false, true>(parity, k3, minFromStep, a, b);
+            case 2: return launchStepT<false, false, true, false>(parity, k3, minFromStep, a, b);
+            case 3: return launchStepT<false, false, true, true>(parity, k3, minFromStep, a, b);
+            case 4: return launchStepT<false, true, false, false>(parity, k3, minFromStep, a, b);
+            case 5: return launchStepT<false, true, false, true>(parity, k3, minFromStep, a, b);
+            case 6: return launchStepT<false, true, true, false>(parity, k3, minFromStep, a, b);
+            case 7: return launchStepT<false, true, true, true>(parity, k3, minFromStep, a, b);
+            case 8: return launchStepT<true, false, false, false>(parity, k3, minFromStep, a, b);
+            case 9: return launchStepT<true, false, false, true>(parity, k3, minFromStep, a, b);
+            case 10: return launchStepT<true, false, true, false>(parity, k3, minFromStep, a, b);
+            case 11: return launchStepT<true, false, true, true>(parity, k3, minFromStep, a, b);
+            case 12: return launchStepT<true, true, false, false>(parity, k3, minFromStep, a, b);
+            case 13: return launchStepT<true, true, false, true>(parity, k3, minFromStep, a, b);
+            case 14: return launchStepT<true, true, true, false>(parity, k3, minFromStep, a, b);
+            default: return launchStepT<true, true, true, true>(parity, k3, minFromStep, a, b);
+        }
+    }
+
+    // Ends a run of fused steps: the last element pass's gather (k_gn), after k_canon has
+    // brought the state back to buffer 0 if any k_step ran.
+    int closeFused() {
+        if (!openElement) return NF_OK;
+        if (lastK3 >= 0) {
+            const int grid = std::min(blocks_for(4 * E > N ? 4 * E : N), 1024);
+            if (hasWall) hipLaunchKernelGGL(k_canon<true>, dim3(grid), dim3(BLOCK), 0, stream, d, q);
+            else hipLaunchKernelGGL(k_canon<false>, dim3(grid), dim3(BLOCK), 0, stream, d, q);
+            HIP_TRY(hipGetLastError());
+        }
+        gnMinSlot = lastK3 >= 0 ? (lastK3 + 1) % 3 : -1;
+        const int rc = timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_STEP, q, 0, a, b); });
+        gnMinSlot = -1;
+        TRY(rc);
+        q ^= 1;
+        openElement = false;
+        lastK3 = -1;
         return NF_OK;
     }
 
@@ -1968,6 +2423,10 @@ struct nf_solver {
             TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 0, a, b); }));
         }
         q ^= 1;
+        openElement = false;
+        lastK3 = -1;
+        fusedRun = fusedEligible();
+        if (fusedRun) TRY(fusedBuffers());
         if (maxSteps > 0) {
             // A known count: enqueue every step, synchronise once at the end.
             for (long long left = maxSteps; left > 0;) {
@@ -1998,6 +2457,8 @@ struct nf_solver {
                 slot ^= 1;
             }
         }
+        TRY(closeFused());
+        fusedRun = false;
         Ctl full;
         TRY(readCtl(full));
         h = full.s[q];
@@ -2074,6 +2535,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
     };
     int rc;
     if (const char* g = std::getenv("NEOFIZ_GRAPHS")) s->useGraphs = g[0] == '1';
+    if (const char* g = std::getenv("NEOFIZ_FUSED")) s->fusedDisabled = g[0] == '0';
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(NF_E_DEVICE, "hipStreamCreate failed: no usable GPU"));
     const int N = node_count, E = element_count;
@@ -2233,6 +2695,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         h.s[k].dt = cfl_safety * minEdge / s->fastestWave;
         h.s[k].dtPrev = h.s[k].dt;
         h.s[k].minEdgeBits = MAX_BITS;
+        h.fMin[k] = MAX_BITS;
     }
     s->dt0 = h.s[0].dt;
     if (E > 0) {
@@ -2258,11 +2721,12 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         int* po = nullptr;
         uint8_t* pc = nullptr;
         if ((rc = s->alloc(&o, padded.size())) || (rc = s->alloc(&oc, corners.size())) ||
-            (rc = s->alloc(&po, first.size())) || (rc = s->alloc(&pc, P)))
+            (rc = s->alloc(&po, 2 * first.size())) || (rc = s->alloc(&pc, 2 * (size_t)P)))
             return bail(rc);
         if (hipMemcpy(o, padded.data(), sizeof(int) * padded.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(oc, corners.data(), sizeof(int4) * corners.size(), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(po, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(po, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(po + first.size(), first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(NF_E_DEVICE, "upload failed"));
         s->d.order = o;
         s->d.orderConn = oc;
@@ -2271,6 +2735,69 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         s->d.patches = P;
         s->d.patchBlocks = nb;
         s->d.units = units;
+
+        // The fused step kernel's node lists: every patch's corner nodes, each owned by the patch
+        // that holds its lowest-numbered element. A node in no element, or a patch with more than
+        // STEP_PATCH_NODES nodes, leaves the solver on the two-launch step.
+        std::vector<int> firstElem(N, -1);
+        for (int e = E - 1; e >= 0; e--)
+            for (int k = 0; k < 4; k++) firstElem[conn[4 * (size_t)e + k]] = e;
+        std::vector<int> patchOfElem(E);
+        for (size_t k = 0; k < ord.size(); k++) patchOfElem[ord[k]] = (int)(k / 64);
+        bool fusable = true;
+        for (int i = 0; i < N && fusable; i++) fusable = firstElem[i] >= 0;
+        std::vector<int> pnStart(P + 1, 0);
+        std::vector<PatchNode> pnRec;
+        std::vector<uchar4> local(padded.size(), make_uchar4(0, 0, 0, 0));
+        for (int p = 0; p < P && fusable; p++) {
+            std::vector<int> nodes;
+            for (int l = 0; l < 64; l++) {
+                const int e = padded[64 * (size_t)p + l];
+                if (e >= 0)
+                    for (int k = 0; k < 4; k++) nodes.push_back(conn[4 * (size_t)e + k]);
+            }
+            std::sort(nodes.begin(), nodes.end());
+            nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+            if ((int)nodes.size() > STEP_PATCH_NODES) {
+                fusable = false;
+                break;
+            }
+            for (int l = 0; l < 64; l++) {
+                const int e = padded[64 * (size_t)p + l];
+                if (e < 0) continue;
+                unsigned char li[4];
+                for (int k = 0; k < 4; k++)
+                    li[k] = (unsigned char)(std::lower_bound(nodes.begin(), nodes.end(), conn[4 * (size_t)e + k]) - nodes.begin());
+                local[64 * (size_t)p + l] = make_uchar4(li[0], li[1], li[2], li[3]);
+            }
+            for (int node : nodes) {
+                PatchNode rec;
+                std::memset(&rec, 0, sizeof rec);
+                rec.x0 = make_double2(r[node], z[node]);
+                rec.z0 = z[node];
+                rec.slot4 = s4[node];
+                rec.node = node;
+                rec.info = (patchOfElem[firstElem[node]] == p ? 1 : 0) |
+                           (s->slotStart[node + 1] - s->slotStart[node] > 4 ? 2 : 0);
+                pnRec.push_back(rec);
+            }
+            pnStart[p + 1] = (int)pnRec.size();
+        }
+        if (fusable) {
+            int* ps = nullptr;
+            PatchNode* pn = nullptr;
+            uchar4* ol = nullptr;
+            if ((rc = s->alloc(&ps, pnStart.size())) || (rc = s->alloc(&pn, pnRec.size())) ||
+                (rc = s->alloc(&ol, local.size())))
+                return bail(rc);
+            if (hipMemcpy(ps, pnStart.data(), sizeof(int) * pnStart.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(pn, pnRec.data(), sizeof(PatchNode) * pnRec.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(ol, local.data(), sizeof(uchar4) * local.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(NF_E_DEVICE, "upload failed"));
+            s->d.pnStart = ps;
+            s->d.pn = pn;
+            s->d.orderLocal = ol;
+        }
     }
     s->d.nodalBlocks = blocks_for(N);
     s->d.elementBlocks = blocks_for(E + pressure_edge_count);
@@ -3190,10 +3717,10 @@ int nf_get_audits(nf_solver* s, nf_audits* o) {
     return NF_OK;
 }
 
-// Per-kernel timing of the step loop: when enabled, every launch of the nodal, element and
-// gather kernels is bracketed by HIP events on the handle's stream. nf_profile_read returns
-// the summed milliseconds and launch counts (indices 0 nodal, 1 element, 2 gather) and
-// resets them.
+// Per-kernel timing of the step loop: when enabled, every launch of the nodal, element,
+// contact and fused step kernels is bracketed by HIP events on the handle's stream.
+// nf_profile_read returns the summed milliseconds and launch counts (indices 0 nodal,
+// 1 element, 2 contact, 3 fused step) and resets them.
 int nf_profile(nf_solver* s, int32_t enable) {
     g_error.clear();
     TRY(s->harvest());
@@ -3204,7 +3731,7 @@ int nf_profile(nf_solver* s, int32_t enable) {
 int nf_profile_read(nf_solver* s, double* ms, int64_t* counts) {
     g_error.clear();
     TRY(s->harvest());
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
         ms[k] = s->kernelMs[k];
         counts[k] = s->kernelCount[k];
         s->kernelMs[k] = 0.0;
