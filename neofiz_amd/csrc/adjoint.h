@@ -102,7 +102,8 @@ namespace nfa {
 // ------------------------------------------------------------------ recording pass
 
 // Copies what the element pass of step j is about to read, and the nodes' state, into the
-// segment's record (j = 0: the checkpoint's nodes alone). Launched before each recorded pass.
+// segment's record. Launched for j = 0 (the checkpoint's nodes alone); for j >= 1,
+// k_element_rec does the same copies itself.
 __global__ void __launch_bounds__(BLOCK) k_adj_save(Dev d, ARec a, int j, int q) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     const size_t N = d.N, E = d.E;
@@ -133,14 +134,50 @@ __global__ void __launch_bounds__(BLOCK) k_adj_save(Dev d, ARec a, int j, int q)
 
 // k_element for the recorded Taylor configuration, plus the return map's multiplier and
 // outcome and each element's shortest edge. The element arithmetic is reduced_element's.
+// The launch covers max(N, E) threads and also does what k_adj_save (step j) and
+// k_adj_argmin (step j - 1) would do in launches of their own: thread i first copies node i
+// and element i into the record as this pass is about to read them, and, for j > 1, takes
+// part in the first-argmin of step j - 1's shortest edge from the edges that step's pass left
+// in a.edge (which this pass then overwrites, each thread its own element).
 __global__ void __launch_bounds__(BLOCK, 3) k_element_rec(Dev d, ARec a, int j, int q) {
     Ctl* c = d.ctl;
     Step& st = c->s[q];
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    {
+        const size_t N = d.N, E = d.E;
+        if (e < d.N) {
+            a.u[j * N + e] = d.u[e];
+            a.v[j * N + e] = d.v[e];
+            a.f[j * N + e] = d.f[e];
+            a.flags[j * N + e] = d.flags[e];
+        }
+        if (e < d.E) {
+            const size_t b = (size_t)(j - 1);
+            a.sig[(b * 4 + 0) * E + e] = d.sig0[e];
+            a.sig[(b * 4 + 1) * E + e] = d.sig1[e];
+            a.sig[(b * 4 + 2) * E + e] = d.sig2[e];
+            a.sig[(b * 4 + 3) * E + e] = d.sig3[e];
+            a.ep[b * E + e] = d.epsP[e];
+            a.work[b * E + e] = d.work[e];
+            a.hg[(b * 2 + 0) * E + e] = d.hgR[e];
+            a.hg[(b * 2 + 1) * E + e] = d.hgZ[e];
+        }
+        if (e == 0) {
+            a.stepDt[j] = st.dt;
+            a.stepDtPrev[j] = st.dtPrev;
+            a.argmin[j] = 0x7FFFFFFF;
+        }
+        if (j > 1) {  // step j - 1 read Step q ^ 1, which still holds its shortest edge
+            const Step& sp = c->s[q ^ 1];
+            if (e == 0) a.minEdgeBits[j - 1] = sp.nanEdge ? MAX_BITS : sp.minEdgeBits;
+            if (e < d.E && !sp.nanEdge && (unsigned long long)nfm::bits_of(a.edge[e]) == sp.minEdgeBits)
+                atomicMin(&a.argmin[j - 1], e);
+        }
+    }
     if (!st.go) return;
     __shared__ double sLog[97][3];
     __shared__ double sExp[64][2];
     const nfm::Tables T = nfd::stage_tables(sLog, sExp);
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
     double edge = NFD_MAX_VALUE;
     bool sawNaN = false;
     if (e < d.E) {
@@ -179,7 +216,8 @@ __global__ void __launch_bounds__(BLOCK, 3) k_element_rec(Dev d, ARec a, int j, 
 }
 
 // The step's shortest squared edge and the first element holding it (the CFL step's
-// derivative is that element's), after the element pass of step j.
+// derivative is that element's), after the element pass of step j: launched for a segment's
+// last step (k_element_rec of step j + 1 does it for the others).
 __global__ void __launch_bounds__(BLOCK) k_adj_argmin(Dev d, ARec a, int j, int q) {
     const Step& st = d.ctl->s[q];
     const int e = blockIdx.x * BLOCK + threadIdx.x;

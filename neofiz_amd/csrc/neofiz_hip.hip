@@ -2295,13 +2295,14 @@ struct nf_solver {
             TRY(launchGN(PHASE_START, q, 0));
         }
         q ^= 1;
+        // Two launches a step: k_element_rec also saves the step's record and takes the
+        // previous step's argmin; the last step's argmin follows the loop.
         for (int j = 1; j <= count; j++) {
-            hipLaunchKernelGGL(nfa::k_adj_save, dim3(gSave), dim3(BLOCK), 0, stream, d, a, j, q);
-            hipLaunchKernelGGL(nfa::k_element_rec, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, a, j, q);
-            hipLaunchKernelGGL(nfa::k_adj_argmin, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, a, j, q);
+            hipLaunchKernelGGL(nfa::k_element_rec, dim3(gSave), dim3(BLOCK), 0, stream, d, a, j, q);
             TRY(launchGN(PHASE_STEP, q, 0));
             q ^= 1;
         }
+        if (count > 0) hipLaunchKernelGGL(nfa::k_adj_argmin, dim3(blocks_for(E)), dim3(BLOCK), 0, stream, d, a, count, q ^ 1);
         HIP_TRY(hipGetLastError());
         hDt.assign(count + 1, 0.0);
         hDtPrev.assign(count + 1, 0.0);
