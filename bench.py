@@ -186,6 +186,17 @@ def ensemble_leg(count, cap, per_size):
                    for n, mb, b in zip(hit, m2, s2))
         for b in s2:
             b.shutdown()
+        # The persistent form of the same batch: 32 steps per cooperative launch with
+        # grid-wide barriers, against the graph-captured launches above.
+        os.environ["NEOFIZ_BATCH_PERSISTENT"] = "1"
+        try:
+            m3, s3, sec3, _ = ensemble.run_batched(sub, taylor.ROD_DURATION, 0)
+        finally:
+            del os.environ["NEOFIZ_BATCH_PERSISTENT"]
+        same3 = all(b.steps() == steps[n] and ensemble.results(mb, b) == ensemble.results(meshes[n], solvers[n])
+                    for n, mb, b in zip(hit, m3, s3))
+        for b in s3:
+            b.shutdown()
         per_hit, est_hit, sizes_hit = one_by_one(hit)
         work_hit = sum(meshes[n].elementCount * steps[n] for n in hit)
         out["reached"] = {"instances": len(hit), "element_steps": work_hit,
@@ -193,7 +204,9 @@ def ensemble_leg(count, cap, per_size):
                           "one_by_one_seconds_estimated": est_hit,
                           "one_by_one_us_per_step": {str(e): v * 1e6 for e, v in per_hit.items()},
                           "instances_per_size": {str(k): v for k, v in sizes_hit.items()},
-                          "speedup": est_hit / sec2, "same_results_as_full_batch": same}
+                          "speedup": est_hit / sec2, "same_results_as_full_batch": same,
+                          "persistent_launch": {"batched_seconds": sec3, "same_results": same3,
+                                                "graph_over_persistent": sec3 / sec2}}
     out["solo_equals_batched"] = all(sample_checks)
     for s_ in solvers:
         s_.shutdown()

@@ -66,8 +66,8 @@ __device__ __forceinline__ Dev batch_dev(const Dev& d, const BInst& I) {
 // This thread's tile (-1 when its slot of the workgroup is unused) and lane in it. The
 // tile index is uniform across a wave, so it is moved to a scalar register and the tile,
 // instance and step loads below it are scalar loads.
-__device__ __forceinline__ int batch_tile(const BBlock* blocks, int& lane) {
-    const BBlock bb = blocks[blockIdx.x];
+__device__ __forceinline__ int batch_tile(const BBlock* blocks, int& lane, int block) {
+    const BBlock bb = blocks[block];
     const int slot = (int)threadIdx.x / bb.ts;
     lane = (int)threadIdx.x - slot * bb.ts;
     const int t = slot < bb.tiles ? bb.firstTile + slot : -1;
@@ -77,14 +77,13 @@ __device__ __forceinline__ int batch_tile(const BBlock* blocks, int& lane) {
 // The element pass over element tiles (k_element's MODE_STEP / MODE_INITIAL without damage,
 // erosion or pressure edges). MODE_INITIAL runs only for instances that have not stepped
 // and are about to.
+// One workgroup's share, element block `block` of the tile table (the tables T staged).
 template <bool FULL, bool KIN>
-__global__ void __launch_bounds__(BLOCK, FULL ? 2 : 3)
-k_b_element(Dev d, const BInst* insts, const BBlock* blocks, const BTile* tiles, Step* steps, int q, int mode) {
-    __shared__ double sLog[97][3];
-    __shared__ double sExp[64][2];
-    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
+__device__ __forceinline__ void b_element_block(const Dev& d, const BInst* insts, const BBlock* blocks,
+                                                const BTile* tiles, Step* steps, int q, int mode,
+                                                const nfm::Tables& T, int block) {
     int lane;
-    const int ti = batch_tile(blocks, lane);
+    const int ti = batch_tile(blocks, lane, block);
     if (ti < 0) return;
     const BTile t = tiles[ti];
     Step& st = steps[2 * t.inst + q];
@@ -113,15 +112,24 @@ k_b_element(Dev d, const BInst* insts, const BBlock* blocks, const BTile* tiles,
     }
 }
 
+template <bool FULL, bool KIN>
+__global__ void __launch_bounds__(BLOCK, FULL ? 2 : 3)
+k_b_element(Dev d, const BInst* insts, const BBlock* blocks, const BTile* tiles, Step* steps, int q, int mode) {
+    __shared__ double sLog[97][3];
+    __shared__ double sExp[64][2];
+    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
+    b_element_block<FULL, KIN>(d, insts, blocks, tiles, steps, q, mode, T, blockIdx.x);
+}
+
 // The node pass over node tiles: k_gn's PHASE_START and PHASE_STEP per instance, without
 // the wall's audit partials. PHASE_START gathers first exactly for the instances that have
 // not stepped (their zero-interval initial assembly) and starts only instances whose end
 // test passes (the host-side test of a solo run call).
 template <bool WALL>
-__global__ void __launch_bounds__(BLOCK) k_b_gn(Dev d, const BInst* insts, const BBlock* blocks, const BTile* tiles,
-                                               Step* steps, int q, int phase) {
+__device__ __forceinline__ void b_gn_block(const Dev& d, const BInst* insts, const BBlock* blocks, const BTile* tiles,
+                                           Step* steps, int q, int phase, int block) {
     int lane;
-    const int ti = batch_tile(blocks, lane);
+    const int ti = batch_tile(blocks, lane, block);
     if (ti < 0) return;
     const BTile t = tiles[ti];
     const Step& in = steps[2 * t.inst + q];
@@ -173,6 +181,37 @@ __global__ void __launch_bounds__(BLOCK) k_b_gn(Dev d, const BInst* insts, const
     nodal_node<WALL>(dl, i, fi, dt, dtAvg, decay, gain, acc, contacts, stuck);
 }
 
+template <bool WALL>
+__global__ void __launch_bounds__(BLOCK) k_b_gn(Dev d, const BInst* insts, const BBlock* blocks, const BTile* tiles,
+                                               Step* steps, int q, int phase) {
+    b_gn_block<WALL>(d, insts, blocks, tiles, steps, q, phase, blockIdx.x);
+}
+
+// The persistent form: `count` steps of the whole batch in one cooperative launch. Every
+// workgroup walks the element blocks, then the node blocks, with a grid-wide barrier after
+// each walk (cooperative groups' grid sync), so the order of work is the two launches' order
+// and every value is the same; the parity q alternates inside. The loop runs `count` times in
+// every workgroup, and the launch is cooperative (the runtime refuses it unless the whole
+// grid is resident), so every workgroup reaches every barrier.
+template <bool FULL, bool KIN, bool WALL>
+__global__ void __launch_bounds__(BLOCK, FULL ? 2 : 3)
+k_b_persist(Dev d, const BInst* insts, const BBlock* eBlocks, const BTile* eTiles, int eCount,
+            const BBlock* nBlocks, const BTile* nTiles, int nCount, Step* steps, int q, int count) {
+    __shared__ double sLog[97][3];
+    __shared__ double sExp[64][2];
+    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    for (int k = 0; k < count; k++) {
+        for (int b = blockIdx.x; b < eCount; b += gridDim.x)
+            b_element_block<FULL, KIN>(d, insts, eBlocks, eTiles, steps, q, MODE_STEP, T, b);
+        grid.sync();
+        for (int b = blockIdx.x; b < nCount; b += gridDim.x)
+            b_gn_block<WALL>(d, insts, nBlocks, nTiles, steps, q, PHASE_STEP, b);
+        grid.sync();
+        q ^= 1;
+    }
+}
+
 // ------------------------------------------------------------------ host side
 
 struct nf_batch {
@@ -197,6 +236,8 @@ struct nf_batch {
     hipEvent_t statusEvent[2] = {nullptr, nullptr};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool graphDirty = true, useGraphs = true;
+    bool persistent = false;  // NEOFIZ_BATCH_PERSISTENT=1: k_b_persist, one cooperative launch a block
+    int persistGrid = 0;      // workgroups of that launch (all resident)
     static constexpr int GRAPH_STEPS = 32;
 
     ~nf_batch() {
@@ -290,7 +331,42 @@ struct nf_batch {
         q ^= 1;
         return NF_OK;
     }
+    template <bool FULL, bool KIN, bool WALL>
+    int launchPersistT() {
+        auto kernel = k_b_persist<FULL, KIN, WALL>;
+        if (persistGrid == 0) {
+            int dev = 0, perCU = 0;
+            hipDeviceProp_t prop;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(hipGetDeviceProperties(&prop, dev));
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, BLOCK, 0));
+            persistGrid = std::max(1, perCU * prop.multiProcessorCount);
+        }
+        const int grid = std::max(1, std::min(persistGrid, std::max(eBlockCount, nBlockCount)));
+        const BInst* ip = instDev;
+        const BBlock* eb = eBlocks;
+        const BTile* et = eTiles;
+        const BBlock* nb = nBlocks;
+        const BTile* nt = nTiles;
+        int ec = eBlockCount, nc = nBlockCount, qq = q, count = GRAPH_STEPS;
+        void* args[] = {&d, &ip, &eb, &et, &ec, &nb, &nt, &nc, &steps, &qq, &count};
+        HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kernel), dim3(grid), dim3(BLOCK), args, 0, stream));
+        return NF_OK;  // an even number of steps: the parity is unchanged
+    }
+    int launchPersist() {
+        switch ((full ? 4 : 0) | (kin ? 2 : 0) | (wall ? 1 : 0)) {
+            case 0: return launchPersistT<false, false, false>();
+            case 1: return launchPersistT<false, false, true>();
+            case 2: return launchPersistT<false, true, false>();
+            case 3: return launchPersistT<false, true, true>();
+            case 4: return launchPersistT<true, false, false>();
+            case 5: return launchPersistT<true, false, true>();
+            case 6: return launchPersistT<true, true, false>();
+            default: return launchPersistT<true, true, true>();
+        }
+    }
     int launchSteps() {
+        if (persistent) return launchPersist();
         if (!useGraphs) {
             for (int k = 0; k < GRAPH_STEPS; k++) TRY(enqueueStep());
             return NF_OK;

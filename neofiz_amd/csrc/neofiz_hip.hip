@@ -22,6 +22,7 @@
 //   elements  conn[E] int4, matIndex[E], gone[E]
 //   points    sig0..3[P], back0..3[P], epsP[P], work[P]; hgR, hgZ, hgK [E]
 //   slots     slot[4 E + 2 edges] (gr, gz); CSR slotStart[N + 1], slotOf[4 E + 2 edges]
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -3956,6 +3957,7 @@ int nf_batch_create(nf_batch** out, int32_t count, nf_solver* const* solvers) {
     };
     int rc;
     if (const char* g = std::getenv("NEOFIZ_BATCH_GRAPHS")) b->useGraphs = g[0] != '0';
+    if (const char* g = std::getenv("NEOFIZ_BATCH_PERSISTENT")) b->persistent = g[0] == '1';
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(NF_E_DEVICE, "hipStreamCreate failed: no usable GPU"));
     b->full = integration == NF_FULL;

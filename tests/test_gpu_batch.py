@@ -115,3 +115,39 @@ def test_capped_instance_collapses_in_the_oracle_too():
     (dg, tg, ng), (dr, tr, nr_) = traces
     assert np.array_equal(dg.view(np.uint64), dr.view(np.uint64)) and tg == tr and ng == nr_ == 20000
     assert dr[-1] < 0.1 * dr[0] and tr < 80e-6
+
+
+def test_persistent_batch_equals_graph_batch():
+    """The persistent form of the batch (NEOFIZ_BATCH_PERSISTENT=1 at creation: 32 steps of
+    every live instance in one cooperative launch, grid-wide barriers between the element and
+    node walks) against the default graph-captured launches: every instance bit-identical,
+    over mixed mesh sizes and a capped first call."""
+    import os
+    insts = [i for i in ensemble.draw(40, 21) if i.nr <= 32][:8]
+    out = []
+    for persistent in (False, True):
+        old = os.environ.get("NEOFIZ_BATCH_PERSISTENT")
+        os.environ["NEOFIZ_BATCH_PERSISTENT"] = "1" if persistent else "0"
+        try:
+            built = [ensemble.build(i) for i in insts]
+            batch = Batch([s for _, s in built])
+        finally:
+            if old is None:
+                del os.environ["NEOFIZ_BATCH_PERSISTENT"]
+            else:
+                os.environ["NEOFIZ_BATCH_PERSISTENT"] = old
+        batch.runUntil(10e-6, 250)
+        batch.runUntil(10e-6)
+        batch.scatter()
+        batch.close()
+        out.append([(s.steps(), s.time(), {f: s.nodeField(f) for f in FIELDS},
+                     {f: s.pointField(f) for f in ("stress", "plastic_strain", "plastic_work")})
+                    for _, s in built])
+        for _, s in built:
+            s.shutdown()
+    for k, (a, b) in enumerate(zip(*out)):
+        assert a[0] == b[0] and a[1] == b[1], f"instance {k}: steps/time"
+        for f in a[2]:
+            assert first_difference(a[2][f], b[2][f]) is None, f"instance {k}: {f}"
+        for f in a[3]:
+            assert first_difference(a[3][f], b[3][f]) is None, f"instance {k}: {f}"
