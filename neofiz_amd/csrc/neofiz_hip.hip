@@ -108,16 +108,6 @@ struct Ctl {
 
 constexpr unsigned long long MAX_BITS = 0x7FEFFFFFFFFFFFFFULL;  // Double.MAX_VALUE
 
-// One node of a patch in the fused step kernel's node lists: the node's static data in one
-// 64-byte record, read with coalesced loads (info: bit 0 the patch owns the node, bit 1 the node
-// has more than four force slots, the rest on the CSR).
-struct alignas(16) PatchNode {
-    double2 x0;
-    double z0, pad;
-    int4 slot4;
-    int node, info, pad0, pad1;
-};
-
 // Everything a kernel needs, passed by value.
 struct Dev {
     Ctl* ctl;
@@ -141,13 +131,16 @@ struct Dev {
     int* patchOf;         // [2][4 patchBlocks]
     uint8_t* patchCost;   // [2][patches] Newton cost of the patch's slowest lane, last step
     int patches, patchBlocks, units;  // units: the device's compute units
-    // The fused step kernel (k_step). Patch p's nodes are pn[pnStart[p] .. pnStart[p+1])
-    // (ascending node numbers; p owns a node when it holds the node's lowest-numbered element);
-    // orderLocal[64 p + l] are the corners of element order[64 p + l] as indices into that
-    // list. Node state and force slots are double-buffered: buffer 0 is u, v, flags, slot;
-    // buffer 1 is uB, vB, flagsB, slotB.
+    // The fused step kernel (k_step). Patch p's nodes are entries pnStart[p] .. pnStart[p+1]
+    // of pnCode and pnSlot4 (ascending node numbers): pnCode holds the node number in bits
+    // 0-29, bit 30 set when the node has more than four force slots (the rest are on the CSR),
+    // bit 31 set when p owns the node (p holds the node's lowest-numbered element); pnSlot4 the
+    // node's first four slots (slot4). orderLocal[64 p + l] are the corners of element
+    // order[64 p + l] as indices into the patch's list. Node state and force slots are
+    // double-buffered: buffer 0 is u, v, flags, slot; buffer 1 is uB, vB, flagsB, slotB.
     const int* pnStart;
-    const PatchNode* pn;
+    const int* pnCode;
+    const int4* pnSlot4;
     const uchar4* orderLocal;
     double2* uB;
     double2* vB;
@@ -1197,9 +1190,16 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
         count = d.pnStart[patch + 1] - first;
     }
     const bool has0 = lane < count, has1 = lane + 64 < count;
-    PatchNode r0{}, r1{};
-    if (has0) r0 = d.pn[first + lane];
-    if (has1) r1 = d.pn[first + lane + 64];
+    int c0 = 0, c1 = 0;
+    int4 q40 = make_int4(-1, -1, -1, -1), q41 = q40;
+    if (has0) {
+        c0 = d.pnCode[first + lane];
+        q40 = d.pnSlot4[first + lane];
+    }
+    if (has1) {
+        c1 = d.pnCode[first + lane + 64];
+        q41 = d.pnSlot4[first + lane + 64];
+    }
     const nfm::Tables T = nfd::stage_tables(sLog, sExp);
 
     // Node phase: the patch's nodes, at most two per lane, every load of both issued before
@@ -1215,29 +1215,30 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     int contacts = 0, stuck = 0;
     {
         const double2 z2 = make_double2(0.0, 0.0);
-        const int k0 = has0 ? r0.node : 0, k1 = has1 ? r1.node : 0;
+        const int k0 = c0 & 0x3fffffff, k1 = c1 & 0x3fffffff;
         double2 a0[4], a1[4];
-        a0[0] = has0 && r0.slot4.x >= 0 ? slotIn[r0.slot4.x] : z2;
-        a0[1] = has0 && r0.slot4.y >= 0 ? slotIn[r0.slot4.y] : z2;
-        a0[2] = has0 && r0.slot4.z >= 0 ? slotIn[r0.slot4.z] : z2;
-        a0[3] = has0 && r0.slot4.w >= 0 ? slotIn[r0.slot4.w] : z2;
-        a1[0] = has1 && r1.slot4.x >= 0 ? slotIn[r1.slot4.x] : z2;
-        a1[1] = has1 && r1.slot4.y >= 0 ? slotIn[r1.slot4.y] : z2;
-        a1[2] = has1 && r1.slot4.z >= 0 ? slotIn[r1.slot4.z] : z2;
-        a1[3] = has1 && r1.slot4.w >= 0 ? slotIn[r1.slot4.w] : z2;
+        a0[0] = has0 && q40.x >= 0 ? slotIn[q40.x] : z2;
+        a0[1] = has0 && q40.y >= 0 ? slotIn[q40.y] : z2;
+        a0[2] = has0 && q40.z >= 0 ? slotIn[q40.z] : z2;
+        a0[3] = has0 && q40.w >= 0 ? slotIn[q40.w] : z2;
+        a1[0] = has1 && q41.x >= 0 ? slotIn[q41.x] : z2;
+        a1[1] = has1 && q41.y >= 0 ? slotIn[q41.y] : z2;
+        a1[2] = has1 && q41.z >= 0 ? slotIn[q41.z] : z2;
+        a1[3] = has1 && q41.w >= 0 ? slotIn[q41.w] : z2;
         const double m0 = has0 ? d.mass[k0] : 1.0, m1 = has1 ? d.mass[k1] : 1.0;
+        const double2 x00 = has0 ? d.x0[k0] : z2, x01 = has1 ? d.x0[k1] : z2;
         double2 uu0 = has0 ? uIn[k0] : z2, uu1 = has1 ? uIn[k1] : z2;
         double2 vv0 = has0 ? vIn[k0] : z2, vv1 = has1 ? vIn[k1] : z2;
         uint8_t fl0 = has0 ? flagsIn[k0] : 0, fl1 = has1 ? flagsIn[k1] : 0;
         // The gather in the reference's order (solver/ExplicitSolver.java:1147-1165): the first
         // four slots, then any further ones on the CSR, then gravity (gather_slots' sum).
-        auto gathered = [&](const PatchNode& r, const double2 (&a)[4], int k, double m) {
+        auto gathered = [&](int code, const int4& k4, const double2 (&a)[4], int k, double m) {
             double sr = 0.0, sz = 0.0;
-            if (r.slot4.x >= 0) { sr += a[0].x; sz += a[0].y; }
-            if (r.slot4.y >= 0) { sr += a[1].x; sz += a[1].y; }
-            if (r.slot4.z >= 0) { sr += a[2].x; sz += a[2].y; }
-            if (r.slot4.w >= 0) { sr += a[3].x; sz += a[3].y; }
-            if (r.info & 2) {
+            if (k4.x >= 0) { sr += a[0].x; sz += a[0].y; }
+            if (k4.y >= 0) { sr += a[1].x; sz += a[1].y; }
+            if (k4.z >= 0) { sr += a[2].x; sz += a[2].y; }
+            if (k4.w >= 0) { sr += a[3].x; sz += a[3].y; }
+            if (code & 0x40000000) {
                 for (int j = d.slotStart[k] + 4, e = d.slotStart[k + 1]; j < e; j++) {
                     const double2 x = slotIn[d.slotOf[j]];
                     sr += x.x;
@@ -1246,13 +1247,14 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
             }
             return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
         };
-        auto update = [&](const PatchNode& r, int j, int k, double m, double2 fk, double2 uk, double2 vk, uint8_t fl) {
-            const bool owned = r.info & 1;
+        // x0.y is the node's reference axial coordinate, z0.
+        auto update = [&](int code, int j, int k, double m, double2 x0, double2 fk, double2 uk, double2 vk, uint8_t fl) {
+            const bool owned = code < 0;
             if (owned) d.f[k] = fk;
             if (!next) return;
             double a3[3] = {0.0, 0.0, 0.0};
             int cc = 0, cs = 0;
-            nodal_core<WALL>(d, m, r.z0, fl, vk, uk, fk, dt, dtAvg, decay, gain, a3, cc, cs);
+            nodal_core<WALL>(d, m, x0.y, fl, vk, uk, fk, dt, dtAvg, decay, gain, a3, cc, cs);
             if (owned) {
                 uOut[k] = uk;
                 vOut[k] = vk;
@@ -1263,12 +1265,12 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
                 contacts += cc;
                 stuck += cs;
             }
-            sX0[wave][j] = r.x0;
+            sX0[wave][j] = x0;
             sU[wave][j] = uk;
             sV[wave][j] = vk;
         };
-        if (has0) update(r0, lane, k0, m0, gathered(r0, a0, k0, m0), uu0, vv0, fl0);
-        if (has1) update(r1, lane + 64, k1, m1, gathered(r1, a1, k1, m1), uu1, vv1, fl1);
+        if (has0) update(c0, lane, k0, m0, x00, gathered(c0, q40, a0, k0, m0), uu0, vv0, fl0);
+        if (has1) update(c1, lane + 64, k1, m1, x01, gathered(c1, q41, a1, k1, m1), uu1, vv1, fl1);
     }
     if (!next) {  // the gather that closes the last step; no step follows
         if (WALL && in.pendingWall && blockIdx.x == 0)
@@ -2748,8 +2750,8 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         for (size_t k = 0; k < ord.size(); k++) patchOfElem[ord[k]] = (int)(k / 64);
         bool fusable = true;
         for (int i = 0; i < N && fusable; i++) fusable = firstElem[i] >= 0;
-        std::vector<int> pnStart(P + 1, 0);
-        std::vector<PatchNode> pnRec;
+        std::vector<int> pnStart(P + 1, 0), pnCode;
+        std::vector<int4> pnSlot4;
         std::vector<uchar4> local(padded.size(), make_uchar4(0, 0, 0, 0));
         for (int p = 0; p < P && fusable; p++) {
             std::vector<int> nodes;
@@ -2773,31 +2775,29 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
                 local[64 * (size_t)p + l] = make_uchar4(li[0], li[1], li[2], li[3]);
             }
             for (int node : nodes) {
-                PatchNode rec;
-                std::memset(&rec, 0, sizeof rec);
-                rec.x0 = make_double2(r[node], z[node]);
-                rec.z0 = z[node];
-                rec.slot4 = s4[node];
-                rec.node = node;
-                rec.info = (patchOfElem[firstElem[node]] == p ? 1 : 0) |
-                           (s->slotStart[node + 1] - s->slotStart[node] > 4 ? 2 : 0);
-                pnRec.push_back(rec);
+                pnCode.push_back(node | (s->slotStart[node + 1] - s->slotStart[node] > 4 ? 0x40000000 : 0) |
+                                 (patchOfElem[firstElem[node]] == p ? (int)0x80000000u : 0));
+                pnSlot4.push_back(s4[node]);
             }
-            pnStart[p + 1] = (int)pnRec.size();
+            pnStart[p + 1] = (int)pnCode.size();
         }
+        if (fusable && N >= (1 << 30)) fusable = false;
         if (fusable) {
             int* ps = nullptr;
-            PatchNode* pn = nullptr;
+            int* pc = nullptr;
+            int4* p4 = nullptr;
             uchar4* ol = nullptr;
-            if ((rc = s->alloc(&ps, pnStart.size())) || (rc = s->alloc(&pn, pnRec.size())) ||
-                (rc = s->alloc(&ol, local.size())))
+            if ((rc = s->alloc(&ps, pnStart.size())) || (rc = s->alloc(&pc, pnCode.size())) ||
+                (rc = s->alloc(&p4, pnSlot4.size())) || (rc = s->alloc(&ol, local.size())))
                 return bail(rc);
             if (hipMemcpy(ps, pnStart.data(), sizeof(int) * pnStart.size(), hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(pn, pnRec.data(), sizeof(PatchNode) * pnRec.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(pc, pnCode.data(), sizeof(int) * pnCode.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(p4, pnSlot4.data(), sizeof(int4) * pnSlot4.size(), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(ol, local.data(), sizeof(uchar4) * local.size(), hipMemcpyHostToDevice) != hipSuccess)
                 return bail(fail(NF_E_DEVICE, "upload failed"));
             s->d.pnStart = ps;
-            s->d.pn = pn;
+            s->d.pnCode = pc;
+            s->d.pnSlot4 = p4;
             s->d.orderLocal = ol;
         }
     }
