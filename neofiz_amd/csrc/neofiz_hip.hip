@@ -432,7 +432,7 @@ __device__ void schedule_patches(const Dev& d, const uint8_t* __restrict__ costI
 // in Ctl::fMin[minSlot] (and the node state has been brought back to buffer 0, k_canon).
 template <bool WALL>
 __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part, int minSlot) {
-    if (d.order && !d.gone && part != PART_REST && blockIdx.x == gridDim.x - 1) {
+    if (d.order && part != PART_REST && blockIdx.x == gridDim.x - 1) {
         // the workgroup past the others (launchGNT): the schedule of the element launch that
         // reads Step q ^ 1, from the costs of the one that read Step q
         schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)(q ^ 1) * 4 * d.patchBlocks);
@@ -464,11 +464,13 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
     }
     if (part == PART_REST && gathered && blockIdx.x == 0) reduce_contact_partials(d);
     if (phase == PHASE_STEP && d.gone && blockIdx.x == 0) {
-        // eroded energy of step n: the element launch's per-block partials, summed by this
-        // workgroup in a fixed order (strided per thread, then the block tree)
+        // eroded energy of step n: the element launch's per-patch (or, without a schedule,
+        // per-block) partials, summed by this workgroup in a fixed order (strided per thread,
+        // then the block tree)
         __shared__ double ldsE[4 * 1];
         double tot[1] = {0.0};
-        for (int b = threadIdx.x; b < d.elementBlocks; b += BLOCK) tot[0] += d.erodePartial[b];
+        const int parts = d.order ? d.patches : d.elementBlocks;
+        for (int b = threadIdx.x; b < parts; b += BLOCK) tot[0] += d.erodePartial[b];
         block_sum<1>(tot, ldsE);
         if (threadIdx.x == 0) c->erodedEnergy += tot[0];
     }
@@ -997,10 +999,11 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     // Each element's update reads and writes only that element's own state, so which thread
     // takes which element changes no result. With a schedule, the first 256 patchBlocks
-    // threads take patches and the pressure edges follow; erosion's per-block energy sums
-    // are grouped by thread, so with erosion on the threads keep index order. The element and
-    // its corners are read before the table staging's barrier, so that the two overlap.
-    const bool scheduled = d.order && !d.gone;
+    // threads take patches and the pressure edges follow; erosion's energy is summed per
+    // patch (lanes in the patch's own order), so its sum does not depend on the schedule
+    // either. The element and its corners are read before the table staging's barrier, so
+    // that the two overlap.
+    const bool scheduled = d.order != nullptr;
     int e = t, patch = -1, edgeIndex = t - d.E;
     int4 n = make_int4(0, 0, 0, 0);
     if (scheduled) {
@@ -1068,7 +1071,14 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
     if (d.gone && mode == MODE_STEP) {
         __shared__ double ldsE[4 * 1];
         double ev[1] = {eEnergy};
-        block_sum<1>(ev, ldsE);
+        if (scheduled) {
+            // per patch: a fixed shuffle tree over the patch's lanes
+            for (int off = 32; off > 0; off >>= 1) ev[0] += __shfl_xor(ev[0], off, 64);
+            if ((threadIdx.x & 63) == 0 && patch >= 0) d.erodePartial[patch] = ev[0];
+        } else {
+            block_sum<1>(ev, ldsE);
+            if (threadIdx.x == 0) d.erodePartial[blockIdx.x] = ev[0];
+        }
         int a = eCount, b = iCount, sflag = stale ? 1 : 0;
         for (int off = 32; off > 0; off >>= 1) {
             a += __shfl_xor(a, off, 64);
@@ -1084,7 +1094,6 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
             }
             if (sflag) c->surfaceStale = 1;
         }
-        if (threadIdx.x == 0) d.erodePartial[blockIdx.x] = ev[0];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1921,7 +1930,7 @@ struct nf_solver {
     template <bool FULL, bool DAMAGE, bool KIN>
     int launchElementT(int mode, int parity, hipEvent_t a, hipEvent_t b) {
         // scheduled: patch workgroups, then the pressure edges (k_element)
-        const int grid = (d.order && !d.gone) ? d.patchBlocks + blocks_for(edges) : blocks_for(E + edges);
+        const int grid = d.order ? d.patchBlocks + blocks_for(edges) : blocks_for(E + edges);
         if (a) hipExtLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, mode);
         else hipLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
         HIP_TRY(hipGetLastError());
@@ -1947,7 +1956,7 @@ struct nf_solver {
         // the gather before the contact launches carries the contact's segment rebuild
         // one workgroup more for the element pass's schedule (k_gn)
         const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0) +
-                         (d.order && !d.gone && part != PART_REST ? 1 : 0);
+                         (d.order && part != PART_REST ? 1 : 0);
         if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part, minSlot);
         else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part, minSlot);
         HIP_TRY(hipGetLastError());
@@ -2635,7 +2644,7 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         (rc = s->alloc(&slot4, N)) ||
         (rc = s->alloc(&d.wallPartial, 3 * (size_t)blocks_for(N))) ||
         (rc = s->alloc(&d.wallCountPartial, 2 * (size_t)blocks_for(N))) ||
-        (rc = s->alloc(&d.erodePartial, (size_t)blocks_for(E + pressure_edge_count))) ||
+        (rc = s->alloc(&d.erodePartial, std::max((size_t)blocks_for(E + pressure_edge_count), ((size_t)E + 63) / 64))) ||
         (rc = s->alloc(&s->densityOfElement, E)))
         return bail(rc);
     d.ctl = s->ctl;
