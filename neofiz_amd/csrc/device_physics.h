@@ -64,13 +64,16 @@ __device__ __forceinline__ double jc_thermalTerm(const Mat& j, double t, const n
 }
 // strainTerm(e) and thermalTerm(t) together, their two pows side by side
 // (nfm::pow_pair_ordinary). A term that takes no pow gives its pow a stand-in input of 0.5,
-// whose result is not used; each result is the single-term function's own.
+// whose result is not used; each result is the single-term function's own. A wave none of
+// whose points takes a pow (points that never yielded: no plastic strain, no plastic work,
+// so no heating) skips the pows altogether.
 __device__ __forceinline__ void jc_terms(const Mat& j, double e, double t, const nfm::Tables& T,
                                          double& strain, double& thermal) {
     const double homologous = (t - j.troom) / (j.tmelt - j.troom);
     const bool strainPow = !(e <= 0.0), thermalPow = !(homologous <= 0.0) && !(homologous >= 1.0);
-    double ps, pt;
-    nfm::pow_pair_ordinary(strainPow ? e : 0.5, j.n, thermalPow ? homologous : 0.5, j.m, T, ps, pt);
+    double ps = 0.0, pt = 0.0;
+    if (__any(strainPow || thermalPow))
+        nfm::pow_pair_ordinary(strainPow ? e : 0.5, j.n, thermalPow ? homologous : 0.5, j.m, T, ps, pt);
     strain = e <= 0.0 ? j.a : j.a + j.b * ps;
     thermal = homologous <= 0.0 ? 1.0 : (homologous >= 1.0 ? 0.0 : 1.0 - pt);
 }
