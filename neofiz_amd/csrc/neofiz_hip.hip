@@ -1388,6 +1388,28 @@ __global__ void __launch_bounds__(BLOCK) k_canon(Dev d, int q) {
     for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < 4 * (size_t)d.E; i += stride) d.slot[i] = d.slotB[i];
 }
 
+// A checkpoint of the step loop's state (the adjoint's): every array in one launch, regions
+// laid out back to back in `mem`, each rounded up to 16 bytes. save: live -> mem, else back.
+struct CkptRegion {
+    char* live;
+    size_t offset, bytes;
+};
+constexpr int CKPT_REGIONS = 14;
+struct CkptList {
+    CkptRegion r[CKPT_REGIONS];
+};
+__global__ void __launch_bounds__(BLOCK) k_ckpt_copy(CkptList list, char* mem, int save) {
+    const size_t stride = (size_t)gridDim.x * BLOCK, t0 = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    for (int k = 0; k < CKPT_REGIONS; k++) {
+        const CkptRegion r = list.r[k];
+        char* a = save ? mem + r.offset : r.live;
+        const char* b = save ? r.live : mem + r.offset;
+        const size_t words = r.bytes / 16;
+        for (size_t i = t0; i < words; i += stride) ((double2*)a)[i] = ((const double2*)b)[i];
+        for (size_t i = words * 16 + t0; i < r.bytes; i += stride) a[i] = b[i];
+    }
+}
+
 // ------------------------------------------------------------------ film frames
 
 // The element quantities a film frame can carry (enum nf_film_field), at most FILM_MAX.
@@ -2174,27 +2196,32 @@ struct nf_solver {
         return (size_t)N * (3 * sizeof(double2) + 1) + (size_t)E * 9 * sizeof(double) + sizeof(Ctl) + 64;
     }
 
+    // One launch copies every array (k_ckpt_copy); the stream orders it against the steps.
     int checkpointCopy(Checkpoint& c, bool save) {
-        char* p = c.mem;
-        auto move = [&](void* live, size_t bytes) -> int {
-            HIP_TRY(hipMemcpyAsync(save ? (void*)p : live, save ? live : (void*)p, bytes, hipMemcpyDeviceToDevice, stream));
-            p += (bytes + 15) & ~(size_t)15;
-            return NF_OK;
+        CkptList list;
+        size_t at = 0;
+        int k = 0;
+        auto region = [&](void* live, size_t bytes) {
+            list.r[k++] = CkptRegion{(char*)live, at, bytes};
+            at += (bytes + 15) & ~(size_t)15;
         };
-        TRY(move(ctl, sizeof(Ctl)));
-        TRY(move(d.u, sizeof(double2) * N));
-        TRY(move(d.v, sizeof(double2) * N));
-        TRY(move(d.f, sizeof(double2) * N));
-        TRY(move(d.sig0, sizeof(double) * E));
-        TRY(move(d.sig1, sizeof(double) * E));
-        TRY(move(d.sig2, sizeof(double) * E));
-        TRY(move(d.sig3, sizeof(double) * E));
-        TRY(move(d.epsP, sizeof(double) * E));
-        TRY(move(d.work, sizeof(double) * E));
-        TRY(move(d.hgR, sizeof(double) * E));
-        TRY(move(d.hgZ, sizeof(double) * E));
-        TRY(move(d.hgK, sizeof(double) * E));
-        TRY(move(d.flags, N));
+        region(ctl, sizeof(Ctl));
+        region(d.u, sizeof(double2) * N);
+        region(d.v, sizeof(double2) * N);
+        region(d.f, sizeof(double2) * N);
+        region(d.sig0, sizeof(double) * E);
+        region(d.sig1, sizeof(double) * E);
+        region(d.sig2, sizeof(double) * E);
+        region(d.sig3, sizeof(double) * E);
+        region(d.epsP, sizeof(double) * E);
+        region(d.work, sizeof(double) * E);
+        region(d.hgR, sizeof(double) * E);
+        region(d.hgZ, sizeof(double) * E);
+        region(d.hgK, sizeof(double) * E);
+        region(d.flags, N);
+        const int grid = std::min(blocks_for((int)((N > E ? N : E) + 1)), 256);
+        hipLaunchKernelGGL(k_ckpt_copy, dim3(grid), dim3(BLOCK), 0, stream, list, c.mem, save ? 1 : 0);
+        HIP_TRY(hipGetLastError());
         return NF_OK;
     }
 
@@ -2202,14 +2229,13 @@ struct nf_solver {
         if (!c.mem) HIP_TRY(hipMalloc((void**)&c.mem, checkpointBytes() + 16 * 16));
         c.steps = steps;
         c.q = q;
-        TRY(checkpointCopy(c, true));
-        return sync();
+        return checkpointCopy(c, true);
     }
 
     int restoreCheckpoint(Checkpoint& c) {
         TRY(checkpointCopy(c, false));
         q = c.q;
-        return sync();
+        return NF_OK;
     }
 
     // The forward run with checkpoints: the plain steps in chunks that end on multiples of
@@ -2241,7 +2267,8 @@ struct nf_solver {
         if (taken) *taken = total;
         Step e;
         TRY(readStep(e));
-        return saveCheckpoint(finalState, e.steps);
+        TRY(saveCheckpoint(finalState, e.steps));
+        return sync();
     }
 
     int adjointAllocate() {
@@ -2401,7 +2428,8 @@ struct nf_solver {
         for (int w = 0; w < W; w++)
             for (int k = 0; k < NP; k++) grad[w * NP + k] = h.grad[w][k];
         // leave the solver as the forward run left it
-        return restoreCheckpoint(finalState);
+        TRY(restoreCheckpoint(finalState));
+        return sync();
     }
 
     // run steps while time < endTime and fewer than maxSteps have been taken.
