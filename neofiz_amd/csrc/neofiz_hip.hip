@@ -351,7 +351,7 @@ __device__ __forceinline__ double2 gather_node(const Dev& d, int n) { return gat
 
 #include "device_contact.h"
 
-constexpr int PHASE_STEP = 0, PHASE_START = 1, PHASE_ASSEMBLE_END = 2;
+constexpr int PHASE_STEP = 0, PHASE_START = 1, PHASE_ASSEMBLE_END = 2, PHASE_RESUME = 3;
 // With penalty contact the launch is split around the contact launches: PART_GATHER only
 // gathers, PART_REST does everything after the gather.
 constexpr int PART_ALL = 0, PART_GATHER = 1, PART_REST = 2;
@@ -363,7 +363,9 @@ constexpr int PART_ALL = 0, PART_GATHER = 1, PART_REST = 2;
 //                while-time-below-end, and if the next step runs, integrate every node;
 //   PHASE_START  the first nodal update of a run call (gathering first after the initial
 //                zero-interval assembly when gatherFirst is set);
-//   PHASE_ASSEMBLE_END  gather only, after advance().
+//   PHASE_ASSEMBLE_END  gather only, after advance();
+//   PHASE_RESUME  PHASE_START without a gather, if the step's end test passes (the adjoint's
+//                forward enqueues its chunks ahead of knowing whether the run has ended).
 // Wave slot of the patch ranked r (0 = costliest) in a grid of patchBlocks workgroups of
 // four waves: ranks fill workgroups four at a time, and the workgroups are dealt to the compute
 // units in rounds that alternate direction, so that a unit holding one of the costliest
@@ -487,6 +489,9 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
         next = in.time < in.endTime && in.steps < in.stepLimit;
     } else if (phase == PHASE_ASSEMBLE_END) {
         next = false;
+    } else if (phase == PHASE_RESUME) {
+        // PHASE_START whose end test the device makes (a chunk the host enqueued ahead)
+        next = in.time < in.endTime && in.steps < in.stepLimit;
     }
     if (lead) {
         Step o = in;
@@ -1388,6 +1393,12 @@ __global__ void __launch_bounds__(BLOCK) k_canon(Dev d, int q) {
     for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < 4 * (size_t)d.E; i += stride) d.slot[i] = d.slotB[i];
 }
 
+// Sets the current step control's end time and step limit on the device (no host round trip).
+__global__ void k_set_limits(Ctl* c, int q, double endTime, long long stepLimit) {
+    c->s[q].endTime = endTime;
+    c->s[q].stepLimit = stepLimit;
+}
+
 // A checkpoint of the step loop's state (the adjoint's): every array in one launch, regions
 // laid out back to back in `mem`, each rounded up to 16 bytes. save: live -> mem, else back.
 struct CkptRegion {
@@ -2244,30 +2255,75 @@ struct nf_solver {
         Step h;
         TRY(readStep(h));
         const long long start = h.steps;
-        long long total = 0;
-        const long long limit = maxSteps > 0 ? maxSteps : 0x7FFFFFFFFFFFFFFFLL;
-        adjoint = false;  // the chunks run the plain path
-        int rc = NF_OK;
-        while (rc == NF_OK) {
-            const long long cur = start + total;
-            if (cur % adjK == 0 && (checkpoints.empty() || checkpoints.back().steps < cur)) {
-                checkpoints.emplace_back();
-                rc = saveCheckpoint(checkpoints.back(), cur);
-                if (rc) break;
-            }
-            const long long chunk = std::min<long long>(adjK - cur % adjK, limit - total);
-            if (chunk <= 0) break;
-            long long t = 0;
-            rc = runSteps(endTime, chunk, &t);
-            total += t;
-            if (t < chunk) break;
+        if (taken) *taken = 0;
+        const long long limitAbs = maxSteps > 0 ? start + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
+        if (h.time < endTime && maxSteps != 0) {
+            TRY(adjointForward(endTime, start, limitAbs, h.steps == 0));
         }
-        adjoint = true;
-        if (rc) return rc;
-        if (taken) *taken = total;
         Step e;
         TRY(readStep(e));
+        // Checkpoints enqueued past the run's end hold the final state under a step count it
+        // never reached: drop them.
+        while (!checkpoints.empty() && checkpoints.back().steps > e.steps) {
+            if (checkpoints.back().mem) HIP_TRY(hipFree(checkpoints.back().mem));
+            checkpoints.pop_back();
+        }
+        if (taken) *taken = e.steps - start;
         TRY(saveCheckpoint(finalState, e.steps));
+        return sync();
+    }
+
+    // The chunks of the forward run, enqueued one ahead of the status the host reads: each
+    // chunk is a checkpoint of the state it starts from (on a multiple of adjK), a launch that
+    // sets the chunk's step limit on the device, the node launch that opens it (PHASE_RESUME:
+    // a no-op once the run has ended), its steps, and the launches that close it.
+    int adjointForward(double endTime, long long start, long long limitAbs, bool initial) {
+        adjoint = false;  // the chunks run the plain (fused) steps
+        struct Restore {
+            nf_solver* s;
+            ~Restore() { s->adjoint = true; s->fusedRun = false; }
+        } restore{this};
+        openElement = false;
+        lastK3 = -1;
+        fusedRun = fusedEligible();
+        if (fusedRun) TRY(fusedBuffers());
+        if (!statusHost) {
+            HIP_TRY(hipHostMalloc((void**)&statusHost, 2 * sizeof(Step), hipHostMallocDefault));
+            HIP_TRY(hipEventCreateWithFlags(&statusEvent[0], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&statusEvent[1], hipEventDisableTiming));
+        }
+        long long cur = start, pendingEnd = 0;
+        int slot = 0;
+        bool pending = false;
+        while (true) {
+            if (cur % adjK == 0 && (checkpoints.empty() || checkpoints.back().steps < cur)) {
+                checkpoints.emplace_back();
+                TRY(saveCheckpoint(checkpoints.back(), cur));
+            }
+            const long long chunkEnd = std::min<long long>(cur - cur % adjK + adjK, limitAbs);
+            if (chunkEnd <= cur) break;
+            hipLaunchKernelGGL(k_set_limits, dim3(1), dim3(1), 0, stream, ctl, q, endTime, chunkEnd);
+            if (initial && cur == start) {
+                TRY(launchElement(MODE_INITIAL, q));
+                TRY(launchGN(PHASE_START, q, 1));
+            } else {
+                TRY(launchGN(PHASE_RESUME, q, 0));
+            }
+            q ^= 1;
+            for (long long k = cur; k < chunkEnd; k++) TRY(enqueueStep());
+            TRY(closeFused());
+            HIP_TRY(hipMemcpyAsync(&statusHost[slot], &ctl->s[q], sizeof(Step), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipEventRecord(statusEvent[slot], stream));
+            if (pending) {
+                HIP_TRY(hipEventSynchronize(statusEvent[slot ^ 1]));
+                const Step& st = statusHost[slot ^ 1];
+                if (st.steps < pendingEnd || !(st.time < endTime) || st.steps >= limitAbs) break;
+            }
+            pending = true;
+            pendingEnd = chunkEnd;
+            slot ^= 1;
+            cur = chunkEnd;
+        }
         return sync();
     }
 
