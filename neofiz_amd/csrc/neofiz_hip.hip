@@ -104,6 +104,8 @@ struct Ctl {
     // slot is reset while another workgroup of the same launch reads or lowers it.
     unsigned long long fMin[3];
     int fNaN[3];
+    // The step count at the start of the current run call (k_open_run), for its step tally.
+    long long runStart;
 };
 
 constexpr unsigned long long MAX_BITS = 0x7FEFFFFFFFFFFFFFULL;  // Double.MAX_VALUE
@@ -1399,6 +1401,18 @@ __global__ void k_set_limits(Ctl* c, int q, double endTime, long long stepLimit)
     c->s[q].stepLimit = stepLimit;
 }
 
+// Opens a run call on the device, with no host round trip: clears the term-overflow flag (it
+// reports on one call's steps), records the step count the call starts from, and sets the end
+// time and the step limit, maxSteps past that count (none when maxSteps < 0). The opening node
+// launch that follows (PHASE_RESUME) makes the end test itself.
+__global__ void k_open_run(Ctl* c, int q, double endTime, long long maxSteps) {
+    Step& s = c->s[q];
+    c->cOverflow = 0;
+    c->runStart = s.steps;
+    s.endTime = endTime;
+    s.stepLimit = maxSteps > 0 ? s.steps + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
+}
+
 // A checkpoint of the step loop's state (the adjoint's): every array in one launch, regions
 // laid out back to back in `mem`, each rounded up to 16 bytes. save: live -> mem, else back.
 struct CkptRegion {
@@ -1868,6 +1882,7 @@ struct nf_solver {
         for (hipEvent_t e : statusEvent)
             if (e) (void)hipEventDestroy(e);
         if (statusHost) (void)hipHostFree(statusHost);
+        if (ctlHost) (void)hipHostFree(ctlHost);
         for (Checkpoint& c : checkpoints)
             if (c.mem) (void)hipFree(c.mem);
         if (finalState.mem) (void)hipFree(finalState.mem);
@@ -1931,13 +1946,18 @@ struct nf_solver {
         return NF_OK;
     }
 
+    // The control block's host copy goes through pinned memory (ctlHost), which the copy
+    // engine reads and writes directly, without a staging copy.
     int readCtl(Ctl& out) {
-        HIP_TRY(hipMemcpyAsync(&out, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+        if (!ctlHost) HIP_TRY(hipHostMalloc((void**)&ctlHost, sizeof(Ctl), hipHostMallocDefault));
+        HIP_TRY(hipMemcpyAsync(ctlHost, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        out = *ctlHost;
         return NF_OK;
     }
 
     int writeCtl(const Ctl& in) {
+        fastOpen = false;
         HIP_TRY(hipMemcpyAsync(ctl, &in, sizeof(Ctl), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         return NF_OK;
@@ -1952,6 +1972,7 @@ struct nf_solver {
     }
 
     int writeStep(const Step& in) {
+        fastOpen = false;
         HIP_TRY(hipMemcpyAsync(&ctl->s[q], &in, sizeof(Step), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         return NF_OK;
@@ -2149,6 +2170,10 @@ struct nf_solver {
     static constexpr int GRAPH_STEPS = 32;
     hipGraphExec_t graphExec[2] = {nullptr, nullptr};
     Step* statusHost = nullptr;  // pinned
+    Ctl* ctlHost = nullptr;      // pinned (readCtl)
+    // The last run call ended past step 0 and nothing has written the control block from the
+    // host since: the next call opens on the device (k_open_run), with no host round trip.
+    bool fastOpen = false;
     hipEvent_t statusEvent[2] = {nullptr, nullptr};
     bool graphDirty = true;
     bool useGraphs = false;  // NEOFIZ_GRAPHS=1 captures blocks of steps in hipGraphs instead
@@ -2244,6 +2269,7 @@ struct nf_solver {
     }
 
     int restoreCheckpoint(Checkpoint& c) {
+        fastOpen = false;
         TRY(checkpointCopy(c, false));
         q = c.q;
         return NF_OK;
@@ -2493,38 +2519,54 @@ struct nf_solver {
         if (adjoint) return adjointRun(endTime, maxSteps, taken);
         if (tangent && (hasContact || hasDamage || hasErosion))
             return fail(NF_E_STATE, "the sensitivity kernels do not carry contact, damage or erosion");
-        Step h;
-        {
-            // The term-overflow flag reports on this call's steps alone.
-            Ctl full;
-            TRY(readCtl(full));
-            if (full.cOverflow) {
-                full.cOverflow = 0;
-                TRY(writeCtl(full));
-            }
-            h = full.s[q];
-        }
-        const long long start = h.steps;
         if (taken) *taken = 0;
-        if (!(h.time < endTime) || maxSteps == 0) return NF_OK;
-        h.endTime = endTime;
-        h.stepLimit = maxSteps > 0 ? start + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
-        TRY(writeStep(h));
-        // Open the first step of this call.
-        if (tangent) {
-            TRY(launchTangent(h.steps == 0 ? MODE_INITIAL : -1, PHASE_START, h.steps == 0 ? 1 : 0));
-        } else if (h.steps == 0) {
-            TRY(launchElement(MODE_INITIAL, q));  // :945
-            TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 1, a, b); }));
+        long long start = -1;  // known on the host, or -1: recorded by k_open_run
+        const bool fast = fastOpen && !tangent;
+        fastOpen = false;  // set again only by a call that completes
+        if (fast) {
+            // Past step 0 with the control block as the last call left it: the opening is made
+            // on the device. A call whose end has already passed runs as no-op launches.
+            if (maxSteps == 0) return NF_OK;
+            hipLaunchKernelGGL(k_open_run, dim3(1), dim3(1), 0, stream, ctl, q, endTime, maxSteps);
+            TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_RESUME, q, 0, a, b); }));
         } else {
-            TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 0, a, b); }));
+            Step h;
+            {
+                // The term-overflow flag reports on this call's steps alone.
+                Ctl full;
+                TRY(readCtl(full));
+                if (full.cOverflow) {
+                    full.cOverflow = 0;
+                    TRY(writeCtl(full));
+                }
+                h = full.s[q];
+            }
+            start = h.steps;
+            if (!(h.time < endTime) || maxSteps == 0) return NF_OK;
+            h.endTime = endTime;
+            h.stepLimit = maxSteps > 0 ? start + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
+            TRY(writeStep(h));
+            // Open the first step of this call.
+            if (tangent) {
+                TRY(launchTangent(h.steps == 0 ? MODE_INITIAL : -1, PHASE_START, h.steps == 0 ? 1 : 0));
+            } else if (h.steps == 0) {
+                TRY(launchElement(MODE_INITIAL, q));  // :945
+                TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 1, a, b); }));
+            } else {
+                TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 0, a, b); }));
+            }
         }
+        HIP_TRY(hipGetLastError());
         q ^= 1;
         openElement = false;
         lastK3 = -1;
         fusedRun = fusedEligible();
         if (fusedRun) TRY(fusedBuffers());
-        if (maxSteps > 0) {
+        // A step count with no end time (or a short one) is enqueued whole. With an end time the
+        // device's end test decides, and the host keeps only a block of steps ahead of it (below):
+        // enqueuing a large step cap whole would leave every step past the end time as a no-op
+        // launch.
+        if (maxSteps > 0 && (!(endTime < HUGE_VAL) || maxSteps <= 2 * GRAPH_STEPS)) {
             // A known count: enqueue every step, synchronise once at the end.
             for (long long left = maxSteps; left > 0;) {
                 const int n = (int)(left < GRAPH_STEPS ? left : GRAPH_STEPS);
@@ -2532,9 +2574,9 @@ struct nf_solver {
                 left -= n;
             }
         } else {
-            // Open-ended: keep one block of steps in flight beyond the one whose status is
-            // being read, so the GPU never idles while the host decides. Blocks enqueued
-            // after the end test fails run as no-ops on the device.
+            // Up to an end time (and the step limit, if any): keep one block of steps in flight
+            // beyond the one whose status is being read, so the GPU never idles while the host
+            // decides. Blocks enqueued after the end test fails run as no-ops on the device.
             if (!statusHost) {
                 HIP_TRY(hipHostMalloc((void**)&statusHost, 2 * sizeof(Step), hipHostMallocDefault));
                 HIP_TRY(hipEventCreateWithFlags(&statusEvent[0], hipEventDisableTiming));
@@ -2558,9 +2600,10 @@ struct nf_solver {
         fusedRun = false;
         Ctl full;
         TRY(readCtl(full));
-        h = full.s[q];
+        const Step h = full.s[q];
         TRY(harvest());
-        if (taken) *taken = h.steps - start;
+        if (taken) *taken = h.steps - (start >= 0 ? start : full.runStart);
+        fastOpen = !tangent && h.steps > 0;
         if (full.cOverflow)
             return fail(NF_E_STATE, "contact: a node received more than " + std::to_string(d.cMaxTerms) +
                                         " pair terms in one step");

@@ -1,0 +1,90 @@
+"""Run calls in sequence, on the GPU against the oracle.
+
+A run call past step 0, with the control block as the previous call left it, is opened on the
+device (the end time, the step limit and the call's starting step count set by a one-thread
+launch; the opening node launch makes the end test itself), with no host round trip before
+its launches. A call after a host write of the control block (a new rigid wall, mass scaling)
+opens on the host as before. A run up to an end time keeps one block of steps ahead of the
+device's end test rather than enqueuing its whole step cap. These paths must take the same
+steps as the oracle (solver/ExplicitSolver.java:1022-1075, run and runUntil): calls whose end
+has already passed take no step and change nothing, a binding cap takes exactly its count, and
+the state stays bit-identical through every kind of call.
+"""
+import numpy as np
+import pytest
+
+from conftest import first_difference
+from neofiz_amd import taylor
+from neofiz_amd.solver import ExplicitSolver, RigidWall
+from oracle.pyoracle import OracleSolver
+
+pytestmark = pytest.mark.gpu
+
+NODE = ("ur", "uz", "vr", "vz", "fr", "fz", "mass", "on_wall")
+POINT = ("stress", "plastic_strain", "plastic_work")
+
+
+def rod(cls, nr=8, nz=80):
+    _, s, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER, taylor.ROD_SPEED,
+                           nr, nz, friction=taylor.ROD_JC_FRICTION, solver_cls=cls)
+    return s
+
+
+def state(s):
+    out = {f: s.nodeField(f) for f in NODE}
+    out.update({f: s.pointField(f) for f in POINT})
+    st = s.state()
+    out["control"] = np.array([st.time, st.dt, float(st.steps)])
+    return out
+
+
+def same(a, b, label):
+    for k in b:
+        d = first_difference(a[k], b[k])
+        assert d is None, f"{label}: {k}: {d}"
+
+
+def script(s):
+    """A sequence of run calls of every kind; returns the step counts the calls report."""
+    taken = []
+    s.run(3)                                   # from step 0: opened on the host
+    s.run(1)                                   # opened on the device from here on
+    t = s.state().time
+    taken.append(s.runUntil(t))                # end already reached: no step
+    taken.append(s.runUntil(0.5 * t, 1000))    # end in the past, with a cap: no step
+    s.run(0)
+    taken.append(s.runUntil(t + 40 * s.state().dt, 500))  # end time binds, cap of 500
+    taken.append(s.runUntil(1.0, 97))          # cap binds (more than two blocks of 32)
+    taken.append(s.runUntil(1.0, 5))           # a short cap
+    s.setRigidWall(RigidWall.atZ(0.0, 0.2))    # a host write of the control block
+    s.run(7)                                   # opened on the host again
+    taken.append(s.runUntil(s.state().time + 1e-7))  # open-ended, to an end time
+    s.run(2)
+    return taken
+
+
+def test_run_call_sequence_matches_the_oracle():
+    gpu, ref = rod(ExplicitSolver), rod(OracleSolver)
+    tg, tr = script(gpu), script(ref)
+    assert tg == tr, (tg, tr)
+    assert tg[0] == 0 and tg[1] == 0 and tg[3] == 97 and tg[4] == 5
+    same(state(gpu), state(ref), "after the call sequence")
+    gpu.shutdown()
+    ref.shutdown()
+
+
+def test_calls_that_take_no_step_change_nothing():
+    a, b = rod(ExplicitSolver), rod(ExplicitSolver)
+    a.run(30)
+    b.run(30)
+    before = state(a)
+    t = a.state().time
+    assert a.runUntil(t) == 0
+    assert a.runUntil(0.0, 100) == 0
+    a.run(0)
+    same(state(a), before, "no-op calls")
+    a.run(25)
+    b.run(25)
+    same(state(a), state(b), "continued after no-op calls")
+    a.shutdown()
+    b.shutdown()
