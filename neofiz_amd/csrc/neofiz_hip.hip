@@ -141,6 +141,10 @@ struct Dev {
     // order[64 p + l] as indices into the patch's list. Node state and force slots are
     // double-buffered: buffer 0 is u, v, flags, slot; buffer 1 is uB, vB, flagsB, slotB.
     const int* pnStart;
+    // patchRun[s] = {patchOf[s], pnStart[p], pnStart[p+1] - pnStart[p], 0} for p = patchOf[s]
+    // (or {-1, 0, 0, 0}), double-buffered as patchOf: one scalar load gives a fused-step wave
+    // its patch and node list.
+    int4* patchRun;
     const int* pnCode;
     const int4* pnSlot4;
     const uchar4* orderLocal;
@@ -383,7 +387,8 @@ __host__ __device__ __forceinline__ int patch_slot(int r, int patchBlocks, int u
 // (costs in, schedule out): a counting sort of the patches by cost, costliest first, into
 // patch_slot order. One workgroup does it. Patches of equal cost are ranked in the order the
 // LDS atomics give: the schedule decides only which wave takes a patch, never a result.
-__device__ void schedule_patches(const Dev& d, const uint8_t* __restrict__ costIn, int* __restrict__ patchOfOut) {
+__device__ void schedule_patches(const Dev& d, const uint8_t* __restrict__ costIn, int* __restrict__ patchOfOut,
+                                 int4* __restrict__ runOut) {
     constexpr int CHUNK = 4096;  // costs staged in LDS per pass, read 16 at a time
     __shared__ int start[256];
     __shared__ uint4 staged[CHUNK / 16];
@@ -426,7 +431,12 @@ __device__ void schedule_patches(const Dev& d, const uint8_t* __restrict__ costI
         const int n = whole ? (__syncthreads(), P) : stage(from);
         for (int p = threadIdx.x; p < n; p += BLOCK) {
             const int r = atomicAdd(&start[cost[p]], 1);
-            patchOfOut[patch_slot(r, d.patchBlocks, d.units)] = from + p;
+            const int slot = patch_slot(r, d.patchBlocks, d.units);
+            patchOfOut[slot] = from + p;
+            if (runOut) {
+                const int b0 = d.pnStart[from + p], b1 = d.pnStart[from + p + 1];
+                runOut[slot] = make_int4(from + p, b0, b1 - b0, 0);
+            }
         }
     }
 }
@@ -439,7 +449,8 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
     if (d.order && part != PART_REST && blockIdx.x == gridDim.x - 1) {
         // the workgroup past the others (launchGNT): the schedule of the element launch that
         // reads Step q ^ 1, from the costs of the one that read Step q
-        schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)(q ^ 1) * 4 * d.patchBlocks);
+        schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)(q ^ 1) * 4 * d.patchBlocks,
+                         d.patchRun ? d.patchRun + (size_t)(q ^ 1) * 4 * d.patchBlocks : nullptr);
         return;
     }
     Ctl* c = d.ctl;
@@ -1141,7 +1152,8 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
 template <bool FULL, bool DAMAGE, bool KIN, bool WALL>
 __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d, int q, int k3, int minFromStep) {
     if (blockIdx.x == gridDim.x - 1) {
-        schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)q * 4 * d.patchBlocks);
+        schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)q * 4 * d.patchBlocks,
+                         d.patchRun + (size_t)q * 4 * d.patchBlocks);
         return;
     }
     Ctl* c = d.ctl;
@@ -1149,8 +1161,9 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     Step& out = c->s[q ^ 1];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int patch = d.patchOf[(size_t)(q ^ 1) * 4 * d.patchBlocks +
+    const int4 run = d.patchRun[(size_t)(q ^ 1) * 4 * d.patchBlocks +
                                 __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / 64) + wave)];
+    const int patch = run.x;
     const int next3 = k3 == 2 ? 0 : k3 + 1, last3 = k3 == 0 ? 2 : k3 - 1;
     if (!in.go) {  // nothing ran: carry the control over
         if (lead) {
@@ -1200,10 +1213,14 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     // is the only workgroup barrier before the element phase.
     __shared__ double sLog[97][3];
     __shared__ double sExp[64][2];
-    int first = 0, count = 0;
+    const int first = run.y, count = run.z;
+    // The element phase's element and its corners' LDS slots, loaded now so that the point
+    // state's loads (indexed by the element) issue as soon as the node phase is done.
+    int eEl = -1;
+    uchar4 eLi = make_uchar4(0, 0, 0, 0);
     if (patch >= 0) {
-        first = d.pnStart[patch];
-        count = d.pnStart[patch + 1] - first;
+        eEl = d.order[64 * patch + lane];
+        eLi = d.orderLocal[64 * patch + lane];
     }
     const bool has0 = lane < count, has1 = lane + 64 < count;
     int c0 = 0, c1 = 0;
@@ -1305,9 +1322,9 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     bool sawNaN = false;
     int iterations = 0;
     if (patch >= 0) {
-        const int e = d.order[64 * patch + lane];
+        const int e = eEl;
         if (e >= 0) {
-            const uchar4 li = d.orderLocal[64 * patch + lane];
+            const uchar4 li = eLi;
             const Corners cr{sX0[wave][li.x], sX0[wave][li.y], sX0[wave][li.z], sX0[wave][li.w],
                              sU[wave][li.x], sU[wave][li.y], sU[wave][li.z], sU[wave][li.w],
                              sV[wave][li.x], sV[wave][li.y], sV[wave][li.z], sV[wave][li.w]};
@@ -2931,6 +2948,16 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
                 hipMemcpy(p4, pnSlot4.data(), sizeof(int4) * pnSlot4.size(), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(ol, local.data(), sizeof(uchar4) * local.size(), hipMemcpyHostToDevice) != hipSuccess)
                 return bail(fail(NF_E_DEVICE, "upload failed"));
+            // the first schedule's patch records, in both buffers
+            std::vector<int4> runs(first.size(), make_int4(-1, 0, 0, 0));
+            for (size_t k = 0; k < first.size(); k++)
+                if (first[k] >= 0) runs[k] = make_int4(first[k], pnStart[first[k]], pnStart[first[k] + 1] - pnStart[first[k]], 0);
+            int4* pr = nullptr;
+            if ((rc = s->alloc(&pr, 2 * runs.size()))) return bail(rc);
+            if (hipMemcpy(pr, runs.data(), sizeof(int4) * runs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(pr + runs.size(), runs.data(), sizeof(int4) * runs.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(NF_E_DEVICE, "upload failed"));
+            s->d.patchRun = pr;
             s->d.pnStart = ps;
             s->d.pnCode = pc;
             s->d.pnSlot4 = p4;
