@@ -1209,10 +1209,18 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     double2* vOut = b ? d.v : d.vB;
     uint8_t* flagsOut = b ? d.flags : d.flagsB;
 
-    // The log/exp tables first: their loads overlap the node records' below, and their barrier
-    // is the only workgroup barrier before the element phase.
-    __shared__ double sLog[97][3];
-    __shared__ double sExp[64][2];
+    // The log/exp tables, one copy per wave: their loads are issued first and written to the
+    // wave's LDS copy with the node phase's values, so that no workgroup barrier stands between
+    // the node records' loads and the gathers that depend on them.
+    __shared__ double sLog[BLOCK / 64][97][3];
+    __shared__ double sExp[BLOCK / 64][64][2];
+    constexpr int TABLE_DOUBLES = 97 * 3 + 64 * 2, TABLE_LOADS = (TABLE_DOUBLES + 63) / 64;
+    double tv[TABLE_LOADS];
+#pragma unroll
+    for (int k = 0; k < TABLE_LOADS; k++) {
+        const int i = lane + 64 * k;
+        tv[k] = i < 97 * 3 ? (&NFM_LOG_TBL[0][0])[i] : (i < TABLE_DOUBLES ? (&NFM_EXP_TBL[0][0])[i - 97 * 3] : 0.0);
+    }
     const int first = run.y, count = run.z;
     // The element phase's element and its corners' LDS slots, loaded now so that the point
     // state's loads (indexed by the element) issue as soon as the node phase is done.
@@ -1233,7 +1241,7 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
         c1 = d.pnCode[first + lane + 64];
         q41 = d.pnSlot4[first + lane + 64];
     }
-    const nfm::Tables T = nfd::stage_tables(sLog, sExp);
+    const nfm::Tables T{sLog[wave], sExp[wave]};
 
     // Node phase: the patch's nodes, at most two per lane, every load of both issued before
     // either is used.
@@ -1302,6 +1310,12 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
             sU[wave][j] = uk;
             sV[wave][j] = vk;
         };
+#pragma unroll
+        for (int k = 0; k < TABLE_LOADS; k++) {
+            const int i = lane + 64 * k;
+            if (i < 97 * 3) (&sLog[wave][0][0])[i] = tv[k];
+            else if (i < TABLE_DOUBLES) (&sExp[wave][0][0])[i - 97 * 3] = tv[k];
+        }
         if (has0) update(c0, lane, k0, m0, x00, gathered(c0, q40, a0, k0, m0), uu0, vv0, fl0);
         if (has1) update(c1, lane + 64, k1, m1, x01, gathered(c1, q41, a1, k1, m1), uu1, vv1, fl1);
     }
