@@ -134,24 +134,28 @@ struct Dev {
     uint8_t* patchCost;   // [2][patches] Newton cost of the patch's slowest lane, last step
     int patches, patchBlocks, units;  // units: the device's compute units
     // The fused step kernel (k_step). Patch p's nodes are entries pnStart[p] .. pnStart[p+1]
-    // of pnCode and pnSlot4 (ascending node numbers): pnCode holds the node number in bits
-    // 0-29, bit 30 set when the node has more than four force slots (the rest are on the CSR),
-    // bit 31 set when p owns the node (p holds the node's lowest-numbered element); pnSlot4 the
-    // node's first four slots (slot4). orderLocal[64 p + l] are the corners of element
+    // of pnCode (ascending node numbers): pnCode holds the node number in bits 0-27, the
+    // node's force-slot count less one in bits 28-29, and bit 31 set when p owns the node (p
+    // holds the node's lowest-numbered element). orderLocal[64 p + l] are the corners of element
     // order[64 p + l] as indices into the patch's list. Node state and force slots are
-    // double-buffered: buffer 0 is u, v, flags, slot; buffer 1 is uB, vB, flagsB, slotB.
+    // double-buffered: buffer 0 is u, v, flags, slotN; buffer 1 is uB, vB, flagsB, slotNB.
+    // The fused run's force slots are node-major: slotN[4 n + r] is node n's r-th slot in
+    // the gather's order (every node has at most four), so that a node reads its slots as one
+    // 64-byte record; orderDest[64 p + l] gives element order[64 p + l]'s four corner slots'
+    // places there. The two-launch step keeps the element-major d.slot.
     const int* pnStart;
     // patchRun[s] = {patchOf[s], pnStart[p], pnStart[p+1] - pnStart[p], 0} for p = patchOf[s]
     // (or {-1, 0, 0, 0}), double-buffered as patchOf: one scalar load gives a fused-step wave
     // its patch and node list.
     int4* patchRun;
     const int* pnCode;
-    const int4* pnSlot4;
     const uchar4* orderLocal;
+    const int4* orderDest;
     double2* uB;
     double2* vB;
     uint8_t* flagsB;
-    double2* slotB;
+    double2* slotN;
+    double2* slotNB;
     double* wallPartialF;     // [2][patchBlocks * 3]
     int* wallCountPartialF;   // [2][patchBlocks * 2]
     const int* matIndex;  // null when uniform
@@ -354,6 +358,19 @@ __device__ __forceinline__ double2 gather_slots(const Dev& d, const double2* __r
     return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
 }
 __device__ __forceinline__ double2 gather_node(const Dev& d, int n) { return gather_slots(d, d.slot, n); }
+// The same gather from a fused run's node-major slots (slotN: at most four per node, in order).
+__device__ __forceinline__ double2 gather_node_major(const Dev& d, int n) {
+    const int slots = d.slotStart[n + 1] - d.slotStart[n];
+    const double2* a = d.slotN + 4 * (size_t)n;
+    const double2 s0 = a[0], s1 = a[1], s2 = a[2], s3 = a[3];
+    double sr = 0.0, sz = 0.0;
+    sr += s0.x; sz += s0.y;
+    if (slots > 1) { sr += s1.x; sz += s1.y; }
+    if (slots > 2) { sr += s2.x; sz += s2.y; }
+    if (slots > 3) { sr += s3.x; sz += s3.y; }
+    const double m = d.mass[n];
+    return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
+}
 
 #include "device_contact.h"
 
@@ -444,8 +461,10 @@ __device__ void schedule_patches(const Dev& d, const uint8_t* __restrict__ costI
 // Reads control buffer q; block 0 writes buffer q ^ 1.
 // minSlot >= 0: the preceding element pass was the fused step kernel's, whose shortest edge is
 // in Ctl::fMin[minSlot] (and the node state has been brought back to buffer 0, k_canon).
+// nodeMajor: the gather closes a fused run, whose slots are node-major (slotN).
 template <bool WALL>
-__global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part, int minSlot) {
+__global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part, int minSlot,
+                                              int nodeMajor) {
     if (d.order && part != PART_REST && blockIdx.x == gridDim.x - 1) {
         // the workgroup past the others (launchGNT): the schedule of the element launch that
         // reads Step q ^ 1, from the costs of the one that read Step q
@@ -474,7 +493,7 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
             return;
         }
         const int i = blockIdx.x * BLOCK + threadIdx.x;
-        if (gather && i < d.N) d.f[i] = gather_node(d, i);
+        if (gather && i < d.N) d.f[i] = nodeMajor ? gather_node_major(d, i) : gather_node(d, i);
         return;
     }
     if (part == PART_REST && gathered && blockIdx.x == 0) reduce_contact_partials(d);
@@ -526,7 +545,7 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
     double2 fi = make_double2(0.0, 0.0);
     if (i < d.N) {
         if (gather) {
-            fi = gather_node(d, i);
+            fi = nodeMajor ? gather_node_major(d, i) : gather_node(d, i);
             d.f[i] = fi;
         } else {
             fi = d.f[i];
@@ -723,6 +742,9 @@ __device__ __forceinline__ void write_zero_slots(const Dev& d, int e) {
 // One element under reduced integration, solver/ExplicitSolver.java:1373-1504.
 // REC (the adjoint's recording pass only): *dGammaOut and *status receive the return map's
 // multiplier and outcome (j2_update).
+// The places of element e's four corner slots in the element-major d.slot.
+__device__ __forceinline__ int4 elementSlots(int e) { return make_int4(4 * e, 4 * e + 1, 4 * e + 2, 4 * e + 3); }
+
 // The reference positions, displacements and velocities of an element's four corners.
 struct Corners {
     double2 p0, p1, p2, p3, u0, u1, u2, u3, v0, v1, v2, v3;
@@ -733,10 +755,11 @@ __device__ __forceinline__ Corners load_corners(const Dev& d, int4 n) {
 }
 
 // The element on corner values already in registers (c), writing its four force slots to
-// slotOut (d.slot, or the fused step kernel's other slot buffer).
+// slotOut[dest.x .. dest.w] (d.slot at elementSlots(e), or a fused run's node-major slots at
+// orderDest).
 template <bool DAMAGE, bool KIN, bool REC = false>
 __device__ __forceinline__ double reduced_element_at(const Dev& d, int e, const Corners& c, double2* slotOut,
-                                       double strainDt, const nfm::Tables& T, double* dGammaOut = nullptr,
+                                       int4 dest, double strainDt, const nfm::Tables& T, double* dGammaOut = nullptr,
                                        int* status = nullptr, int* iterations = nullptr) {
     const Mat fl = load_mat(d, e);
     const double mu = fl.mu;
@@ -841,10 +864,10 @@ __device__ __forceinline__ double reduced_element_at(const Dev& d, int e, const 
         g0z -= y0 * qz; g1z -= y1 * qz; g2z -= y2 * qz; g3z -= y3 * qz;
         d.hgK[e] = k;
     }
-    slotOut[4 * e] = make_double2(g0r, g0z);
-    slotOut[4 * e + 1] = make_double2(g1r, g1z);
-    slotOut[4 * e + 2] = make_double2(g2r, g2z);
-    slotOut[4 * e + 3] = make_double2(g3r, g3z);
+    slotOut[dest.x] = make_double2(g0r, g0z);
+    slotOut[dest.y] = make_double2(g1r, g1z);
+    slotOut[dest.z] = make_double2(g2r, g2z);
+    slotOut[dest.w] = make_double2(g3r, g3z);
     return minEdge;
 }
 
@@ -852,7 +875,7 @@ template <bool DAMAGE, bool KIN, bool REC = false>
 __device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, double strainDt,
                                        const nfm::Tables& T, double* dGammaOut = nullptr,
                                        int* status = nullptr, int* iterations = nullptr) {
-    return reduced_element_at<DAMAGE, KIN, REC>(d, e, load_corners(d, n), d.slot, strainDt, T, dGammaOut,
+    return reduced_element_at<DAMAGE, KIN, REC>(d, e, load_corners(d, n), d.slot, elementSlots(e), strainDt, T, dGammaOut,
                                                 status, iterations);
 }
 
@@ -860,7 +883,7 @@ __device__ __forceinline__ double reduced_element(const Dev& d, int e, int4 n, d
 // values in registers.
 template <bool DAMAGE, bool KIN>
 __device__ __forceinline__ double full_element_at(const Dev& d, int e, const Corners& c, double2* slotOut,
-                                       double strainDt, const nfm::Tables& T, int* iterations = nullptr) {
+                                       int4 dest, double strainDt, const nfm::Tables& T, int* iterations = nullptr) {
     const Mat fl = load_mat(d, e);
     const double mid = -0.5 * strainDt;
     const double2 p0 = c.p0, p1 = c.p1, p2 = c.p2, p3 = c.p3;
@@ -955,17 +978,17 @@ __device__ __forceinline__ double full_element_at(const Dev& d, int e, const Cor
         g2z -= (d2z * sZ + d2r * sRZ) * dV;
         g3z -= (d3z * sZ + d3r * sRZ) * dV;
     }
-    slotOut[4 * e] = make_double2(g0r, g0z);
-    slotOut[4 * e + 1] = make_double2(g1r, g1z);
-    slotOut[4 * e + 2] = make_double2(g2r, g2z);
-    slotOut[4 * e + 3] = make_double2(g3r, g3z);
+    slotOut[dest.x] = make_double2(g0r, g0z);
+    slotOut[dest.y] = make_double2(g1r, g1z);
+    slotOut[dest.z] = make_double2(g2r, g2z);
+    slotOut[dest.w] = make_double2(g3r, g3z);
     return minEdge;
 }
 
 template <bool DAMAGE, bool KIN>
 __device__ __forceinline__ double full_element(const Dev& d, int e, int4 n, double strainDt,
                                        const nfm::Tables& T, int* iterations = nullptr) {
-    return full_element_at<DAMAGE, KIN>(d, e, load_corners(d, n), d.slot, strainDt, T, iterations);
+    return full_element_at<DAMAGE, KIN>(d, e, load_corners(d, n), d.slot, elementSlots(e), strainDt, T, iterations);
 }
 
 // Pressure edge k, solver/ExplicitSolver.java:1511-1556 (coordinates at the current, not the
@@ -1008,7 +1031,9 @@ __device__ void pressure_edge(const Dev& d, int k, double p) {
 }
 
 template <bool FULL, bool DAMAGE, bool KIN>
-__global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev d, int q, int mode) {
+// nodeMajor: the first element pass of a fused run writes its slots node-major (slotN at
+// orderDest), where the fused step kernel that follows gathers them.
+__global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev d, int q, int mode, int nodeMajor) {
     Ctl* c = d.ctl;
     Step& st = c->s[q];
     if (mode == MODE_STEP && !st.go) return;
@@ -1023,7 +1048,7 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
     // that the two overlap.
     const bool scheduled = d.order != nullptr;
     int e = t, patch = -1, edgeIndex = t - d.E;
-    int4 n = make_int4(0, 0, 0, 0);
+    int4 n = make_int4(0, 0, 0, 0), dest = make_int4(0, 0, 0, 0);
     if (scheduled) {
         const int elementThreads = BLOCK * d.patchBlocks;
         edgeIndex = t - elementThreads;
@@ -1033,6 +1058,7 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
             if (patch >= 0) {
                 e = d.order[64 * patch + (t & 63)];
                 n = d.orderConn[64 * patch + (t & 63)];
+                if (nodeMajor) dest = d.orderDest[64 * patch + (t & 63)];
             }
         }
     } else if (t < d.E) {
@@ -1054,8 +1080,13 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
         if (gone) {
             write_zero_slots(d, e);
         } else {
-            edge = FULL ? full_element<DAMAGE, KIN>(d, e, n, strainDt, T, &iterations)
-                        : reduced_element<DAMAGE, KIN>(d, e, n, strainDt, T, nullptr, nullptr, &iterations);
+            if (nodeMajor)
+                edge = FULL ? full_element_at<DAMAGE, KIN>(d, e, load_corners(d, n), d.slotN, dest, strainDt, T, &iterations)
+                            : reduced_element_at<DAMAGE, KIN>(d, e, load_corners(d, n), d.slotN, dest, strainDt, T, nullptr,
+                                                              nullptr, &iterations);
+            else
+                edge = FULL ? full_element<DAMAGE, KIN>(d, e, n, strainDt, T, &iterations)
+                            : reduced_element<DAMAGE, KIN>(d, e, n, strainDt, T, nullptr, nullptr, &iterations);
             sawNaN = edge != edge;
         }
     } else if (edgeIndex >= 0 && edgeIndex < d.edges) {
@@ -1200,8 +1231,8 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
         c->fMin[last3] = MAX_BITS;
         c->fNaN[last3] = 0;
     }
-    const double2* slotIn = b ? d.slotB : d.slot;
-    double2* slotOut = b ? d.slot : d.slotB;
+    const double2* slotIn = b ? d.slotNB : d.slotN;
+    double2* slotOut = b ? d.slotN : d.slotNB;
     const double2* uIn = b ? d.uB : d.u;
     const double2* vIn = b ? d.vB : d.v;
     const uint8_t* flagsIn = b ? d.flagsB : d.flags;
@@ -1226,21 +1257,14 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     // state's loads (indexed by the element) issue as soon as the node phase is done.
     int eEl = -1;
     uchar4 eLi = make_uchar4(0, 0, 0, 0);
+    int4 eDest = make_int4(0, 0, 0, 0);
     if (patch >= 0) {
         eEl = d.order[64 * patch + lane];
         eLi = d.orderLocal[64 * patch + lane];
+        eDest = d.orderDest[64 * patch + lane];
     }
     const bool has0 = lane < count, has1 = lane + 64 < count;
-    int c0 = 0, c1 = 0;
-    int4 q40 = make_int4(-1, -1, -1, -1), q41 = q40;
-    if (has0) {
-        c0 = d.pnCode[first + lane];
-        q40 = d.pnSlot4[first + lane];
-    }
-    if (has1) {
-        c1 = d.pnCode[first + lane + 64];
-        q41 = d.pnSlot4[first + lane + 64];
-    }
+    const int c0 = has0 ? d.pnCode[first + lane] : 0, c1 = has1 ? d.pnCode[first + lane + 64] : 0;
     const nfm::Tables T{sLog[wave], sExp[wave]};
 
     // Node phase: the patch's nodes, at most two per lane, every load of both issued before
@@ -1256,36 +1280,28 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     int contacts = 0, stuck = 0;
     {
         const double2 z2 = make_double2(0.0, 0.0);
-        const int k0 = c0 & 0x3fffffff, k1 = c1 & 0x3fffffff;
+        const int k0 = c0 & 0x0fffffff, k1 = c1 & 0x0fffffff;
+        // a node's slots: one 64-byte node-major record (entries past its slot count unused)
         double2 a0[4], a1[4];
-        a0[0] = has0 && q40.x >= 0 ? slotIn[q40.x] : z2;
-        a0[1] = has0 && q40.y >= 0 ? slotIn[q40.y] : z2;
-        a0[2] = has0 && q40.z >= 0 ? slotIn[q40.z] : z2;
-        a0[3] = has0 && q40.w >= 0 ? slotIn[q40.w] : z2;
-        a1[0] = has1 && q41.x >= 0 ? slotIn[q41.x] : z2;
-        a1[1] = has1 && q41.y >= 0 ? slotIn[q41.y] : z2;
-        a1[2] = has1 && q41.z >= 0 ? slotIn[q41.z] : z2;
-        a1[3] = has1 && q41.w >= 0 ? slotIn[q41.w] : z2;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            a0[r] = has0 ? slotIn[4 * (size_t)k0 + r] : z2;
+            a1[r] = has1 ? slotIn[4 * (size_t)k1 + r] : z2;
+        }
         const double m0 = has0 ? d.mass[k0] : 1.0, m1 = has1 ? d.mass[k1] : 1.0;
         const double2 x00 = has0 ? d.x0[k0] : z2, x01 = has1 ? d.x0[k1] : z2;
         double2 uu0 = has0 ? uIn[k0] : z2, uu1 = has1 ? uIn[k1] : z2;
         double2 vv0 = has0 ? vIn[k0] : z2, vv1 = has1 ? vIn[k1] : z2;
         uint8_t fl0 = has0 ? flagsIn[k0] : 0, fl1 = has1 ? flagsIn[k1] : 0;
-        // The gather in the reference's order (solver/ExplicitSolver.java:1147-1165): the first
-        // four slots, then any further ones on the CSR, then gravity (gather_slots' sum).
-        auto gathered = [&](int code, const int4& k4, const double2 (&a)[4], int k, double m) {
+        // The gather in the reference's order (solver/ExplicitSolver.java:1147-1165): the
+        // node's slots in order, then gravity (gather_slots' sum).
+        auto gathered = [&](int code, const double2 (&a)[4], double m) {
+            const int slots = ((code >> 28) & 3) + 1;
             double sr = 0.0, sz = 0.0;
-            if (k4.x >= 0) { sr += a[0].x; sz += a[0].y; }
-            if (k4.y >= 0) { sr += a[1].x; sz += a[1].y; }
-            if (k4.z >= 0) { sr += a[2].x; sz += a[2].y; }
-            if (k4.w >= 0) { sr += a[3].x; sz += a[3].y; }
-            if (code & 0x40000000) {
-                for (int j = d.slotStart[k] + 4, e = d.slotStart[k + 1]; j < e; j++) {
-                    const double2 x = slotIn[d.slotOf[j]];
-                    sr += x.x;
-                    sz += x.y;
-                }
-            }
+            sr += a[0].x; sz += a[0].y;
+            if (slots > 1) { sr += a[1].x; sz += a[1].y; }
+            if (slots > 2) { sr += a[2].x; sz += a[2].y; }
+            if (slots > 3) { sr += a[3].x; sz += a[3].y; }
             return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
         };
         // x0.y is the node's reference axial coordinate, z0.
@@ -1316,8 +1332,8 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
             if (i < 97 * 3) (&sLog[wave][0][0])[i] = tv[k];
             else if (i < TABLE_DOUBLES) (&sExp[wave][0][0])[i - 97 * 3] = tv[k];
         }
-        if (has0) update(c0, lane, k0, m0, x00, gathered(c0, q40, a0, k0, m0), uu0, vv0, fl0);
-        if (has1) update(c1, lane + 64, k1, m1, x01, gathered(c1, q41, a1, k1, m1), uu1, vv1, fl1);
+        if (has0) update(c0, lane, k0, m0, x00, gathered(c0, a0, m0), uu0, vv0, fl0);
+        if (has1) update(c1, lane + 64, k1, m1, x01, gathered(c1, a1, m1), uu1, vv1, fl1);
     }
     if (!next) {  // the gather that closes the last step; no step follows
         if (WALL && in.pendingWall && blockIdx.x == 0)
@@ -1342,8 +1358,8 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
             const Corners cr{sX0[wave][li.x], sX0[wave][li.y], sX0[wave][li.z], sX0[wave][li.w],
                              sU[wave][li.x], sU[wave][li.y], sU[wave][li.z], sU[wave][li.w],
                              sV[wave][li.x], sV[wave][li.y], sV[wave][li.z], sV[wave][li.w]};
-            edge = FULL ? full_element_at<DAMAGE, KIN>(d, e, cr, slotOut, dt, T, &iterations)
-                        : reduced_element_at<DAMAGE, KIN>(d, e, cr, slotOut, dt, T, nullptr, nullptr, &iterations);
+            edge = FULL ? full_element_at<DAMAGE, KIN>(d, e, cr, slotOut, eDest, dt, T, &iterations)
+                        : reduced_element_at<DAMAGE, KIN>(d, e, cr, slotOut, eDest, dt, T, nullptr, nullptr, &iterations);
             sawNaN = edge != edge;
         }
         int m = iterations;
@@ -1423,7 +1439,7 @@ __global__ void __launch_bounds__(BLOCK) k_canon(Dev d, int q) {
         d.v[i] = d.vB[i];
         d.flags[i] = d.flagsB[i];
     }
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < 4 * (size_t)d.E; i += stride) d.slot[i] = d.slotB[i];
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < 4 * (size_t)d.N; i += stride) d.slotN[i] = d.slotNB[i];
 }
 
 // Sets the current step control's end time and step limit on the device (no host round trip).
@@ -2013,37 +2029,38 @@ struct nf_solver {
     // (hipExtLaunchKernelGGL), so a profiled duration is the kernel's own, not the gap
     // around it.
     template <bool FULL, bool DAMAGE, bool KIN>
-    int launchElementT(int mode, int parity, hipEvent_t a, hipEvent_t b) {
+    int launchElementT(int mode, int parity, hipEvent_t a, hipEvent_t b, int nodeMajor) {
         // scheduled: patch workgroups, then the pressure edges (k_element)
         const int grid = d.order ? d.patchBlocks + blocks_for(edges) : blocks_for(E + edges);
-        if (a) hipExtLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, mode);
-        else hipLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode);
+        if (a) hipExtLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, mode, nodeMajor);
+        else hipLaunchKernelGGL((k_element<FULL, DAMAGE, KIN>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, mode, nodeMajor);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
 
-    int launchElement(int mode, int parity, hipEvent_t a = nullptr, hipEvent_t b = nullptr) {
+    // nodeMajor: a fused run's first element pass (node-major slots, k_element).
+    int launchElement(int mode, int parity, hipEvent_t a = nullptr, hipEvent_t b = nullptr, int nodeMajor = 0) {
         switch ((integration == NF_FULL ? 4 : 0) | (hasDamage ? 2 : 0) | (hasKin ? 1 : 0)) {
-            case 0: return launchElementT<false, false, false>(mode, parity, a, b);
-            case 1: return launchElementT<false, false, true>(mode, parity, a, b);
-            case 2: return launchElementT<false, true, false>(mode, parity, a, b);
-            case 3: return launchElementT<false, true, true>(mode, parity, a, b);
-            case 4: return launchElementT<true, false, false>(mode, parity, a, b);
-            case 5: return launchElementT<true, false, true>(mode, parity, a, b);
-            case 6: return launchElementT<true, true, false>(mode, parity, a, b);
-            default: return launchElementT<true, true, true>(mode, parity, a, b);
+            case 0: return launchElementT<false, false, false>(mode, parity, a, b, nodeMajor);
+            case 1: return launchElementT<false, false, true>(mode, parity, a, b, nodeMajor);
+            case 2: return launchElementT<false, true, false>(mode, parity, a, b, nodeMajor);
+            case 3: return launchElementT<false, true, true>(mode, parity, a, b, nodeMajor);
+            case 4: return launchElementT<true, false, false>(mode, parity, a, b, nodeMajor);
+            case 5: return launchElementT<true, false, true>(mode, parity, a, b, nodeMajor);
+            case 6: return launchElementT<true, true, false>(mode, parity, a, b, nodeMajor);
+            default: return launchElementT<true, true, true>(mode, parity, a, b, nodeMajor);
         }
     }
 
     template <bool WALL>
     int launchGNT(int phase, int parity, int gatherFirst, int part, hipEvent_t a, hipEvent_t b) {
-        const int minSlot = gnMinSlot;
+        const int minSlot = gnMinSlot, nodeMajor = gnNodeMajor ? 1 : 0;
         // the gather before the contact launches carries the contact's segment rebuild
         // one workgroup more for the element pass's schedule (k_gn)
         const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0) +
                          (d.order && part != PART_REST ? 1 : 0);
-        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part, minSlot);
-        else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part, minSlot);
+        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part, minSlot, nodeMajor);
+        else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part, minSlot, nodeMajor);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
@@ -2104,7 +2121,7 @@ struct nf_solver {
         if (fusedRun) {
             // k_element, then k_step for each later step; closeFused adds k_canon and k_gn.
             if (!openElement) {
-                TRY(timed(1, [&](hipEvent_t a, hipEvent_t b) { return launchElement(MODE_STEP, q, a, b); }));
+                TRY(timed(1, [&](hipEvent_t a, hipEvent_t b) { return launchElement(MODE_STEP, q, a, b, 1); }));
                 openElement = true;
                 lastK3 = -1;
                 return NF_OK;
@@ -2128,6 +2145,7 @@ struct nf_solver {
     int lastK3 = -1;            // rotation index of the last k_step since that element pass
     long long fusedCount = 0;   // k_step launches so far (their fMin rotation)
     int gnMinSlot = -1;         // k_gn's minSlot argument for the next node launch
+    bool gnNodeMajor = false;   // k_gn's nodeMajor argument for the next node launch
     bool fusedDisabled = false; // NEOFIZ_FUSED=0
 
     bool fusedEligible() const {
@@ -2136,11 +2154,12 @@ struct nf_solver {
     }
 
     int fusedBuffers() {
-        if (d.uB) return NF_OK;
+        if (d.uB) return NF_OK;  // node state and node-major slots, both buffers
         TRY(alloc(&d.uB, N));
         TRY(alloc(&d.vB, N));
         TRY(alloc(&d.flagsB, N));
-        TRY(alloc(&d.slotB, 4 * (size_t)E));
+        TRY(alloc(&d.slotN, 4 * (size_t)N));
+        TRY(alloc(&d.slotNB, 4 * (size_t)N));
         TRY(alloc(&d.wallPartialF, 2 * 3 * (size_t)d.patchBlocks));
         TRY(alloc(&d.wallCountPartialF, 2 * 2 * (size_t)d.patchBlocks));
         return NF_OK;
@@ -2187,8 +2206,10 @@ struct nf_solver {
             HIP_TRY(hipGetLastError());
         }
         gnMinSlot = lastK3 >= 0 ? (lastK3 + 1) % 3 : -1;
+        gnNodeMajor = true;
         const int rc = timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_STEP, q, 0, a, b); });
         gnMinSlot = -1;
+        gnNodeMajor = false;
         TRY(rc);
         q ^= 1;
         openElement = false;
@@ -2908,18 +2929,25 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
         s->d.units = units;
 
         // The fused step kernel's node lists: every patch's corner nodes, each owned by the patch
-        // that holds its lowest-numbered element. A node in no element, or a patch with more than
-        // STEP_PATCH_NODES nodes, leaves the solver on the two-launch step.
+        // that holds its lowest-numbered element. A node in no element or with more than four
+        // force slots, or a patch with more than STEP_PATCH_NODES nodes, leaves the solver on the
+        // two-launch step.
         std::vector<int> firstElem(N, -1);
         for (int e = E - 1; e >= 0; e--)
             for (int k = 0; k < 4; k++) firstElem[conn[4 * (size_t)e + k]] = e;
         std::vector<int> patchOfElem(E);
         for (size_t k = 0; k < ord.size(); k++) patchOfElem[ord[k]] = (int)(k / 64);
-        bool fusable = true;
-        for (int i = 0; i < N && fusable; i++) fusable = firstElem[i] >= 0;
+        bool fusable = N < (1 << 28);
+        for (int i = 0; i < N && fusable; i++)
+            fusable = firstElem[i] >= 0 && s->slotStart[i + 1] - s->slotStart[i] <= 4;
+        // Each element slot's place in the node-major slots: 4 n + its rank in node n's gather.
+        std::vector<int> nodeMajor(4 * (size_t)E, 0);
+        for (int n = 0; n < N && fusable; n++)
+            for (int j = s->slotStart[n]; j < s->slotStart[n + 1]; j++)
+                if (s->slotOf[j] < 4 * E) nodeMajor[s->slotOf[j]] = 4 * n + (j - s->slotStart[n]);
         std::vector<int> pnStart(P + 1, 0), pnCode;
-        std::vector<int4> pnSlot4;
         std::vector<uchar4> local(padded.size(), make_uchar4(0, 0, 0, 0));
+        std::vector<int4> dest(padded.size(), make_int4(0, 0, 0, 0));
         for (int p = 0; p < P && fusable; p++) {
             std::vector<int> nodes;
             for (int l = 0; l < 64; l++) {
@@ -2940,27 +2968,27 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
                 for (int k = 0; k < 4; k++)
                     li[k] = (unsigned char)(std::lower_bound(nodes.begin(), nodes.end(), conn[4 * (size_t)e + k]) - nodes.begin());
                 local[64 * (size_t)p + l] = make_uchar4(li[0], li[1], li[2], li[3]);
+                const size_t s0 = 4 * (size_t)e;
+                dest[64 * (size_t)p + l] = make_int4(nodeMajor[s0], nodeMajor[s0 + 1], nodeMajor[s0 + 2], nodeMajor[s0 + 3]);
             }
             for (int node : nodes) {
-                pnCode.push_back(node | (s->slotStart[node + 1] - s->slotStart[node] > 4 ? 0x40000000 : 0) |
-                                 (patchOfElem[firstElem[node]] == p ? (int)0x80000000u : 0));
-                pnSlot4.push_back(s4[node]);
+                const int slots = s->slotStart[node + 1] - s->slotStart[node];
+                pnCode.push_back(node | ((slots - 1) << 28) | (patchOfElem[firstElem[node]] == p ? (int)0x80000000u : 0));
             }
             pnStart[p + 1] = (int)pnCode.size();
         }
-        if (fusable && N >= (1 << 30)) fusable = false;
         if (fusable) {
             int* ps = nullptr;
             int* pc = nullptr;
-            int4* p4 = nullptr;
             uchar4* ol = nullptr;
+            int4* od = nullptr;
             if ((rc = s->alloc(&ps, pnStart.size())) || (rc = s->alloc(&pc, pnCode.size())) ||
-                (rc = s->alloc(&p4, pnSlot4.size())) || (rc = s->alloc(&ol, local.size())))
+                (rc = s->alloc(&ol, local.size())) || (rc = s->alloc(&od, dest.size())))
                 return bail(rc);
             if (hipMemcpy(ps, pnStart.data(), sizeof(int) * pnStart.size(), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(pc, pnCode.data(), sizeof(int) * pnCode.size(), hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(p4, pnSlot4.data(), sizeof(int4) * pnSlot4.size(), hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(ol, local.data(), sizeof(uchar4) * local.size(), hipMemcpyHostToDevice) != hipSuccess)
+                hipMemcpy(ol, local.data(), sizeof(uchar4) * local.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(od, dest.data(), sizeof(int4) * dest.size(), hipMemcpyHostToDevice) != hipSuccess)
                 return bail(fail(NF_E_DEVICE, "upload failed"));
             // the first schedule's patch records, in both buffers
             std::vector<int4> runs(first.size(), make_int4(-1, 0, 0, 0));
@@ -2974,8 +3002,8 @@ int nf_solver_create(nf_solver** out, int32_t node_count, const double* r, const
             s->d.patchRun = pr;
             s->d.pnStart = ps;
             s->d.pnCode = pc;
-            s->d.pnSlot4 = p4;
             s->d.orderLocal = ol;
+            s->d.orderDest = od;
         }
     }
     s->d.nodalBlocks = blocks_for(N);
