@@ -1341,6 +1341,31 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
                                  d.wallCountPartialF + (size_t)q * 2 * d.patchBlocks, d.patchBlocks);
         return;
     }
+    // This nodal update's wall audits, summed over the wave now (a fixed shuffle tree) and over
+    // the workgroup's waves in wave order after the element phase: block_sum's order, with the
+    // workgroup's one barrier at the end.
+    __shared__ double ldsW[BLOCK / 64][3];
+    __shared__ int ldsWi[BLOCK / 64][2];
+    if (WALL) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double x = acc[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            acc[k] = x;
+        }
+        int a = contacts, bb = stuck;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            bb += __shfl_xor(bb, off, 64);
+        }
+        if (lane == 0) {
+            ldsW[wave][0] = acc[0];
+            ldsW[wave][1] = acc[1];
+            ldsW[wave][2] = acc[2];
+            ldsWi[wave][0] = a;
+            ldsWi[wave][1] = bb;
+        }
+    }
     // The element phase reads only its own wave's nodes: a wave-level fence and barrier publish
     // the LDS writes to the other lanes.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1369,32 +1394,6 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
         }
         if (lane == 0) d.patchCost[(size_t)(q ^ 1) * d.patches + patch] = (uint8_t)(m < 255 ? m : 255);
     }
-    if (WALL) {
-        // this nodal update's wall audits, per workgroup (fixed-order block sum), for the next launch
-        __shared__ double ldsW[4 * 3];
-        __shared__ int ldsWi[2 * 4];
-        block_sum<3>(acc, ldsW);
-        int a = contacts, bb = stuck;
-        for (int off = 32; off > 0; off >>= 1) {
-            a += __shfl_xor(a, off, 64);
-            bb += __shfl_xor(bb, off, 64);
-        }
-        if (lane == 0) {
-            ldsWi[wave * 2] = a;
-            ldsWi[wave * 2 + 1] = bb;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double* pw = d.wallPartialF + (size_t)(q ^ 1) * 3 * d.patchBlocks + 3 * blockIdx.x;
-            pw[0] = acc[0];
-            pw[1] = acc[1];
-            pw[2] = acc[2];
-            int* pc = d.wallCountPartialF + (size_t)(q ^ 1) * 2 * d.patchBlocks + 2 * blockIdx.x;
-            pc[0] = ldsWi[0] + ldsWi[2] + ldsWi[4] + ldsWi[6];
-            pc[1] = ldsWi[1] + ldsWi[3] + ldsWi[5] + ldsWi[7];
-        }
-    }
-
     // Shortest edge: exact minimum over non-negative doubles by their bit patterns.
     unsigned long long bits = sawNaN ? MAX_BITS : (unsigned long long)nfm::bits_of(edge);
     for (int off = 32; off > 0; off >>= 1) {
@@ -1403,19 +1402,33 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     }
     const unsigned long long anyNaN = __ballot(sawNaN);
     __shared__ unsigned long long ldsMin[BLOCK / 64];
-    __shared__ int ldsNaN;
-    if (threadIdx.x == 0) ldsNaN = 0;
-    __syncthreads();
+    __shared__ int ldsNaN[BLOCK / 64];
     if (lane == 0) {
         ldsMin[wave] = bits;
-        if (anyNaN) ldsNaN = 1;
+        ldsNaN[wave] = anyNaN ? 1 : 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (WALL) {
+            double* pw = d.wallPartialF + (size_t)(q ^ 1) * 3 * d.patchBlocks + 3 * blockIdx.x;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double sum = 0.0;
+                for (int w = 0; w < BLOCK / 64; w++) sum += ldsW[w][k];
+                pw[k] = sum;
+            }
+            int* pc = d.wallCountPartialF + (size_t)(q ^ 1) * 2 * d.patchBlocks + 2 * blockIdx.x;
+            pc[0] = ldsWi[0][0] + ldsWi[1][0] + ldsWi[2][0] + ldsWi[3][0];
+            pc[1] = ldsWi[0][1] + ldsWi[1][1] + ldsWi[2][1] + ldsWi[3][1];
+        }
         unsigned long long m = ldsMin[0];
-        for (int w = 1; w < BLOCK / 64; w++) m = ldsMin[w] < m ? ldsMin[w] : m;
+        int nan = ldsNaN[0];
+        for (int w = 1; w < BLOCK / 64; w++) {
+            m = ldsMin[w] < m ? ldsMin[w] : m;
+            nan |= ldsNaN[w];
+        }
         if (m != MAX_BITS) atomicMin(&c->fMin[next3], m);
-        if (ldsNaN) atomicOr(&c->fNaN[next3], 1);
+        if (nan) atomicOr(&c->fNaN[next3], 1);
     }
     if (WALL && in.pendingWall && blockIdx.x == 0)
         reduce_wall_partials(d, d.wallPartialF + (size_t)q * 3 * d.patchBlocks,
