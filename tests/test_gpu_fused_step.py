@@ -79,12 +79,38 @@ def block(integration, kinematics, formulation, kinematic_hardening=False, gravi
     return make
 
 
+def fan(integration):
+    """Six quads around one centre node (six force slots there): the fused step's node-major
+    slots hold at most four a node, so this mesh runs the two-launch step even when the fused
+    one is asked for."""
+    def make():
+        ring = 6
+        r, z = [0.05], [0.05]
+        for k in range(ring):
+            a = 2 * math.pi * k / ring
+            r += [0.05 + 0.01 * math.cos(a), 0.05 + 0.012 * math.cos(a + math.pi / ring)]
+            z += [0.05 + 0.01 * math.sin(a), 0.05 + 0.012 * math.sin(a + math.pi / ring)]
+        conn = []
+        for k in range(ring):
+            spoke, mid, nxt = 1 + 2 * k, 2 + 2 * k, 1 + 2 * ((k + 1) % ring)
+            conn += [0, spoke, mid, nxt]
+        mesh = QuadMesh(np.array(r), np.array(z), np.array(conn, dtype=np.int32))
+        mat = Material("steel", 200e9, 0.3, 7800.0).yielding(3e8, 1e9, 0.0)
+        s = ExplicitSolver(mesh, mat, PLANE_STRAIN, integration, FINITE_STRAIN, 0.01, 0.5)
+        rng = np.random.default_rng(11)
+        s.putNodeField("vr", rng.uniform(-50, 50, mesh.nodeCount))
+        s.putNodeField("vz", rng.uniform(-50, 50, mesh.nodeCount))
+        return s
+    return make
+
+
 CASES = {
     "jc-rod-friction": rod(16, 160, 0.09),
     "jc-rod-welded": rod(8, 80, math.inf),
     "jc-rod-full": rod(8, 80, 0.2, FULL),
     "kinematic-gravity-damping": block(REDUCED, FINITE_STRAIN, AXISYMMETRIC, True, True, True),
     "plane-strain-full-small": block(FULL, SMALL_STRAIN, PLANE_STRAIN),
+    "fan-six-slot-node": fan(REDUCED),
 }
 
 
