@@ -1180,8 +1180,15 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
 // launch with rotation index k3 reads slot k3 (or, first after a plain element launch, Step q),
 // lowers slot (k3 + 1) % 3 and resets slot (k3 + 2) % 3. Wall partials go to
 // wallPartialF[q ^ 1]; the next launch (or k_canon) adds them up when Step::pendingWall says so.
-template <bool FULL, bool DAMAGE, bool KIN, bool WALL>
-__global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d, int q, int k3, int minFromStep) {
+// from: where this launch's step comes from. STEP_AFTER_STEP: a k_step launch, whose shortest
+// edge is in Ctl::fMin[k3]; STEP_AFTER_ELEMENT: the two-launch element pass, whose shortest edge
+// is in the Step. OPEN (its own instantiation, so that the other launches' code is untouched):
+// the first launch of a run call, which opens its first step from the closed state a run call
+// leaves (the step's dt already taken, the forces gathered in d.f), as the node launch's
+// PHASE_RESUME does, with the end test made on the device.
+constexpr int STEP_AFTER_STEP = 0, STEP_AFTER_ELEMENT = 1;
+template <bool FULL, bool DAMAGE, bool KIN, bool WALL, bool OPEN>
+__global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d, int q, int k3, int from) {
     if (blockIdx.x == gridDim.x - 1) {
         schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)q * 4 * d.patchBlocks,
                          d.patchRun + (size_t)q * 4 * d.patchBlocks);
@@ -1196,7 +1203,8 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
                                 __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / 64) + wave)];
     const int patch = run.x;
     const int next3 = k3 == 2 ? 0 : k3 + 1, last3 = k3 == 0 ? 2 : k3 - 1;
-    if (!in.go) {  // nothing ran: carry the control over
+    constexpr bool open = OPEN;
+    if (!open && !in.go) {  // nothing ran: carry the control over
         if (lead) {
             out = in;
             c->fMin[last3] = MAX_BITS;
@@ -1205,9 +1213,10 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
         return;
     }
     double dt = in.dt, dtPrev = in.dtPrev;
-    if (d.finite) {
-        const bool nanEdge = minFromStep ? in.nanEdge != 0 : c->fNaN[k3] != 0;
-        const unsigned long long bits = minFromStep ? in.minEdgeBits : c->fMin[k3];
+    if (d.finite && !open) {
+        const bool afterElement = from == STEP_AFTER_ELEMENT;
+        const bool nanEdge = afterElement ? in.nanEdge != 0 : c->fNaN[k3] != 0;
+        const unsigned long long bits = afterElement ? in.minEdgeBits : c->fMin[k3];
         const double minEdge = nanEdge ? NFD_MAX_VALUE : nfm::from_bits(bits);
         dtPrev = in.dt;
         dt = d.cfl * __builtin_sqrt(minEdge * d.massScale) / d.fastestWave;
@@ -1281,13 +1290,15 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     {
         const double2 z2 = make_double2(0.0, 0.0);
         const int k0 = c0 & 0x0fffffff, k1 = c1 & 0x0fffffff;
-        // a node's slots: one 64-byte node-major record (entries past its slot count unused)
+        // a node's slots: one 64-byte node-major record (entries past its slot count unused);
+        // an opening launch takes the forces the run call before gathered instead
         double2 a0[4], a1[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            a0[r] = has0 ? slotIn[4 * (size_t)k0 + r] : z2;
-            a1[r] = has1 ? slotIn[4 * (size_t)k1 + r] : z2;
+            a0[r] = has0 && !open ? slotIn[4 * (size_t)k0 + r] : z2;
+            a1[r] = has1 && !open ? slotIn[4 * (size_t)k1 + r] : z2;
         }
+        const double2 f0 = has0 && open ? d.f[k0] : z2, f1 = has1 && open ? d.f[k1] : z2;
         const double m0 = has0 ? d.mass[k0] : 1.0, m1 = has1 ? d.mass[k1] : 1.0;
         const double2 x00 = has0 ? d.x0[k0] : z2, x01 = has1 ? d.x0[k1] : z2;
         double2 uu0 = has0 ? uIn[k0] : z2, uu1 = has1 ? uIn[k1] : z2;
@@ -1295,7 +1306,8 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
         uint8_t fl0 = has0 ? flagsIn[k0] : 0, fl1 = has1 ? flagsIn[k1] : 0;
         // The gather in the reference's order (solver/ExplicitSolver.java:1147-1165): the
         // node's slots in order, then gravity (gather_slots' sum).
-        auto gathered = [&](int code, const double2 (&a)[4], double m) {
+        auto gathered = [&](int code, const double2 (&a)[4], double2 fOpen, double m) {
+            if (open) return fOpen;
             const int slots = ((code >> 28) & 3) + 1;
             double sr = 0.0, sz = 0.0;
             sr += a[0].x; sz += a[0].y;
@@ -1307,7 +1319,7 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
         // x0.y is the node's reference axial coordinate, z0.
         auto update = [&](int code, int j, int k, double m, double2 x0, double2 fk, double2 uk, double2 vk, uint8_t fl) {
             const bool owned = code < 0;
-            if (owned) d.f[k] = fk;
+            if (owned && !open) d.f[k] = fk;
             if (!next) return;
             double a3[3] = {0.0, 0.0, 0.0};
             int cc = 0, cs = 0;
@@ -1332,8 +1344,8 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
             if (i < 97 * 3) (&sLog[wave][0][0])[i] = tv[k];
             else if (i < TABLE_DOUBLES) (&sExp[wave][0][0])[i - 97 * 3] = tv[k];
         }
-        if (has0) update(c0, lane, k0, m0, x00, gathered(c0, a0, m0), uu0, vv0, fl0);
-        if (has1) update(c1, lane + 64, k1, m1, x01, gathered(c1, a1, m1), uu1, vv1, fl1);
+        if (has0) update(c0, lane, k0, m0, x00, gathered(c0, a0, f0, m0), uu0, vv0, fl0);
+        if (has1) update(c1, lane + 64, k1, m1, x01, gathered(c1, a1, f1, m1), uu1, vv1, fl1);
     }
     if (!next) {  // the gather that closes the last step; no step follows
         if (WALL && in.pendingWall && blockIdx.x == 0)
@@ -2132,7 +2144,18 @@ struct nf_solver {
             return NF_OK;
         }
         if (fusedRun) {
-            // k_element, then k_step for each later step; closeFused adds k_canon and k_gn.
+            // k_step opening the call's first step (past step 0), or k_element for it (at step
+            // 0), then k_step for each later step; closeFused adds k_canon and k_gn.
+            if (openPending) {
+                const int k3 = (int)(fusedCount % 3);
+                TRY(timed(3, [&](hipEvent_t a, hipEvent_t b) { return launchStep(q, k3, STEP_OPEN, a, b); }));
+                openPending = false;
+                openElement = true;
+                lastK3 = k3;
+                fusedCount++;
+                q ^= 1;
+                return NF_OK;
+            }
             if (!openElement) {
                 TRY(timed(1, [&](hipEvent_t a, hipEvent_t b) { return launchElement(MODE_STEP, q, a, b, 1); }));
                 openElement = true;
@@ -2140,7 +2163,9 @@ struct nf_solver {
                 return NF_OK;
             }
             const int k3 = (int)(fusedCount % 3);
-            TRY(timed(3, [&](hipEvent_t a, hipEvent_t b) { return launchStep(q, k3, lastK3 < 0 ? 1 : 0, a, b); }));
+            TRY(timed(3, [&](hipEvent_t a, hipEvent_t b) {
+                return launchStep(q, k3, lastK3 < 0 ? STEP_AFTER_ELEMENT : STEP_AFTER_STEP, a, b);
+            }));
             lastK3 = k3;
             fusedCount++;
             q ^= 1;
@@ -2155,6 +2180,7 @@ struct nf_solver {
     // ---- the fused step (k_step): one launch per step without contact, erosion, pressure edges
     bool fusedRun = false;      // this run call enqueues fused steps
     bool openElement = false;   // an element pass is enqueued whose closing k_gn is not
+    bool openPending = false;   // the run call's first step is still to be opened (STEP_OPEN)
     int lastK3 = -1;            // rotation index of the last k_step since that element pass
     long long fusedCount = 0;   // k_step launches so far (their fMin rotation)
     int gnMinSlot = -1;         // k_gn's minSlot argument for the next node launch
@@ -2179,32 +2205,39 @@ struct nf_solver {
     }
 
     template <bool FULL, bool DAMAGE, bool KIN, bool WALL>
-    int launchStepT(int parity, int k3, int minFromStep, hipEvent_t a, hipEvent_t b) {
+    int launchStepT(int parity, int k3, int from, hipEvent_t a, hipEvent_t b) {
         const int grid = d.patchBlocks + 1;  // the last workgroup remakes the schedule
-        if (a) hipExtLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, k3, minFromStep);
-        else hipLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, k3, minFromStep);
+        if (from == STEP_OPEN) {
+            if (a) hipExtLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, true>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, k3, from);
+            else hipLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, true>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, k3, from);
+        } else {
+            if (a) hipExtLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, false>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, k3, from);
+            else hipLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, false>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, k3, from);
+        }
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
 
-    int launchStep(int parity, int k3, int minFromStep, hipEvent_t a, hipEvent_t b) {
+    // from: STEP_AFTER_STEP, STEP_AFTER_ELEMENT or STEP_OPEN (the OPEN instantiation)
+    static constexpr int STEP_OPEN = 2;
+    int launchStep(int parity, int k3, int from, hipEvent_t a, hipEvent_t b) {
         switch ((integration == NF_FULL ? 8 : 0) | (hasDamage ? 4 : 0) | (hasKin ? 2 : 0) | (hasWall ? 1 : 0)) {
-            case 0: return launchStepT<false, false, false, false>(parity, k3, minFromStep, a, b);
-            case 1: return launchStepT<false, false, false, true>(parity, k3, minFromStep, a, b);
-            case 2: return launchStepT<false, false, true, false>(parity, k3, minFromStep, a, b);
-            case 3: return launchStepT<false, false, true, true>(parity, k3, minFromStep, a, b);
-            case 4: return launchStepT<false, true, false, false>(parity, k3, minFromStep, a, b);
-            case 5: return launchStepT<false, true, false, true>(parity, k3, minFromStep, a, b);
-            case 6: return launchStepT<false, true, true, false>(parity, k3, minFromStep, a, b);
-            case 7: return launchStepT<false, true, true, true>(parity, k3, minFromStep, a, b);
-            case 8: return launchStepT<true, false, false, false>(parity, k3, minFromStep, a, b);
-            case 9: return launchStepT<true, false, false, true>(parity, k3, minFromStep, a, b);
-            case 10: return launchStepT<true, false, true, false>(parity, k3, minFromStep, a, b);
-            case 11: return launchStepT<true, false, true, true>(parity, k3, minFromStep, a, b);
-            case 12: return launchStepT<true, true, false, false>(parity, k3, minFromStep, a, b);
-            case 13: return launchStepT<true, true, false, true>(parity, k3, minFromStep, a, b);
-            case 14: return launchStepT<true, true, true, false>(parity, k3, minFromStep, a, b);
-            default: return launchStepT<true, true, true, true>(parity, k3, minFromStep, a, b);
+            case 0: return launchStepT<false, false, false, false>(parity, k3, from, a, b);
+            case 1: return launchStepT<false, false, false, true>(parity, k3, from, a, b);
+            case 2: return launchStepT<false, false, true, false>(parity, k3, from, a, b);
+            case 3: return launchStepT<false, false, true, true>(parity, k3, from, a, b);
+            case 4: return launchStepT<false, true, false, false>(parity, k3, from, a, b);
+            case 5: return launchStepT<false, true, false, true>(parity, k3, from, a, b);
+            case 6: return launchStepT<false, true, true, false>(parity, k3, from, a, b);
+            case 7: return launchStepT<false, true, true, true>(parity, k3, from, a, b);
+            case 8: return launchStepT<true, false, false, false>(parity, k3, from, a, b);
+            case 9: return launchStepT<true, false, false, true>(parity, k3, from, a, b);
+            case 10: return launchStepT<true, false, true, false>(parity, k3, from, a, b);
+            case 11: return launchStepT<true, false, true, true>(parity, k3, from, a, b);
+            case 12: return launchStepT<true, true, false, false>(parity, k3, from, a, b);
+            case 13: return launchStepT<true, true, false, true>(parity, k3, from, a, b);
+            case 14: return launchStepT<true, true, true, false>(parity, k3, from, a, b);
+            default: return launchStepT<true, true, true, true>(parity, k3, from, a, b);
         }
     }
 
@@ -2588,12 +2621,26 @@ struct nf_solver {
         long long start = -1;  // known on the host, or -1: recorded by k_open_run
         const bool fast = fastOpen && !tangent;
         fastOpen = false;  // set again only by a call that completes
+        openElement = false;
+        openPending = false;
+        lastK3 = -1;
+        fusedRun = false;
+        const bool fused = fusedEligible();
+        if (fused) TRY(fusedBuffers());
+        // Past step 0, a fused run's first launch opens the call's first step itself (k_step,
+        // STEP_OPEN); otherwise a node launch opens it.
+        bool opened = false;
         if (fast) {
             // Past step 0 with the control block as the last call left it: the opening is made
             // on the device. A call whose end has already passed runs as no-op launches.
             if (maxSteps == 0) return NF_OK;
             hipLaunchKernelGGL(k_open_run, dim3(1), dim3(1), 0, stream, ctl, q, endTime, maxSteps);
-            TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_RESUME, q, 0, a, b); }));
+            if (fused) {
+                openPending = true;
+            } else {
+                TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_RESUME, q, 0, a, b); }));
+                opened = true;
+            }
         } else {
             Step h;
             {
@@ -2612,21 +2659,22 @@ struct nf_solver {
             h.stepLimit = maxSteps > 0 ? start + maxSteps : 0x7FFFFFFFFFFFFFFFLL;
             TRY(writeStep(h));
             // Open the first step of this call.
+            opened = true;
             if (tangent) {
                 TRY(launchTangent(h.steps == 0 ? MODE_INITIAL : -1, PHASE_START, h.steps == 0 ? 1 : 0));
             } else if (h.steps == 0) {
                 TRY(launchElement(MODE_INITIAL, q));  // :945
                 TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 1, a, b); }));
+            } else if (fused) {
+                openPending = true;  // the host's end test passed; the device's agrees
+                opened = false;
             } else {
                 TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_START, q, 0, a, b); }));
             }
         }
         HIP_TRY(hipGetLastError());
-        q ^= 1;
-        openElement = false;
-        lastK3 = -1;
-        fusedRun = fusedEligible();
-        if (fusedRun) TRY(fusedBuffers());
+        if (opened) q ^= 1;
+        fusedRun = fused;
         // A step count with no end time (or a short one) is enqueued whole. With an end time the
         // device's end test decides, and the host keeps only a block of steps ahead of it (below):
         // enqueuing a large step cap whole would leave every step past the end time as a no-op
