@@ -358,10 +358,11 @@ __device__ __forceinline__ double2 gather_slots(const Dev& d, const double2* __r
     return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
 }
 __device__ __forceinline__ double2 gather_node(const Dev& d, int n) { return gather_slots(d, d.slot, n); }
-// The same gather from a fused run's node-major slots (slotN: at most four per node, in order).
-__device__ __forceinline__ double2 gather_node_major(const Dev& d, int n) {
+// The same gather from a fused run's node-major slots (slotN, or slotNB for buffer 1: at most
+// four per node, in order).
+__device__ __forceinline__ double2 gather_node_major(const Dev& d, int n, int buf) {
     const int slots = d.slotStart[n + 1] - d.slotStart[n];
-    const double2* a = d.slotN + 4 * (size_t)n;
+    const double2* a = (buf ? d.slotNB : d.slotN) + 4 * (size_t)n;
     const double2 s0 = a[0], s1 = a[1], s2 = a[2], s3 = a[3];
     double sr = 0.0, sz = 0.0;
     sr += s0.x; sz += s0.y;
@@ -458,10 +459,16 @@ __device__ void schedule_patches(const Dev& d, const uint8_t* __restrict__ costI
     }
 }
 
+__device__ void reduce_wall_partials(const Dev& d, const double* __restrict__ partial,
+                                     const int* __restrict__ countPartial, int blocks);
+
 // Reads control buffer q; block 0 writes buffer q ^ 1.
 // minSlot >= 0: the preceding element pass was the fused step kernel's, whose shortest edge is
-// in Ctl::fMin[minSlot] (and the node state has been brought back to buffer 0, k_canon).
-// nodeMajor: the gather closes a fused run, whose slots are node-major (slotN).
+// in Ctl::fMin[minSlot].
+// nodeMajor: the gather closes a fused run, whose slots are node-major, in the buffer the Step
+// names (slotN, slotNB). It also brings the node state back to buffer 0 (u, v, flags) and adds
+// the last fused launch's wall partials to the totals, which leaves the closed state every
+// other kernel and the host read.
 template <bool WALL>
 __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part, int minSlot,
                                               int nodeMajor) {
@@ -476,6 +483,12 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
     const Step& in = c->s[q];
     Step& out = c->s[q ^ 1];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const int i0 = blockIdx.x * BLOCK + threadIdx.x;
+    if (nodeMajor && in.buf && i0 < d.N) {  // buffer 1 back to buffer 0, each thread its node
+        d.u[i0] = d.uB[i0];
+        d.v[i0] = d.vB[i0];
+        d.flags[i0] = d.flagsB[i0];
+    }
     if (phase == PHASE_STEP && !in.go) {  // nothing ran: carry the state over
         if (lead && part != PART_GATHER) {
             Step o = in;
@@ -493,9 +506,12 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
             return;
         }
         const int i = blockIdx.x * BLOCK + threadIdx.x;
-        if (gather && i < d.N) d.f[i] = nodeMajor ? gather_node_major(d, i) : gather_node(d, i);
+        if (gather && i < d.N) d.f[i] = nodeMajor ? gather_node_major(d, i, in.buf) : gather_node(d, i);
         return;
     }
+    if (nodeMajor && WALL && in.pendingWall && blockIdx.x == 0)
+        reduce_wall_partials(d, d.wallPartialF + (size_t)q * 3 * d.patchBlocks,
+                             d.wallCountPartialF + (size_t)q * 2 * d.patchBlocks, d.patchBlocks);
     if (part == PART_REST && gathered && blockIdx.x == 0) reduce_contact_partials(d);
     if (phase == PHASE_STEP && d.gone && blockIdx.x == 0) {
         // eroded energy of step n: the element launch's per-patch (or, without a schedule,
@@ -545,7 +561,7 @@ __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gathe
     double2 fi = make_double2(0.0, 0.0);
     if (i < d.N) {
         if (gather) {
-            fi = nodeMajor ? gather_node_major(d, i) : gather_node(d, i);
+            fi = nodeMajor ? gather_node_major(d, i, in.buf) : gather_node(d, i);
             d.f[i] = fi;
         } else {
             fi = d.f[i];
@@ -1160,7 +1176,8 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
 // One launch per step where the step has no penalty contact, erosion or pressure edges: the
 // node launch that closes step n - 1 and opens step n (k_gn) and the element launch of step n
 // (k_element) in one kernel. The host enqueues it in place of the pair k_gn, k_element, so a
-// run is: k_element, k_step ... k_step, k_canon, k_gn.
+// run call is: k_step (opening its first step) ... k_step, then the closing k_gn (at step 0:
+// k_element, k_gn, k_element, k_step ... k_step, k_gn).
 //
 // Each wave takes a patch of 64 elements from the element pass's schedule and first does the
 // node launch's work for every node of the patch: the fixed-order gather of step n - 1's force
@@ -1179,16 +1196,21 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_element(Dev
 // launch's costs) for the launch after next. The shortest edge goes through Ctl::fMin: the
 // launch with rotation index k3 reads slot k3 (or, first after a plain element launch, Step q),
 // lowers slot (k3 + 1) % 3 and resets slot (k3 + 2) % 3. Wall partials go to
-// wallPartialF[q ^ 1]; the next launch (or k_canon) adds them up when Step::pendingWall says so.
+// wallPartialF[q ^ 1]; the next launch (or the closing k_gn) adds them up when
+// Step::pendingWall says so.
+//
 // from: where this launch's step comes from. STEP_AFTER_STEP: a k_step launch, whose shortest
 // edge is in Ctl::fMin[k3]; STEP_AFTER_ELEMENT: the two-launch element pass, whose shortest edge
 // is in the Step. OPEN (its own instantiation, so that the other launches' code is untouched):
 // the first launch of a run call, which opens its first step from the closed state a run call
 // leaves (the step's dt already taken, the forces gathered in d.f), as the node launch's
-// PHASE_RESUME does, with the end test made on the device.
+// PHASE_RESUME does, with the end test made on the device; it also opens the call itself, as
+// k_open_run does: the end time, and a step limit maxSteps past the step count it finds (none
+// when maxSteps < 0).
 constexpr int STEP_AFTER_STEP = 0, STEP_AFTER_ELEMENT = 1;
 template <bool FULL, bool DAMAGE, bool KIN, bool WALL, bool OPEN>
-__global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d, int q, int k3, int from) {
+__global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d, int q, int k3, int from,
+                                                                          double endTime, long long maxSteps) {
     if (blockIdx.x == gridDim.x - 1) {
         schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)q * 4 * d.patchBlocks,
                          d.patchRun + (size_t)q * 4 * d.patchBlocks);
@@ -1221,10 +1243,18 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
         dtPrev = in.dt;
         dt = d.cfl * __builtin_sqrt(minEdge * d.massScale) / d.fastestWave;
     }
-    const bool next = in.time < in.endTime && in.steps < in.stepLimit;
+    const double stepEnd = OPEN ? endTime : in.endTime;
+    const long long stepLimit = OPEN ? (maxSteps > 0 ? in.steps + maxSteps : 0x7FFFFFFFFFFFFFFFLL) : in.stepLimit;
+    const bool next = in.time < stepEnd && in.steps < stepLimit;
     const int b = in.buf;
     if (lead) {
         Step o = in;
+        if (OPEN) {
+            o.endTime = stepEnd;
+            o.stepLimit = stepLimit;
+            c->cOverflow = 0;
+            c->runStart = in.steps;
+        }
         o.dt = dt;
         o.dtPrev = dtPrev;
         o.go = next ? 1 : 0;
@@ -1445,26 +1475,6 @@ __global__ void __launch_bounds__(BLOCK, (FULL || DAMAGE) ? 2 : 3) k_step(Dev d,
     if (WALL && in.pendingWall && blockIdx.x == 0)
         reduce_wall_partials(d, d.wallPartialF + (size_t)q * 3 * d.patchBlocks,
                              d.wallCountPartialF + (size_t)q * 2 * d.patchBlocks, d.patchBlocks);
-}
-
-// Closes a run of fused steps: adds the last launch's wall partials to the totals and, if the
-// current state is in buffer 1, copies node state and force slots back to buffer 0, where
-// every other kernel and the host read them. Reads Step q; the k_gn that follows writes
-// Step q ^ 1 with buf = 0.
-template <bool WALL>
-__global__ void __launch_bounds__(BLOCK) k_canon(Dev d, int q) {
-    const Step& in = d.ctl->s[q];
-    if (WALL && in.pendingWall && blockIdx.x == 0)
-        reduce_wall_partials(d, d.wallPartialF + (size_t)q * 3 * d.patchBlocks,
-                             d.wallCountPartialF + (size_t)q * 2 * d.patchBlocks, d.patchBlocks);
-    if (!in.buf) return;
-    const size_t stride = (size_t)gridDim.x * BLOCK;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < (size_t)d.N; i += stride) {
-        d.u[i] = d.uB[i];
-        d.v[i] = d.vB[i];
-        d.flags[i] = d.flagsB[i];
-    }
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < 4 * (size_t)d.N; i += stride) d.slotN[i] = d.slotNB[i];
 }
 
 // Sets the current step control's end time and step limit on the device (no host round trip).
@@ -2145,7 +2155,7 @@ struct nf_solver {
         }
         if (fusedRun) {
             // k_step opening the call's first step (past step 0), or k_element for it (at step
-            // 0), then k_step for each later step; closeFused adds k_canon and k_gn.
+            // 0), then k_step for each later step; closeFused adds the closing k_gn.
             if (openPending) {
                 const int k3 = (int)(fusedCount % 3);
                 TRY(timed(3, [&](hipEvent_t a, hipEvent_t b) { return launchStep(q, k3, STEP_OPEN, a, b); }));
@@ -2181,6 +2191,8 @@ struct nf_solver {
     bool fusedRun = false;      // this run call enqueues fused steps
     bool openElement = false;   // an element pass is enqueued whose closing k_gn is not
     bool openPending = false;   // the run call's first step is still to be opened (STEP_OPEN)
+    double openEnd = 0.0;       // that call's end time and step count (k_step's opening)
+    long long openMaxSteps = 0;
     int lastK3 = -1;            // rotation index of the last k_step since that element pass
     long long fusedCount = 0;   // k_step launches so far (their fMin rotation)
     int gnMinSlot = -1;         // k_gn's minSlot argument for the next node launch
@@ -2207,12 +2219,14 @@ struct nf_solver {
     template <bool FULL, bool DAMAGE, bool KIN, bool WALL>
     int launchStepT(int parity, int k3, int from, hipEvent_t a, hipEvent_t b) {
         const int grid = d.patchBlocks + 1;  // the last workgroup remakes the schedule
+        const double e = openEnd;
+        const long long m = openMaxSteps;
         if (from == STEP_OPEN) {
-            if (a) hipExtLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, true>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, k3, from);
-            else hipLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, true>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, k3, from);
+            if (a) hipExtLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, true>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, k3, from, e, m);
+            else hipLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, true>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, k3, from, e, m);
         } else {
-            if (a) hipExtLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, false>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, k3, from);
-            else hipLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, false>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, k3, from);
+            if (a) hipExtLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, false>), dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, k3, from, e, m);
+            else hipLaunchKernelGGL((k_step<FULL, DAMAGE, KIN, WALL, false>), dim3(grid), dim3(BLOCK), 0, stream, d, parity, k3, from, e, m);
         }
         HIP_TRY(hipGetLastError());
         return NF_OK;
@@ -2241,16 +2255,10 @@ struct nf_solver {
         }
     }
 
-    // Ends a run of fused steps: the last element pass's gather (k_gn), after k_canon has
-    // brought the state back to buffer 0 if any k_step ran.
+    // Ends a run of fused steps: the last element pass's gather (k_gn), which also brings the
+    // node state back to buffer 0 and adds the last fused launch's wall partials.
     int closeFused() {
         if (!openElement) return NF_OK;
-        if (lastK3 >= 0) {
-            const int grid = std::min(blocks_for(4 * E > N ? 4 * E : N), 1024);
-            if (hasWall) hipLaunchKernelGGL(k_canon<true>, dim3(grid), dim3(BLOCK), 0, stream, d, q);
-            else hipLaunchKernelGGL(k_canon<false>, dim3(grid), dim3(BLOCK), 0, stream, d, q);
-            HIP_TRY(hipGetLastError());
-        }
         gnMinSlot = lastK3 >= 0 ? (lastK3 + 1) % 3 : -1;
         gnNodeMajor = true;
         const int rc = timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_STEP, q, 0, a, b); });
@@ -2634,10 +2642,10 @@ struct nf_solver {
             // Past step 0 with the control block as the last call left it: the opening is made
             // on the device. A call whose end has already passed runs as no-op launches.
             if (maxSteps == 0) return NF_OK;
-            hipLaunchKernelGGL(k_open_run, dim3(1), dim3(1), 0, stream, ctl, q, endTime, maxSteps);
             if (fused) {
-                openPending = true;
+                openPending = true;  // the opening k_step opens the call too (k_open_run's work)
             } else {
+                hipLaunchKernelGGL(k_open_run, dim3(1), dim3(1), 0, stream, ctl, q, endTime, maxSteps);
                 TRY(timed(0, [&](hipEvent_t a, hipEvent_t b) { return launchGN(PHASE_RESUME, q, 0, a, b); }));
                 opened = true;
             }
@@ -2675,6 +2683,8 @@ struct nf_solver {
         HIP_TRY(hipGetLastError());
         if (opened) q ^= 1;
         fusedRun = fused;
+        openEnd = endTime;
+        openMaxSteps = maxSteps;
         // A step count with no end time (or a short one) is enqueued whole. With an end time the
         // device's end test decides, and the host keeps only a block of steps ahead of it (below):
         // enqueuing a large step cap whole would leave every step past the end time as a no-op
