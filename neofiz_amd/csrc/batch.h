@@ -461,6 +461,9 @@ static int batch_copy(T* dst, const T* src, size_t count, hipStream_t stream) {
 static int batch_transfer(nf_batch* b, bool toBatch) {
     for (size_t i = 0; i < b->solvers.size(); i++) {
         nf_solver* s = b->solvers[i];
+        // the solver's control block and state change underneath it (back from the batch)
+        s->touch();
+        s->fastOpen = false;
         const BInst& I = b->inst[i];
         const size_t nb = I.nodeBase, eb = I.elemBase, pb = (size_t)I.elemBase * s->ppe;
         const size_t N = s->N, E = s->E, P = s->P;

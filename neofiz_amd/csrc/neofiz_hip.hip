@@ -2029,8 +2029,18 @@ struct nf_solver {
     }
 
     // The control block's host copy goes through pinned memory (ctlHost), which the copy
-    // engine reads and writes directly, without a staging copy.
+    // engine reads and writes directly, without a staging copy. A run call ends by reading the
+    // control block with the device idle; until the next entry point that may change it
+    // (touch()), readCtl returns that copy without waiting on the device, so that a harness
+    // asking for the time after every step pays no second round trip.
+    Ctl ctlCache;
+    bool ctlCached = false;
+    void touch() { ctlCached = false; }
     int readCtl(Ctl& out) {
+        if (ctlCached) {
+            out = ctlCache;
+            return NF_OK;
+        }
         if (!ctlHost) HIP_TRY(hipHostMalloc((void**)&ctlHost, sizeof(Ctl), hipHostMallocDefault));
         HIP_TRY(hipMemcpyAsync(ctlHost, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
@@ -2040,6 +2050,7 @@ struct nf_solver {
 
     int writeCtl(const Ctl& in) {
         fastOpen = false;
+        touch();
         HIP_TRY(hipMemcpyAsync(ctl, &in, sizeof(Ctl), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         return NF_OK;
@@ -2055,6 +2066,7 @@ struct nf_solver {
 
     int writeStep(const Step& in) {
         fastOpen = false;
+        touch();
         HIP_TRY(hipMemcpyAsync(&ctl->s[q], &in, sizeof(Step), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         return NF_OK;
@@ -2376,6 +2388,7 @@ struct nf_solver {
 
     int restoreCheckpoint(Checkpoint& c) {
         fastOpen = false;
+        touch();
         TRY(checkpointCopy(c, false));
         q = c.q;
         return NF_OK;
@@ -2626,6 +2639,7 @@ struct nf_solver {
         if (tangent && (hasContact || hasDamage || hasErosion))
             return fail(NF_E_STATE, "the sensitivity kernels do not carry contact, damage or erosion");
         if (taken) *taken = 0;
+        touch();
         long long start = -1;  // known on the host, or -1: recorded by k_open_run
         const bool fast = fastOpen && !tangent;
         fastOpen = false;  // set again only by a call that completes
@@ -2725,6 +2739,8 @@ struct nf_solver {
         TRY(readCtl(full));
         const Step h = full.s[q];
         TRY(harvest());
+        ctlCache = full;
+        ctlCached = true;
         if (taken) *taken = h.steps - (start >= 0 ? start : full.runStart);
         fastOpen = !tangent && h.steps > 0;
         if (full.cOverflow)
@@ -3090,6 +3106,7 @@ int nf_solver_destroy(nf_solver* s) {
 }
 
 int nf_set_threads(nf_solver* s, int32_t n) {
+    s->touch();
     g_error.clear();
     if (n < 1) return fail(NF_E_ARGUMENT, "need at least one thread");
     return NF_OK;  // one GPU does the work; the answer never depended on the count
@@ -3107,28 +3124,33 @@ static int upload_flags(nf_solver* s) {
 }
 
 int nf_fix_radial(nf_solver* s, int32_t node) {
+    s->touch();
     g_error.clear();
     if (node < 0 || node >= s->N) return fail(NF_E_ARGUMENT, "node out of range");
     s->flagsHost[node] |= FIXED_R;
     return upload_flags(s);
 }
 int nf_fix_axial(nf_solver* s, int32_t node) {
+    s->touch();
     g_error.clear();
     if (node < 0 || node >= s->N) return fail(NF_E_ARGUMENT, "node out of range");
     s->flagsHost[node] |= FIXED_Z;
     return upload_flags(s);
 }
 int nf_fix_all_axial(nf_solver* s) {
+    s->touch();
     for (auto& f : s->flagsHost) f |= FIXED_Z;
     return upload_flags(s);
 }
 int nf_set_pressure_ramp(nf_solver* s, double pascals, double ramp) {
+    s->touch();
     s->d.targetPressure = pascals;
     s->d.rampDuration = ramp;
     s->graphDirty = true;
     return NF_OK;
 }
 int nf_set_relaxation_damping(nf_solver* s, double fraction, double omega) {
+    s->touch();
     g_error.clear();
     if (omega <= 0.0) return fail(NF_E_ARGUMENT, "angular frequency must be positive");
     s->d.damping = 2.0 * fraction * omega;
@@ -3136,6 +3158,7 @@ int nf_set_relaxation_damping(nf_solver* s, double fraction, double omega) {
     return NF_OK;
 }
 int nf_set_hourglass_coefficient(nf_solver* s, double q) {
+    s->touch();
     g_error.clear();
     if (q < 0.0) return fail(NF_E_ARGUMENT, "hourglass coefficient must be >= 0");
     s->d.qhg = q;
@@ -3143,6 +3166,7 @@ int nf_set_hourglass_coefficient(nf_solver* s, double q) {
     return NF_OK;
 }
 int nf_set_rigid_wall(nf_solver* s, int32_t enabled, double plane_z, double friction) {
+    s->touch();
     g_error.clear();
     if (!enabled) {
         s->hasWall = false;
@@ -3174,6 +3198,7 @@ static int refuse_if_sensitive(const nf_solver* s, const char* what) {
 }
 
 int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double damping, double scale) {
+    s->touch();
     g_error.clear();
     if (enabled) TRY(refuse_if_sensitive(s, "contact"));
     if (!enabled) {
@@ -3344,6 +3369,7 @@ int nf_set_contact(nf_solver* s, int32_t enabled, double friction, double dampin
     return NF_OK;
 }
 int nf_contact_set_limits(nf_solver* s, int32_t capture_capacity, int32_t max_terms) {
+    s->touch();
     g_error.clear();
     if (!s->hasContact) return fail(NF_E_STATE, "contact is not set");
     if (max_terms == 0 || max_terms > C_MAX_TERMS || max_terms < -1)
@@ -3355,6 +3381,7 @@ int nf_contact_set_limits(nf_solver* s, int32_t capture_capacity, int32_t max_te
     return NF_OK;
 }
 int nf_set_gravity(nf_solver* s, double ar, double az) {
+    s->touch();
     s->d.gravityR = ar;
     s->d.gravityZ = az;
     s->graphDirty = true;
@@ -3365,6 +3392,7 @@ int nf_set_damage(nf_solver* s, const double* gf, const double* eps) {  // :729-
     TRY(refuse_if_sensitive(s, "damage"));
     if (!gf || !eps) return fail(NF_E_ARGUMENT, "fracture energy and failure strain are required");
     for (const HostMaterial& m : s->materials) {
+    s->touch();
         const double worst = 2.0 * m.mu() + (2.0 / 3.0) * (m.raw.kinematic_hardening - m.raw.youngs_modulus);
         if (!(worst > 0.0)) return fail(NF_E_ARGUMENT, "a fully clamped softening modulus would make the return map singular");
     }
@@ -3429,6 +3457,7 @@ int nf_set_damage(nf_solver* s, const double* gf, const double* eps) {  // :729-
     return NF_OK;
 }
 int nf_set_erosion(nf_solver* s, double threshold) {  // :549-561
+    s->touch();
     g_error.clear();
     TRY(refuse_if_sensitive(s, "erosion"));
     if (!s->hasDamage) return fail(NF_E_STATE, "erosion needs a failure model; call setDamage first");
@@ -3436,6 +3465,7 @@ int nf_set_erosion(nf_solver* s, double threshold) {  // :549-561
         return fail(NF_E_ARGUMENT, "the erosion threshold is a damage, so it lies in (0, 1]");
     s->d.erosionDamage = threshold;
     if (!s->d.gone) {
+    s->touch();
         uint8_t* g = nullptr;
         TRY(s->alloc(&g, s->E));
         s->d.gone = g;
@@ -3445,6 +3475,7 @@ int nf_set_erosion(nf_solver* s, double threshold) {  // :549-561
     return NF_OK;
 }
 int nf_set_mass_scaling(nf_solver* s, double target) {  // :601-619
+    s->touch();
     g_error.clear();
     if (!(target > 0.0)) return fail(NF_E_ARGUMENT, "the target step must be positive");
     Step h;
@@ -3464,6 +3495,7 @@ int nf_set_mass_scaling(nf_solver* s, double target) {  // :601-619
     return s->writeStep(h);
 }
 int nf_set_velocity(nf_solver* s, int32_t node, double vr, double vz) {
+    s->touch();
     g_error.clear();
     if (node < 0 || node >= s->N) return fail(NF_E_ARGUMENT, "node out of range");
     const double2 v = make_double2(vr, vz);
@@ -3471,17 +3503,20 @@ int nf_set_velocity(nf_solver* s, int32_t node, double vr, double vz) {
     return s->sync();
 }
 int nf_set_uniform_velocity(nf_solver* s, double vr, double vz) {
+    s->touch();
     hipLaunchKernelGGL(k_fill2, dim3(blocks_for(s->N)), dim3(BLOCK), 0, s->stream, s->d.v, s->N, vr, vz);
     HIP_TRY(hipGetLastError());
     return s->sync();
 }
 int nf_run(nf_solver* s, int64_t steps) {
+    s->touch();
     g_error.clear();
     if (steps <= 0) return NF_OK;
     long long taken = 0;
     return s->runSteps(HUGE_VAL, steps, &taken);
 }
 int nf_run_until(nf_solver* s, double end_time, int64_t max_steps, int64_t* steps_taken) {
+    s->touch();
     g_error.clear();
     long long taken = 0;
     TRY(s->runSteps(end_time, max_steps > 0 ? max_steps : -1, &taken));
@@ -3489,6 +3524,7 @@ int nf_run_until(nf_solver* s, double end_time, int64_t max_steps, int64_t* step
     return NF_OK;
 }
 int nf_advance(nf_solver* s) {
+    s->touch();
     g_error.clear();
     hipLaunchKernelGGL(k_advance_disp, dim3(blocks_for(s->N)), dim3(BLOCK), 0, s->stream, s->d, s->q);
     HIP_TRY(hipGetLastError());
@@ -3526,6 +3562,7 @@ int nf_get_node_field(nf_solver* s, int32_t field, double* out) {
 }
 
 int nf_put_node_field(nf_solver* s, int32_t field, const double* in) {
+    s->touch();
     g_error.clear();
     if (field < NF_NODE_UR || field > NF_NODE_VZ) return fail(NF_E_ARGUMENT, "only UR, UZ, VR and VZ may be written");
     double2* dst = field <= NF_NODE_UZ ? s->d.u : s->d.v;
@@ -3643,6 +3680,7 @@ int nf_film_capture(nf_solver* s, int32_t field_count, const int32_t* fields, do
                     double* values) {
     g_error.clear();
     if (!s) return fail(NF_E_ARGUMENT, "null handle");
+    s->touch();
     if (field_count < 0 || field_count > FILM_MAX)
         return fail(NF_E_ARGUMENT, "a frame carries 0 to " + std::to_string(FILM_MAX) + " fields");
     if (!r || !z || (field_count && (!fields || !values))) return fail(NF_E_ARGUMENT, "null array");
@@ -3710,6 +3748,7 @@ int nf_get_state(nf_solver* s, nf_state* o) {
 
 // The contact's pairs of the last step, in surface-node order, as (node, a, b).
 int nf_contact_pairs(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count) {
+    s->touch();
     g_error.clear();
     if (!count) return fail(NF_E_ARGUMENT, "count must not be null");
     *count = 0;
@@ -3736,6 +3775,7 @@ int nf_contact_pairs(nf_solver* s, int32_t capacity, int32_t* out, int32_t* coun
 
 // The contact's current surface segments (a, b) in surface order.
 int nf_contact_surface(nf_solver* s, int32_t capacity, int32_t* out, int32_t* count) {
+    s->touch();
     g_error.clear();
     if (!count) return fail(NF_E_ARGUMENT, "count must not be null");
     *count = 0;
@@ -3759,6 +3799,7 @@ int nf_contact_surface(nf_solver* s, int32_t capacity, int32_t* out, int32_t* co
 // ExplicitSolver's single-value queries (solver/ExplicitSolver.java:826, :1778-1794,
 // :1832-1840, :1867-1877, :1928-1930), from the device state copied out.
 int nf_temperature(nf_solver* s, int32_t point, double* out) {
+    s->touch();
     g_error.clear();
     if (point < 0 || point >= s->P) return fail(NF_E_ARGUMENT, "point out of range");
     const HostMaterial& m = s->materials[s->materialOf.empty() ? 0 : s->materialOf[point / s->ppe]];
@@ -3772,6 +3813,7 @@ int nf_temperature(nf_solver* s, int32_t point, double* out) {
 }
 
 int nf_plastic_front_radius(nf_solver* s, double* out) {
+    s->touch();
     g_error.clear();
     std::vector<double> ep(s->P);
     TRY(nf_get_point_field(s, NF_POINT_PLASTIC_STRAIN, ep.data()));
@@ -3790,6 +3832,7 @@ int nf_plastic_front_radius(nf_solver* s, double* out) {
 }
 
 int nf_most_damaged_element(nf_solver* s, int32_t* out) {
+    s->touch();
     g_error.clear();
     *out = -1;
     if (!s->hasDamage) return NF_OK;
@@ -3813,6 +3856,7 @@ int nf_bore_pressure(nf_solver* s, double* out) {  // pressureAt(time) :1203-120
     TRY(s->readStep(h));
     const Dev& d = s->d;
     if (d.rampDuration <= 0.0) {
+    s->touch();
         *out = d.targetPressure;
         return NF_OK;
     }
@@ -3823,6 +3867,7 @@ int nf_bore_pressure(nf_solver* s, double* out) {  // pressureAt(time) :1203-120
 }
 
 int nf_mesh_z(nf_solver* s, int32_t node, double* out) {
+    s->touch();
     g_error.clear();
     if (node < 0 || node >= s->N) return fail(NF_E_ARGUMENT, "node out of range");
     *out = s->z[node];
@@ -3838,6 +3883,7 @@ int nf_set_damage_uniform(nf_solver* s, double fracture_energy, const double* fa
 // Audits from the device state, in the reference's summation order
 // (solver/ExplicitSolver.java:875-886, :1610-1860).
 int nf_get_audits(nf_solver* s, nf_audits* o) {
+    s->touch();
     g_error.clear();
     std::memset(o, 0, sizeof *o);
     const int N = s->N, E = s->E, ppe = s->ppe, P = s->P;
@@ -4000,6 +4046,7 @@ int nf_get_audits(nf_solver* s, nf_audits* o) {
 // nf_profile_read returns the summed milliseconds and launch counts (indices 0 nodal,
 // 1 element, 2 contact, 3 fused step) and resets them.
 int nf_profile(nf_solver* s, int32_t enable) {
+    s->touch();
     g_error.clear();
     TRY(s->harvest());
     s->profiling = enable != 0;
@@ -4007,6 +4054,7 @@ int nf_profile(nf_solver* s, int32_t enable) {
 }
 
 int nf_profile_read(nf_solver* s, double* ms, int64_t* counts) {
+    s->touch();
     g_error.clear();
     TRY(s->harvest());
     for (int k = 0; k < 4; k++) {
@@ -4107,6 +4155,7 @@ int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int3
 // ------------------------------------------------------------------ sensitivities
 
 int nf_enable_sensitivity(nf_solver* s) {
+    s->touch();
     g_error.clear();
     if (s->tangent) return NF_OK;
     Step h;
@@ -4139,6 +4188,7 @@ int nf_enable_sensitivity(nf_solver* s) {
 }
 
 int nf_get_node_sensitivity(nf_solver* s, int32_t field, int32_t param, double* out) {
+    s->touch();
     g_error.clear();
     if (!s->tangent) return fail(NF_E_STATE, "sensitivities are not enabled");
     if (param < 0 || param >= NP) return fail(NF_E_ARGUMENT, "parameter out of range");
@@ -4158,6 +4208,7 @@ int nf_get_node_sensitivity(nf_solver* s, int32_t field, int32_t param, double* 
 }
 
 int nf_get_step_sensitivity(nf_solver* s, double* out) {
+    s->touch();
     g_error.clear();
     if (!s->tangent) return fail(NF_E_STATE, "sensitivities are not enabled");
     TRY(s->sync());
@@ -4166,6 +4217,7 @@ int nf_get_step_sensitivity(nf_solver* s, double* out) {
 }
 
 int nf_adjoint_enable(nf_solver* s, int32_t checkpoint_interval) {
+    s->touch();
     g_error.clear();
     if (s->adjoint) return NF_OK;
     if (checkpoint_interval < 1 || checkpoint_interval > 4096)
@@ -4185,6 +4237,7 @@ int nf_adjoint_enable(nf_solver* s, int32_t checkpoint_interval) {
 }
 
 int nf_adjoint_gradient(nf_solver* s, int32_t objective_count, const double* seeds, double* gradient) {
+    s->touch();
     g_error.clear();
     if (!s->adjoint) return fail(NF_E_STATE, "the adjoint is not enabled");
     if (objective_count < 1 || objective_count > AW_MAX)
@@ -4198,6 +4251,7 @@ int nf_adjoint_gradient(nf_solver* s, int32_t objective_count, const double* see
 }
 
 int nf_adjoint_info(nf_solver* s, int64_t* out) {
+    s->touch();
     g_error.clear();
     out[0] = s->finalState.steps;
     out[1] = (int64_t)s->checkpoints.size();
