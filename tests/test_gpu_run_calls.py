@@ -88,3 +88,33 @@ def test_calls_that_take_no_step_change_nothing():
     same(state(a), state(b), "continued after no-op calls")
     a.shutdown()
     b.shutdown()
+
+
+def test_step_loop_queries_match_the_oracle():
+    """The reference harness's loop (validate/TaylorImpactCase.java:255-259): step() until
+    time() passes the duration, asking for the time after every step. The time and step count
+    after each call, the state at the end, and the audits a new rigid wall resets, all as the
+    oracle has them."""
+    gpu, ref = rod(ExplicitSolver), rod(OracleSolver)
+    duration = 1e-6
+    seen = []
+    for s in (gpu, ref):
+        trace = []
+        while s.time() < duration:
+            s.step()
+            st = s.state()
+            trace.append((st.time, st.steps, st.dt))
+        seen.append(trace)
+    assert seen[0] == seen[1] and len(seen[0]) > 20
+    same(state(gpu), state(ref), "after the step loop")
+    for s in (gpu, ref):
+        assert s.state().wall_impulse != 0.0
+        s.setRigidWall(RigidWall.atZ(0.0, 0.05))  # a new wall starts with zero audits
+    assert gpu.state().wall_impulse == ref.state().wall_impulse == 0.0
+    gpu.run(3)
+    ref.run(3)
+    a, b = gpu.state(), ref.state()
+    assert (a.time, a.steps) == (b.time, b.steps)
+    assert abs(a.wall_impulse - b.wall_impulse) <= 1e-12 * abs(b.wall_impulse)
+    gpu.shutdown()
+    ref.shutdown()
