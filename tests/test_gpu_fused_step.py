@@ -25,6 +25,7 @@ pytestmark = pytest.mark.gpu
 
 NODE = ("ur", "uz", "vr", "vz", "fr", "fz", "mass", "on_wall")
 POINT = ("stress", "back", "plastic_strain", "plastic_work")
+DAMAGE = ("damage_strength", "damage_softening", "damage_width", "damage")
 
 
 def build(fn, fused):
@@ -43,7 +44,7 @@ def compare(a, b, label):
     for f in NODE:
         d = first_difference(a.nodeField(f), b.nodeField(f))
         assert d is None, f"{label}: {f}: {d}"
-    for f in POINT:
+    for f in POINT + (DAMAGE if getattr(a, "_damaged", False) else ()):
         d = first_difference(a.pointField(f), b.pointField(f))
         assert d is None, f"{label}: {f}: {d}"
     sa, sb = a.state(), b.state()
@@ -104,6 +105,20 @@ def fan(integration):
     return make
 
 
+def damaged_rod(integration):
+    """The Johnson-Cook rod with damage and no erosion (the fused step carries damage; erosion
+    leaves the solver on the two-launch step): failure strains low enough that points soften
+    within the run."""
+    def make():
+        mesh, s, _ = taylor.build(taylor.ROD_JC, taylor.ROD_LENGTH, taylor.ROD_DIAMETER, taylor.ROD_SPEED,
+                                  8, 80, integration, 0.09, ExplicitSolver)
+        rng = np.random.default_rng(3)
+        s.setDamage(np.full(mesh.elementCount, 2.0e3), rng.uniform(0.02, 0.08, mesh.elementCount))
+        s._damaged = True
+        return s
+    return make
+
+
 CASES = {
     "jc-rod-friction": rod(16, 160, 0.09),
     "jc-rod-welded": rod(8, 80, math.inf),
@@ -111,6 +126,8 @@ CASES = {
     "kinematic-gravity-damping": block(REDUCED, FINITE_STRAIN, AXISYMMETRIC, True, True, True),
     "plane-strain-full-small": block(FULL, SMALL_STRAIN, PLANE_STRAIN),
     "fan-six-slot-node": fan(REDUCED),
+    "jc-rod-damage": damaged_rod(REDUCED),
+    "jc-rod-damage-full": damaged_rod(FULL),
 }
 
 
