@@ -162,7 +162,9 @@ __device__ __forceinline__ int ff_leading_bisections(const FFSolve& f, double lo
 // a quarter of one for each leading bisection it proved (ff_leading_bisections); the element
 // pass schedules its waves by it.
 // FF = false evaluates every iteration (the device self-check compares the two forms).
-template <bool UNIT_RATE, bool REC = false, bool FF = true>
+// Y01: the strain exponent n lies in (0, 1), so the loop's pow needs no range selects
+// (nfm::pow_and_log_ordinary_y01, the same bits).
+template <bool UNIT_RATE, bool REC = false, bool FF = true, bool Y01 = false>
 __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, double plasticStrain,
                                                 double thermal, double dt, const Mat& j,
                                                 const nfm::Tables& T, int* status = nullptr,
@@ -195,7 +197,8 @@ __device__ __forceinline__ double j2_solve_loop(double norm, double stiffness, d
         const double eps = plasticStrain + dEps;
         const double rate = dt > 0.0 ? dEps / dt : 0.0;
         double pw, lg;
-        nfm::pow_and_log_ordinary(eps, j.n, UNIT_RATE ? rate : rate / j.eps0, T, pw, lg);
+        if constexpr (Y01) nfm::pow_and_log_ordinary_y01(eps, j.n, UNIT_RATE ? rate : rate / j.eps0, T, pw, lg);
+        else nfm::pow_and_log_ordinary(eps, j.n, UNIT_RATE ? rate : rate / j.eps0, T, pw, lg);
         const double strain = eps <= 0.0 ? j.a : j.a + j.b * pw;  // strainTerm
         const bool clamped = rate <= j.eps0;
         const double rateFactor = clamped ? 1.0 : 1.0 + j.c * lg;  // rateTerm
@@ -221,8 +224,11 @@ __device__ __forceinline__ double j2_solve(double norm, double stiffness, double
                                            double thermal, double dt, const Mat& j,
                                            const nfm::Tables& T, int* status = nullptr,
                                            int* iterations = nullptr) {
-    return j.eps0 == 1.0 ? j2_solve_loop<true, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status, iterations)
-                         : j2_solve_loop<false, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status, iterations);
+    if (j.eps0 != 1.0)
+        return j2_solve_loop<false, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status, iterations);
+    return j.n > 0.0 && j.n < 1.0
+               ? j2_solve_loop<true, REC, true, true>(norm, stiffness, plasticStrain, thermal, dt, j, T, status, iterations)
+               : j2_solve_loop<true, REC>(norm, stiffness, plasticStrain, thermal, dt, j, T, status, iterations);
 }
 
 // Damage state of one Gauss point (solver/Damage.java:156-192).

@@ -331,6 +331,28 @@ NF_HD void pow_and_log_ordinary(double x, double y, double z, const Tables& T, d
     }
 }
 
+// pow_and_log_ordinary for 0 < y < 1 (the Johnson-Cook strain exponent): with x ordinary,
+// log x lies in [-708.4, 709.79] and y log x, rounded as a double-double, in the same interval,
+// inside exp's range [-745.2, 709.782712893384], where exp_dd is exp_dd_main exactly; so the
+// range selects are left out and the results are pow_and_log_ordinary's, bit for bit.
+NF_HD void pow_and_log_ordinary_y01(double x, double y, double z, const Tables& T, double& pw, double& lg) {
+    if (pow_input_ordinary(x, y) && z >= 0x1p-1022 && z < inf()) {
+        double l_hi, l_lo, m_hi, m_lo;
+        log_dd_impl<false>(x, l_hi, l_lo, T.log);
+        log_dd_impl<false>(z, m_hi, m_lo, T.log);
+        lg = m_hi + m_lo;
+        double p_hi, p_lo;
+        two_prod(y, l_hi, p_hi, p_lo);
+        p_lo = p_lo + y * l_lo;
+        double t_hi, t_lo;
+        fast_two_sum(p_hi, p_lo, t_hi, t_lo);
+        pw = exp_dd_main(t_hi, t_lo, T.exp);
+    } else {
+        pw = pow_ordinary(x, y, T);
+        lg = log_ordinary(z, T);
+    }
+}
+
 // p1 = pow_ordinary(x1, y1), p2 = pow_ordinary(x2, y2)
 NF_HD void pow_pair_ordinary(double x1, double y1, double x2, double y2, const Tables& T, double& p1,
                              double& p2) {
