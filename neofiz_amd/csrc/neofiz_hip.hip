@@ -1731,6 +1731,13 @@ __global__ void k_math_eval(int fn, const double* x, const double* y, double* ou
             r = fn == 16 ? p1 : p2;
             break;
         }
+        case 24:
+        case 25: {  // the Newton loop's pow(x, y) and log(y) for 0 < y < 1, no range selects
+            double pw, lg;
+            nfm::pow_and_log_ordinary_y01(x[i], y[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, pw, lg);
+            r = fn == 24 ? pw : lg;
+            break;
+        }
         // the hardware's single-precision log2, exp2 and reciprocal, whose error the Newton
         // solve's proven bisections (nfd::ff_bisects) bound
         case 18: r = (double)__builtin_amdgcn_logf((float)x[i]); break;

@@ -2155,6 +2155,13 @@ extern "C" int nfo_math_eval(int32_t fn, const double* x, const double* y, doubl
                 r = fn == 16 ? p1 : p2;
                 break;
             }
+            case 24:
+            case 25: {  // the Newton loop's pow(x, y) and log(y) for 0 < y < 1, no range selects
+                double pw, lg;
+                nfm::pow_and_log_ordinary_y01(x[i], y[i], y[i], nfm::Tables{NFM_LOG_TBL, NFM_EXP_TBL}, pw, lg);
+                r = fn == 24 ? pw : lg;
+                break;
+            }
             default: return fail(NF_E_ARGUMENT, "unknown function");
         }
         out[i] = r;

@@ -64,12 +64,14 @@ def rel(a, b):
     (15, -30.0, 3.0, (1e-3, 1e8)),  # and the log
     (16, -30.0, 3.0, (0.05, 1.5)),  # the yield check's two pows side by side
     (17, -30.0, 3.0, (0.05, 1.5)),
+    (24, -30.0, 3.0, (0.05, 0.95)),  # the Newton loop's pair for an exponent in (0, 1): the pow
+    (25, -30.0, 3.0, (0.05, 0.95)),  # and the log
 ])
 def test_device_math_matches_host(hip, oracle, fn, lo, hi, second):
     rng = np.random.default_rng(100 + fn)
     n = 200_000
     x = rng.uniform(lo, hi, n)
-    if fn in (0, 2, 7, 10, 11, 12, 13, 14, 15, 16, 17):
+    if fn in (0, 2, 7, 10, 11, 12, 13, 14, 15, 16, 17, 24, 25):
         x = np.exp(x)
     y = rng.uniform(*second, n) if second else np.zeros(n)
     import ctypes as C

@@ -106,3 +106,8 @@ def test_ordinary_input_variants_are_pow_and_log(oracle):
     assert np.array_equal(_bits(evaluate(b, 15, x, y)), _bits(evaluate(b, 0, y, None)))
     assert np.array_equal(_bits(evaluate(b, 16, x, y)), _bits(evaluate(b, 2, x, y)))
     assert np.array_equal(_bits(evaluate(b, 17, x, y)), _bits(evaluate(b, 2, y, x)))
+    # the Newton loop's pair for an exponent in (0, 1), without exp's range selects (24, 25)
+    inside = (y > 0.0) & (y < 1.0)
+    assert inside.sum() > 1000
+    assert np.array_equal(_bits(evaluate(b, 24, x[inside], y[inside])), _bits(evaluate(b, 2, x[inside], y[inside])))
+    assert np.array_equal(_bits(evaluate(b, 25, x[inside], y[inside])), _bits(evaluate(b, 0, y[inside], None)))
