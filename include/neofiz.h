@@ -440,7 +440,9 @@ int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_
  * bisections proven rather than evaluated (21), with every iteration evaluated (22; the same
  * results as 21), and the number of bisections 21 proved (23); for 21-23, x holds five
  * values per solve (norm, stiffness, plastic strain, thermal factor, dt) and y the five
- * material values A, B, n, C and the reference strain rate. These are the
+ * material values A, B, n, C and the reference strain rate; and the Newton loop's pow(x, y)
+ * (24) and log(y) (25) for an exponent 0 < y < 1, without exp's range selects (the same
+ * results as pow and log). These are the
  * functions the kernels use in place of java.lang.Math's (core/JohnsonCook.java:139, :182,
  * :196; solver/Damage.java:315-322; core/Weibull.java:118, :130; core/DefectField.java:331),
  * evaluated where the kernels run. */

@@ -4138,8 +4138,8 @@ int nf_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps, double* ms_per_
 int nf_math_eval(int32_t fn, const double* x, const double* y, double* out, int32_t n) {
     g_error.clear();
     if (n <= 0) return NF_OK;
-    if (fn < 0 || fn > 23) return fail(NF_E_ARGUMENT, "unknown function");
-    const bool solve = fn >= 21;
+    if (fn < 0 || fn > 25) return fail(NF_E_ARGUMENT, "unknown function");
+    const bool solve = fn >= 21 && fn <= 23;
     if (solve && !y) return fail(NF_E_ARGUMENT, "the solve needs the material values");
     const size_t nx = solve ? 5 * (size_t)n : (size_t)n, ny = solve ? 5 : (size_t)n;
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;
