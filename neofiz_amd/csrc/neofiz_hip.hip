@@ -471,8 +471,8 @@ __device__ void reduce_wall_partials(const Dev& d, const double* __restrict__ pa
 // other kernel and the host read.
 template <bool WALL>
 __global__ void __launch_bounds__(BLOCK) k_gn(Dev d, int q, int phase, int gatherFirst, int part, int minSlot,
-                                              int nodeMajor) {
-    if (d.order && part != PART_REST && blockIdx.x == gridDim.x - 1) {
+                                              int nodeMajor, int schedule) {
+    if (schedule && blockIdx.x == gridDim.x - 1) {
         // the workgroup past the others (launchGNT): the schedule of the element launch that
         // reads Step q ^ 1, from the costs of the one that read Step q
         schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)(q ^ 1) * 4 * d.patchBlocks,
@@ -2110,11 +2110,13 @@ struct nf_solver {
     int launchGNT(int phase, int parity, int gatherFirst, int part, hipEvent_t a, hipEvent_t b) {
         const int minSlot = gnMinSlot, nodeMajor = gnNodeMajor ? 1 : 0;
         // the gather before the contact launches carries the contact's segment rebuild
-        // one workgroup more for the element pass's schedule (k_gn)
-        const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0) +
-                         (d.order && part != PART_REST ? 1 : 0);
-        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part, minSlot, nodeMajor);
-        else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part, minSlot, nodeMajor);
+        // one workgroup more for the element pass's schedule (k_gn), unless the launch closes
+        // a fused run (gnNodeMajor): the next run call opens with k_step, and every schedule
+        // buffer always holds a whole schedule, so none is made there
+        const int schedule = d.order && part != PART_REST && !gnNodeMajor ? 1 : 0;
+        const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0) + schedule;
+        if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part, minSlot, nodeMajor, schedule);
+        else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part, minSlot, nodeMajor, schedule);
         HIP_TRY(hipGetLastError());
         return NF_OK;
     }
