@@ -527,7 +527,15 @@ __device__ __forceinline__ double c_depth(const Dev& d, double2 xn, int p, int a
 
 // nearestPenetrated :570-606 and accumulate :446-462 for one surface node, with resolve
 // :660-762 evaluated into the node's pair record.
-__global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
+// schedule: the last workgroup makes the element pass's schedule instead (as the node pass's
+// extra workgroup does without contact: from the costs of the element launch that read Step
+// q, for the one that reads Step q ^ 1), and the others stride without it.
+__global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo, int schedule) {
+    if (schedule && blockIdx.x == gridDim.x - 1) {
+        schedule_patches(d, d.patchCost + (size_t)q * d.patches, d.patchOf + (size_t)(q ^ 1) * 4 * d.patchBlocks,
+                         d.patchRun ? d.patchRun + (size_t)(q ^ 1) * 4 * d.patchBlocks : nullptr);
+        return;
+    }
     if (c_skip(d, q, checkGo)) return;
     Ctl* c = d.ctl;
     const int nAct = c->cNAct, nSurf = c->cNSurf;
@@ -535,7 +543,7 @@ __global__ void __launch_bounds__(BLOCK) k_c_search(Dev d, int q, int checkGo) {
     const CGrid g = c_grid(d);
     const unsigned fmask = c_capture_size(d, nAct) - 1u;
     const double dt = c->s[q].dt;
-    const int stride = gridDim.x * BLOCK;
+    const int stride = (gridDim.x - schedule) * BLOCK;
     long long escaped = 0;
     // C_SUB consecutive lanes per surface node (stride is a multiple of C_SUB, so a group
     // stays together): lane `sub` walks list `sub` of the node's capture bucket, the group

@@ -373,6 +373,10 @@ __device__ __forceinline__ double2 gather_node_major(const Dev& d, int n, int bu
     return make_double2(sr + m * d.gravityR, sz + m * d.gravityZ);
 }
 
+// The element pass's schedule from the patch costs (defined below; the contact search's extra
+// workgroup makes it when contact is on).
+__device__ void schedule_patches(const Dev& d, const uint8_t* __restrict__ costIn, int* __restrict__ patchOfOut,
+                                 int4* __restrict__ runOut);
 #include "device_contact.h"
 
 constexpr int PHASE_STEP = 0, PHASE_START = 1, PHASE_ASSEMBLE_END = 2, PHASE_RESUME = 3;
@@ -2112,8 +2116,9 @@ struct nf_solver {
         // the gather before the contact launches carries the contact's segment rebuild
         // one workgroup more for the element pass's schedule (k_gn), unless the launch closes
         // a fused run (gnNodeMajor): the next run call opens with k_step, and every schedule
-        // buffer always holds a whole schedule, so none is made there
-        const int schedule = d.order && part != PART_REST && !gnNodeMajor ? 1 : 0;
+        // buffer always holds a whole schedule, so none is made there. With contact the
+        // contact search's extra workgroup makes it (launchContact).
+        const int schedule = d.order && part == PART_ALL && !gnNodeMajor ? 1 : 0;
         const int grid = blocks_for(N) + (part == PART_GATHER ? blocks_for_chunks(4 * E, C_CHUNK_SEG) : 0) + schedule;
         if (a) hipExtLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, a, b, 0, d, parity, phase, gatherFirst, part, minSlot, nodeMajor, schedule);
         else hipLaunchKernelGGL(k_gn<WALL>, dim3(grid), dim3(BLOCK), 0, stream, d, parity, phase, gatherFirst, part, minSlot, nodeMajor, schedule);
@@ -2136,7 +2141,10 @@ struct nf_solver {
         const int nb = blocks_for_chunks(N, C_CHUNK_NODE);
         hipLaunchKernelGGL(k_c_nodes_geometry, dim3(nb + g4), dim3(BLOCK), 0, stream, d, parity, checkGo, nb);
         hipLaunchKernelGGL(k_c_grid, dim3(3 * g4), dim3(BLOCK), 0, stream, d, parity, checkGo);
-        hipLaunchKernelGGL(k_c_search, dim3(gn), dim3(BLOCK), 0, stream, d, parity, checkGo);
+        // one workgroup more for the element pass's schedule, which the gather before these
+        // launches (too short to hide the sort) leaves to the search
+        const int schedule = d.order ? 1 : 0;
+        hipLaunchKernelGGL(k_c_search, dim3(gn + schedule), dim3(BLOCK), 0, stream, d, parity, checkGo, schedule);
         hipLaunchKernelGGL(k_c_apply, dim3(d.cApplyBlocks), dim3(BLOCK), 0, stream, d, parity, checkGo);
         if (b) HIP_TRY(hipEventRecord(b, stream));
         HIP_TRY(hipGetLastError());
